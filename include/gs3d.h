@@ -1,0 +1,240 @@
+/*
+ * gs3d.h — C ABI of the MI355X-native per-frame splat pipeline
+ * (libgs3d.so, built from wgpu-3dgs-viewer_amd/csrc).
+ *
+ * Drop-in boundary for wgpu-3dgs-viewer 0.6.1's Viewer / Preprocessor /
+ * RadixSorter / Renderer (Rust, src/lib.rs, src/preprocessor.rs,
+ * src/radix_sorter.rs, src/renderer.rs). Each entry point names the
+ * reference interface it replaces. Plain pointers and sizes only; every call
+ * returns a gs3d_status and gs3d_last_error() describes the last failure on
+ * the calling thread.
+ *
+ * Execution model: a gs3d_device owns one HIP stream (the reference's
+ * wgpu::Queue). Recording calls (render, preprocess, sort, draw) only enqueue
+ * work, as the reference's calls only record into a CommandEncoder;
+ * gs3d_device_poll() waits for it (queue.submit + device.poll(wait)). A
+ * device, and everything created from it, must be used by one thread at a
+ * time.
+ */
+#ifndef GS3D_H
+#define GS3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status (src/error.rs:7-50) --------------------------------------- */
+typedef enum gs3d_status {
+    GS3D_OK = 0,
+    GS3D_ERROR_INVALID_ARGUMENT = 1,
+    /* RendererCreateError / PreprocessorCreateError::ModelSizeExceedsDeviceLimit
+     * (src/error.rs:7-33; checks at src/preprocessor.rs:239-246,
+     * src/renderer.rs:130-135) */
+    GS3D_ERROR_MODEL_SIZE_EXCEEDS_DEVICE_LIMIT = 2,
+    GS3D_ERROR_OUT_OF_MEMORY = 3,
+    GS3D_ERROR_DEVICE = 4,
+    /* MultiModelViewerAccessError::ModelNotFound (src/error.rs:47-50) */
+    GS3D_ERROR_MODEL_NOT_FOUND = 5,
+    /* a kernel reported an internal error (look-back wait limit) */
+    GS3D_ERROR_KERNEL = 6
+} gs3d_status;
+
+/* ---- records ------------------------------------------------------------ */
+
+/* wgpu_3dgs_core::Gaussian (used at tests/e2e/viewer.rs:42-48): rotation
+ * quaternion (x, y, z, w), position, RGBA8 colour (rgb = 0.5 + SH_C0 * dc,
+ * a = opacity), 15 SH coefficients per channel (degree 1..3), scale. */
+typedef struct gs3d_gaussian {
+    float rot[4];
+    float pos[3];
+    uint8_t color[4];
+    float sh[15][3];
+    float scale[3];
+} gs3d_gaussian;
+
+/* CameraPod (src/buffer/camera.rs:61-79): column-major view and projection,
+ * target size in pixels; 144 bytes. */
+typedef struct gs3d_camera_pod {
+    float view[16];
+    float proj[16];
+    float size[2];
+    uint32_t padding[2];
+} gs3d_camera_pod;
+
+/* Camera (src/camera.rs:18-29). */
+typedef struct gs3d_camera {
+    float pos[3];
+    float z_start;
+    float z_end;
+    float vertical_fov;
+    float pitch;
+    float yaw;
+} gs3d_camera;
+
+/* GaussianPod choice (src/lib.rs:44). */
+typedef enum gs3d_gaussian_pod {
+    /* SH and cov3d in f32 (the reference's DefaultGaussianPod) */
+    GS3D_POD_SINGLE = 0,
+    /* SH, rotation and scale in f16, opacity u8; cov3d built on the GPU */
+    GS3D_POD_HALF = 1
+} gs3d_gaussian_pod;
+
+/* wgpu::TextureFormat of the render target (src/renderer.rs:120-125). */
+typedef enum gs3d_texture_format {
+    GS3D_FORMAT_RGBA8_UNORM = 0,
+    GS3D_FORMAT_RGBA32_FLOAT = 1
+} gs3d_texture_format;
+
+/* GaussianDisplayMode (render.wesl:12-14). */
+typedef enum gs3d_display_mode {
+    GS3D_DISPLAY_SPLAT = 0,
+    GS3D_DISPLAY_ELLIPSE = 1,
+    GS3D_DISPLAY_POINT = 2
+} gs3d_display_mode;
+
+/* How the compositor blends (the reference uses fixed-function
+ * ALPHA_BLENDING, src/renderer.rs:296-300). FAST: front to back in f32 with
+ * early termination. EXACT: back to front, destination rounded to the target
+ * format after every splat, as the blender does. */
+typedef enum gs3d_blend_mode {
+    GS3D_BLEND_FAST = 0,
+    GS3D_BLEND_EXACT = 1
+} gs3d_blend_mode;
+
+/* Per-stage GPU times of the last rendered frame (gs3d_viewer_stage_times). */
+typedef enum gs3d_stage {
+    GS3D_STAGE_PREPROCESS = 0,
+    GS3D_STAGE_SORT = 1,
+    GS3D_STAGE_BIN = 2,
+    GS3D_STAGE_TILE_SORT = 3,
+    GS3D_STAGE_COMPOSITE = 4,
+    GS3D_STAGE_COUNT = 5
+} gs3d_stage;
+
+typedef struct gs3d_device gs3d_device;
+typedef struct gs3d_texture gs3d_texture;
+typedef struct gs3d_viewer gs3d_viewer;
+typedef struct gs3d_radix_sorter gs3d_radix_sorter;
+
+/* ---- errors ------------------------------------------------------------- */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* gs3d_last_error(void);
+
+/* ---- device and queue (wgpu::Device + wgpu::Queue) ---------------------- */
+
+gs3d_status gs3d_device_create(int hip_device, gs3d_device** out);
+void gs3d_device_destroy(gs3d_device* dev);
+/* queue.submit + device.poll(wait) (tests/e2e/viewer.rs:31-34): waits for all
+ * enqueued work. A frame whose tile lists overflowed is re-rendered with
+ * larger lists before this returns. */
+gs3d_status gs3d_device_poll(gs3d_device* dev);
+/* The largest single buffer a model may need, the counterpart of
+ * max_storage_buffer_binding_size (src/preprocessor.rs:239). Defaults to the
+ * device's HBM size. */
+gs3d_status gs3d_device_set_buffer_limit(gs3d_device* dev, uint64_t bytes);
+gs3d_status gs3d_device_compute_units(gs3d_device* dev, int* out);
+
+/* ---- render targets (wgpu::Texture / TextureView) ----------------------- */
+
+gs3d_status gs3d_texture_create(gs3d_device* dev, uint32_t width, uint32_t height, gs3d_texture_format format,
+                                gs3d_texture** out);
+void gs3d_texture_destroy(gs3d_texture* tex);
+/* Copies the texture to host memory (row-major, top row first, 4 channels);
+ * waits for the queue first. */
+gs3d_status gs3d_texture_read(gs3d_texture* tex, void* dst, size_t bytes);
+
+/* ---- Viewer (src/lib.rs:65-276) ------------------------------------------ */
+
+/* Viewer::new (src/lib.rs:86-97): uploads the Gaussians in the chosen pod
+ * and creates every buffer, the preprocessor, the sorter and the renderer. */
+gs3d_status gs3d_viewer_new(gs3d_device* dev, gs3d_texture_format format, const gs3d_gaussian* gaussians,
+                            uint32_t count, gs3d_gaussian_pod pod, gs3d_viewer** out);
+void gs3d_viewer_destroy(gs3d_viewer* v);
+/* Viewer::update_camera (src/lib.rs:202-209). */
+gs3d_status gs3d_viewer_update_camera(gs3d_viewer* v, const gs3d_camera* camera, uint32_t width, uint32_t height);
+/* Viewer::update_camera_with_pod (src/lib.rs:212-214). */
+gs3d_status gs3d_viewer_update_camera_with_pod(gs3d_viewer* v, const gs3d_camera_pod* pod);
+/* Viewer::update_model_transform (src/lib.rs:217-225). */
+gs3d_status gs3d_viewer_update_model_transform(gs3d_viewer* v, const float pos[3], const float rot[4],
+                                               const float scale[3]);
+/* Viewer::update_gaussian_transform (src/lib.rs:237-254). */
+gs3d_status gs3d_viewer_update_gaussian_transform(gs3d_viewer* v, float size, gs3d_display_mode display_mode,
+                                                  uint32_t sh_deg, int no_sh0, float max_std_dev);
+/* Viewer::render (src/lib.rs:266-275): preprocess -> sort -> render. */
+gs3d_status gs3d_viewer_render(gs3d_viewer* v, gs3d_texture* target);
+/* Preprocessor::preprocess (src/preprocessor.rs:283-290). */
+gs3d_status gs3d_viewer_preprocess(gs3d_viewer* v);
+/* RadixSorter::sort (src/radix_sorter.rs:56-66). */
+gs3d_status gs3d_viewer_sort(gs3d_viewer* v);
+/* Renderer::render (src/renderer.rs:163-184), using the splats the fused
+ * preprocessor computed. */
+gs3d_status gs3d_viewer_draw(gs3d_viewer* v, gs3d_texture* target);
+/* Renderer::render driven as a standalone operator: recomputes every drawn
+ * instance's splat from the Gaussians and the sorted indices (the vertex
+ * stage, render.wesl:75-130) before binning and compositing. */
+gs3d_status gs3d_viewer_draw_standalone(gs3d_viewer* v, gs3d_texture* target);
+
+gs3d_status gs3d_viewer_set_blend_mode(gs3d_viewer* v, gs3d_blend_mode mode);
+/* Compositing tile size: 8x8, 16x16 (default) or 32x8 pixels. */
+gs3d_status gs3d_viewer_set_tile_size(gs3d_viewer* v, uint32_t tile_w, uint32_t tile_h);
+/* Record HIP events around each stage of every render (off by default). */
+gs3d_status gs3d_viewer_set_profiling(gs3d_viewer* v, int enabled);
+/* Milliseconds of each gs3d_stage in the last profiled frame; waits. */
+gs3d_status gs3d_viewer_stage_times(gs3d_viewer* v, float ms[GS3D_STAGE_COUNT]);
+
+/* Buffer downloads (the Viewer's pub buffers, src/lib.rs:66-73); each waits
+ * for the queue first. */
+gs3d_status gs3d_viewer_gaussian_count(gs3d_viewer* v, uint32_t* out);
+/* IndirectArgsBuffer {vertex_count 6, instance_count, 0, 0}
+ * (src/buffer/indirect_args.rs:12-26). */
+gs3d_status gs3d_viewer_read_indirect_args(gs3d_viewer* v, uint32_t out[4]);
+/* RadixSortIndirectArgsBuffer {x, y, z} (src/buffer/indirect_args.rs:63-71). */
+gs3d_status gs3d_viewer_read_radix_sort_indirect_args(gs3d_viewer* v, uint32_t out[3]);
+/* IndirectIndicesBuffer, first `count` entries. */
+gs3d_status gs3d_viewer_read_indirect_indices(gs3d_viewer* v, uint32_t* out, uint32_t count);
+/* GaussiansDepthBuffer as f32 keys, first `count` entries (the buffer holds
+ * gs3d_depth_buffer_len(N) keys). */
+gs3d_status gs3d_viewer_read_gaussians_depth(gs3d_viewer* v, float* out, uint32_t count);
+/* Number of keys in a GaussiansDepthBuffer for N Gaussians: N rounded up to
+ * the reference's sort block of 3840 (src/radix_sorter.rs:922-939). */
+uint32_t gs3d_depth_buffer_len(uint32_t gaussian_count);
+
+/* ---- RadixSorter as a standalone operator (src/radix_sorter.rs:9-96) ---- */
+
+/* RadixSorter::new with its own key/payload buffers of `capacity` entries. */
+gs3d_status gs3d_radix_sorter_new(gs3d_device* dev, uint32_t capacity, gs3d_radix_sorter** out);
+void gs3d_radix_sorter_destroy(gs3d_radix_sorter* s);
+/* Uploads n (key, value) pairs. */
+gs3d_status gs3d_radix_sorter_write(gs3d_radix_sorter* s, const uint32_t* keys, const uint32_t* vals, uint32_t n);
+/* RadixSorter::sort: stable ascending sort of the n pairs by key. */
+gs3d_status gs3d_radix_sorter_sort(gs3d_radix_sorter* s);
+gs3d_status gs3d_radix_sorter_read(gs3d_radix_sorter* s, uint32_t* keys, uint32_t* vals, uint32_t n);
+
+/* ---- host helpers -------------------------------------------------------- */
+
+/* CameraPod::new (src/buffer/camera.rs:72-79) from a Camera. */
+gs3d_status gs3d_camera_pod_new(const gs3d_camera* camera, uint32_t width, uint32_t height, gs3d_camera_pod* out);
+/* A camera pod looking from `eye` at `target` (+Y up). */
+gs3d_status gs3d_camera_pod_look_at(const float eye[3], const float target[3], float vertical_fov, float z_near,
+                                    float z_far, uint32_t width, uint32_t height, gs3d_camera_pod* out);
+/* cov3d of a Gaussian as the f32 pod stores it: (R S)(R S)^T as
+ * xx, xy, xz, yy, yz, zz. */
+gs3d_status gs3d_gaussian_cov3d(const gs3d_gaussian* g, float out[6]);
+/* The f16 pod's rotation/scale (8 halves per Gaussian: rot xyzw, scale xyz,
+ * 0) and SH (48 halves: 45 coefficients, 3 zeros) as uploaded. */
+gs3d_status gs3d_pack_half(const gs3d_gaussian* g, uint32_t count, uint16_t* rot_scale, uint16_t* sh);
+
+/* Synthetic scenes. config 0: means U[-1,1]^3; 1: means U[-4,4]^3 (both
+ * log-scale N(-4,0.5)); 2: 64 anisotropic clusters + 20% on the plane
+ * y = -2, log-scale N(-5,0.7). Counter-based RNG: record i depends only on
+ * (seed, i). */
+gs3d_status gs3d_synth_gaussians(int config, uint64_t seed, uint32_t first, uint32_t count, gs3d_gaussian* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GS3D_H */

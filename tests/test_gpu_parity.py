@@ -1,0 +1,181 @@
+"""HIP path vs the CPU oracle through the C ABI (GPU tests).
+
+Bar: visible set, indirect counts, f32 depth-key bits and the sorted index
+permutation bit-exact; images within 2/255 per channel and >= 45 dB PSNR of
+the oracle's render (fast blend), within 1/255 in the exact unorm8 blend.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from helpers import CONFIG0, CONFIG1, camera_pod, max_abs_diff, oracle_inputs, oracle_inputs_half, psnr
+
+pytestmark = pytest.mark.gpu
+
+HISTO_BLOCK_KVS = 3840
+
+
+def _case(name):
+    print(f"[case] {name}", flush=True)
+
+
+def _frame(gs, orc, config, n, seed, w, h, azimuth=0.0, **kw):
+    g = gs.synth_gaussians(config, n, seed)
+    pod = camera_pod(gs, config, w, h, azimuth)
+    return g, pod, orc.frame_from_pod(pod.to_bytes(), **kw)
+
+
+@pytest.fixture(scope="module")
+def scene0(gs, orc, device):
+    config, n, seed, w, h = CONFIG0
+    g, pod, frame = _frame(gs, orc, config, n, seed, w, h)
+    viewer = gs.Viewer(device, gs.TextureFormat.RGBA8_UNORM, g)
+    viewer.update_camera_with_pod(pod)
+    inputs = oracle_inputs(orc, g)
+    return g, pod, frame, viewer, inputs, (w, h)
+
+
+def test_preprocess_config0_bit_exact(gs, orc, device, scene0):
+    _case("preprocess config0 65536 1280x720")
+    g, pod, frame, viewer, (pos, color, sh, cov), (w, h) = scene0
+    viewer.preprocess()
+    device.poll()
+    oidx, okey = orc.preprocess(frame, pos, cov)
+    v = len(oidx)
+    args = viewer.indirect_args()
+    assert args.tolist() == [6, v, 0, 0]  # preprocess.wesl:101 + buffer init {6,0,0,0}
+    assert viewer.radix_sort_indirect_args().tolist() == [math.ceil(v / HISTO_BLOCK_KVS), 1, 1]
+    idx = viewer.indirect_indices(v)
+    assert np.array_equal(idx, oidx)
+    depth = viewer.gaussians_depth()
+    assert np.array_equal(depth[:v].view(np.uint32), okey.view(np.uint32))
+    pad_end = min(math.ceil(v / HISTO_BLOCK_KVS) * HISTO_BLOCK_KVS, depth.shape[0])
+    assert np.all(depth[v:pad_end].view(np.uint32) == 0x40000000)  # preprocess.wesl:118-125
+    assert 0 < v <= g.shape[0]
+
+
+def test_sort_config0_bit_exact(gs, orc, device, scene0):
+    _case("sort config0")
+    g, pod, frame, viewer, (pos, color, sh, cov), (w, h) = scene0
+    viewer.preprocess()
+    viewer.sort()
+    device.poll()
+    oidx, okey = orc.preprocess(frame, pos, cov)
+    skey, sidx = orc.stable_sort(okey.view(np.uint32), oidx)
+    v = len(oidx)
+    assert np.array_equal(viewer.indirect_indices(v), sidx)
+    depth = viewer.gaussians_depth()
+    assert np.array_equal(depth[:v].view(np.uint32), skey)
+    # keys past N are the sorter's 0xFFFFFFFF padding (radix_sort.wgsl:92-110)
+    assert np.all(depth[g.shape[0]:].view(np.uint32) == 0xFFFFFFFF)
+
+
+@pytest.mark.parametrize("blend", ["fast", "exact"])
+def test_render_config0(gs, orc, device, scene0, blend):
+    _case(f"render config0 blend={blend}")
+    g, pod, frame, viewer, (pos, color, sh, cov), (w, h) = scene0
+    viewer.set_blend_mode(gs.BlendMode.FAST if blend == "fast" else gs.BlendMode.EXACT)
+    target = gs.Texture(device, w, h, gs.TextureFormat.RGBA8_UNORM)
+    viewer.render(target)
+    device.poll()
+    img = target.read()
+    viewer.set_blend_mode(gs.BlendMode.FAST)
+    tgt = orc.TARGET_RGBA8_F32 if blend == "fast" else orc.TARGET_RGBA8_STEP
+    oidx, okey, oimg = orc.full_frame(frame, pos, color, sh, cov, w, h, tgt, threads=8)
+    assert np.array_equal(viewer.indirect_indices(len(oidx)), oidx)
+    d = max_abs_diff(img, oimg)
+    p = psnr(img, oimg)
+    print(f"config0 {blend}: max diff {d}/255, PSNR {p:.2f} dB", flush=True)
+    assert d <= (2 if blend == "fast" else 1)
+    assert p >= 45.0
+    assert np.all(img[..., 3] == 255)  # clear alpha 1 stays 1 under ALPHA_BLENDING
+
+
+def test_render_config0_rgba32f(gs, orc, device):
+    _case("render config0 rgba32f")
+    config, n, seed, w, h = CONFIG0
+    g, pod, frame = _frame(gs, orc, config, n, seed, w, h)
+    viewer = gs.Viewer(device, gs.TextureFormat.RGBA32_FLOAT, g)
+    viewer.update_camera_with_pod(pod)
+    target = gs.Texture(device, w, h, gs.TextureFormat.RGBA32_FLOAT)
+    for blend in (gs.BlendMode.FAST, gs.BlendMode.EXACT):
+        viewer.set_blend_mode(blend)
+        viewer.render(target)
+        device.poll()
+        img = target.read()
+        pos, color, sh, cov = oracle_inputs(orc, g)
+        _, _, oimg = orc.full_frame(frame, pos, color, sh, cov, w, h, orc.TARGET_RGBA32F, threads=8)
+        d = max_abs_diff(img[..., :3], oimg[..., :3])
+        print(f"rgba32f blend={int(blend)}: max diff {d:.3e}", flush=True)
+        assert d <= 1e-3
+
+
+def test_standalone_renderer_matches_fused(gs, orc, device, scene0):
+    _case("standalone renderer vs fused")
+    g, pod, frame, viewer, _, (w, h) = scene0
+    a = gs.Texture(device, w, h, gs.TextureFormat.RGBA8_UNORM)
+    b = gs.Texture(device, w, h, gs.TextureFormat.RGBA8_UNORM)
+    viewer.render(a)
+    viewer.draw(b, standalone=True)
+    device.poll()
+    assert np.array_equal(a.read(), b.read())
+
+
+@pytest.mark.parametrize("tile", [(8, 8), (32, 8)])
+def test_tile_sizes_identical(gs, device, scene0, tile):
+    _case(f"tile size {tile}")
+    g, pod, frame, viewer, _, (w, h) = scene0
+    viewer.set_blend_mode(gs.BlendMode.EXACT)
+    ref = gs.Texture(device, w, h, gs.TextureFormat.RGBA8_UNORM)
+    viewer.set_tile_size(16, 16)
+    viewer.render(ref)
+    other = gs.Texture(device, w, h, gs.TextureFormat.RGBA8_UNORM)
+    viewer.set_tile_size(*tile)
+    viewer.render(other)
+    device.poll()
+    viewer.set_tile_size(16, 16)
+    viewer.set_blend_mode(gs.BlendMode.FAST)
+    assert np.array_equal(ref.read(), other.read())
+
+
+def test_half_pod_config0_bit_exact(gs, orc, device):
+    _case("half pod config0")
+    config, n, seed, w, h = CONFIG0
+    g, pod, frame = _frame(gs, orc, config, n, seed, w, h)
+    viewer = gs.Viewer(device, gs.TextureFormat.RGBA8_UNORM, g, gs.GaussianPod.HALF)
+    viewer.update_camera_with_pod(pod)
+    target = gs.Texture(device, w, h, gs.TextureFormat.RGBA8_UNORM)
+    viewer.render(target)
+    device.poll()
+    pos, color, sh, cov = oracle_inputs_half(orc, g)
+    oidx, okey, oimg = orc.full_frame(frame, pos, color, sh, cov, w, h, orc.TARGET_RGBA8_F32, threads=8)
+    v = int(viewer.indirect_args()[1])
+    assert v == len(oidx)
+    assert np.array_equal(viewer.indirect_indices(v), oidx)
+    assert np.array_equal(viewer.gaussians_depth(v).view(np.uint32), okey.view(np.uint32))
+    d = max_abs_diff(target.read(), oimg)
+    assert d <= 2
+
+
+def test_config1_full_frame(gs, orc, device):
+    config, n, seed, w, h = CONFIG1
+    _case(f"config1 {n} {w}x{h}")
+    g, pod, frame = _frame(gs, orc, config, n, seed, w, h)
+    viewer = gs.Viewer(device, gs.TextureFormat.RGBA8_UNORM, g)
+    viewer.update_camera_with_pod(pod)
+    target = gs.Texture(device, w, h, gs.TextureFormat.RGBA8_UNORM)
+    viewer.render(target)
+    device.poll()
+    pos, color, sh, cov = oracle_inputs(orc, g)
+    oidx, okey, oimg = orc.full_frame(frame, pos, color, sh, cov, w, h, orc.TARGET_RGBA8_F32, threads=16)
+    v = int(viewer.indirect_args()[1])
+    assert v == len(oidx)
+    assert np.array_equal(viewer.indirect_indices(v), oidx)
+    assert np.array_equal(viewer.gaussians_depth(v).view(np.uint32), okey.view(np.uint32))
+    img = target.read()
+    d, p = max_abs_diff(img, oimg), psnr(img, oimg)
+    print(f"config1: V={v} max diff {d}/255 PSNR {p:.2f} dB", flush=True)
+    assert d <= 2 and p >= 45.0

@@ -1,0 +1,488 @@
+"""MI355X-native per-frame splat pipeline (Python mirror of the C ABI).
+
+The compute path is libgs3d.so (HIP kernels for gfx950 + C++ host library);
+this module binds its C ABI (include/gs3d.h) with ctypes and mirrors the
+reference's Rust API shape (wgpu-3dgs-viewer 0.6.1, src/lib.rs):
+
+    dev = Device()
+    viewer = Viewer(dev, TextureFormat.RGBA8_UNORM, gaussians)
+    viewer.update_camera_with_pod(pod)          # or update_camera(camera, (w, h))
+    target = Texture(dev, w, h, TextureFormat.RGBA8_UNORM)
+    viewer.render(target)
+    dev.poll()
+    image = target.read()
+
+There is no CPU fallback: every call goes to libgs3d.so, and a missing or
+unloadable library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgs3d.so")
+
+# gs3d_gaussian (wgpu_3dgs_core::Gaussian): 224 bytes
+GAUSSIAN_DTYPE = np.dtype([("rot", "<f4", (4,)), ("pos", "<f4", (3,)), ("color", "u1", (4,)),
+                           ("sh", "<f4", (15, 3)), ("scale", "<f4", (3,))])
+assert GAUSSIAN_DTYPE.itemsize == 224
+
+STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "MODEL_SIZE_EXCEEDS_DEVICE_LIMIT", 3: "OUT_OF_MEMORY",
+                4: "DEVICE", 5: "MODEL_NOT_FOUND", 6: "KERNEL"}
+
+
+class Gs3dError(RuntimeError):
+    """A gs3d_status other than GS3D_OK (the reference's error enums, src/error.rs)."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class ModelSizeExceedsDeviceLimit(Gs3dError):
+    """PreprocessorCreateError / RendererCreateError::ModelSizeExceedsDeviceLimit."""
+
+
+class ModelNotFound(Gs3dError):
+    """MultiModelViewerAccessError::ModelNotFound."""
+
+
+class TextureFormat(enum.IntEnum):
+    RGBA8_UNORM = 0
+    RGBA32_FLOAT = 1
+
+
+class GaussianPod(enum.IntEnum):
+    SINGLE = 0  # f32 SH + f32 cov3d (the reference's DefaultGaussianPod)
+    HALF = 1    # f16 SH + f16 rotation/scale, u8 opacity
+
+
+class DisplayMode(enum.IntEnum):
+    SPLAT = 0
+    ELLIPSE = 1
+    POINT = 2
+
+
+class BlendMode(enum.IntEnum):
+    FAST = 0
+    EXACT = 1
+
+
+class Stage(enum.IntEnum):
+    PREPROCESS = 0
+    SORT = 1
+    BIN = 2
+    TILE_SORT = 3
+    COMPOSITE = 4
+
+
+class CameraPod(C.Structure):
+    """CameraPod (src/buffer/camera.rs:61-79), 144 bytes."""
+
+    _fields_ = [("view", C.c_float * 16), ("proj", C.c_float * 16), ("size", C.c_float * 2),
+                ("padding", C.c_uint32 * 2)]
+
+    def to_bytes(self) -> bytes:
+        return bytes(C.string_at(C.addressof(self), C.sizeof(self)))
+
+    @classmethod
+    def from_bytes(cls, b: bytes) -> "CameraPod":
+        if len(b) != 144:
+            raise ValueError("a CameraPod is 144 bytes")
+        return cls.from_buffer_copy(b)
+
+    @classmethod
+    def new(cls, camera: "Camera", size) -> "CameraPod":
+        """CameraPod::new (src/buffer/camera.rs:72-79)."""
+        out = cls()
+        _check(lib().gs3d_camera_pod_new(C.byref(camera._c()), int(size[0]), int(size[1]), C.byref(out)))
+        return out
+
+    @classmethod
+    def look_at(cls, eye, target, size, vertical_fov=np.pi / 3, z_near=0.1, z_far=1e4) -> "CameraPod":
+        out = cls()
+        e = (C.c_float * 3)(*[float(x) for x in eye])
+        t = (C.c_float * 3)(*[float(x) for x in target])
+        _check(lib().gs3d_camera_pod_look_at(e, t, float(vertical_fov), float(z_near), float(z_far), int(size[0]),
+                                             int(size[1]), C.byref(out)))
+        return out
+
+
+class _CCamera(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("z_start", C.c_float), ("z_end", C.c_float),
+                ("vertical_fov", C.c_float), ("pitch", C.c_float), ("yaw", C.c_float)]
+
+
+class Camera:
+    """Camera (src/camera.rs:18-93): position, z range, vertical fov, pitch, yaw."""
+
+    def __init__(self, z=(0.1, 1e4), vertical_fov=np.pi / 3, pos=(0.0, 0.0, 0.0), pitch=0.0, yaw=0.0):
+        self.pos = [float(x) for x in pos]
+        self.z = (float(z[0]), float(z[1]))
+        self.vertical_fov = float(vertical_fov)
+        self.pitch = float(pitch)
+        self.yaw = float(yaw)
+
+    def _c(self) -> _CCamera:
+        c = _CCamera()
+        c.pos[:] = self.pos
+        c.z_start, c.z_end = self.z
+        c.vertical_fov, c.pitch, c.yaw = self.vertical_fov, self.pitch, self.yaw
+        return c
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libgs3d.so (built by build.py). Raises if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L):
+    vp, u32, i32, f32, u64 = C.c_void_p, C.c_uint32, C.c_int, C.c_float, C.c_uint64
+    pp = C.POINTER(C.c_void_p)
+    sigs = {
+        "gs3d_last_error": ([], C.c_char_p),
+        "gs3d_device_create": ([i32, pp], i32),
+        "gs3d_device_destroy": ([vp], None),
+        "gs3d_device_poll": ([vp], i32),
+        "gs3d_device_set_buffer_limit": ([vp, u64], i32),
+        "gs3d_device_compute_units": ([vp, C.POINTER(C.c_int)], i32),
+        "gs3d_texture_create": ([vp, u32, u32, i32, pp], i32),
+        "gs3d_texture_destroy": ([vp], None),
+        "gs3d_texture_read": ([vp, vp, C.c_size_t], i32),
+        "gs3d_viewer_new": ([vp, i32, vp, u32, i32, pp], i32),
+        "gs3d_viewer_destroy": ([vp], None),
+        "gs3d_viewer_update_camera": ([vp, vp, u32, u32], i32),
+        "gs3d_viewer_update_camera_with_pod": ([vp, vp], i32),
+        "gs3d_viewer_update_model_transform": ([vp, vp, vp, vp], i32),
+        "gs3d_viewer_update_gaussian_transform": ([vp, f32, i32, u32, i32, f32], i32),
+        "gs3d_viewer_render": ([vp, vp], i32),
+        "gs3d_viewer_preprocess": ([vp], i32),
+        "gs3d_viewer_sort": ([vp], i32),
+        "gs3d_viewer_draw": ([vp, vp], i32),
+        "gs3d_viewer_draw_standalone": ([vp, vp], i32),
+        "gs3d_viewer_set_blend_mode": ([vp, i32], i32),
+        "gs3d_viewer_set_tile_size": ([vp, u32, u32], i32),
+        "gs3d_viewer_set_profiling": ([vp, i32], i32),
+        "gs3d_viewer_stage_times": ([vp, vp], i32),
+        "gs3d_viewer_gaussian_count": ([vp, vp], i32),
+        "gs3d_viewer_read_indirect_args": ([vp, vp], i32),
+        "gs3d_viewer_read_radix_sort_indirect_args": ([vp, vp], i32),
+        "gs3d_viewer_read_indirect_indices": ([vp, vp, u32], i32),
+        "gs3d_viewer_read_gaussians_depth": ([vp, vp, u32], i32),
+        "gs3d_depth_buffer_len": ([u32], u32),
+        "gs3d_radix_sorter_new": ([vp, u32, pp], i32),
+        "gs3d_radix_sorter_destroy": ([vp], None),
+        "gs3d_radix_sorter_write": ([vp, vp, vp, u32], i32),
+        "gs3d_radix_sorter_sort": ([vp], i32),
+        "gs3d_radix_sorter_read": ([vp, vp, vp, u32], i32),
+        "gs3d_camera_pod_new": ([vp, u32, u32, vp], i32),
+        "gs3d_camera_pod_look_at": ([vp, vp, f32, f32, f32, u32, u32, vp], i32),
+        "gs3d_gaussian_cov3d": ([vp, vp], i32),
+        "gs3d_pack_half": ([vp, u32, vp, vp], i32),
+        "gs3d_synth_gaussians": ([i32, u64, u32, u32, vp], i32),
+    }
+    for name, (args, res) in sigs.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+
+
+EXPORTED_SYMBOLS = (
+    "gs3d_last_error", "gs3d_device_create", "gs3d_device_destroy", "gs3d_device_poll",
+    "gs3d_device_set_buffer_limit", "gs3d_device_compute_units", "gs3d_texture_create", "gs3d_texture_destroy",
+    "gs3d_texture_read", "gs3d_viewer_new", "gs3d_viewer_destroy", "gs3d_viewer_update_camera",
+    "gs3d_viewer_update_camera_with_pod", "gs3d_viewer_update_model_transform",
+    "gs3d_viewer_update_gaussian_transform", "gs3d_viewer_render", "gs3d_viewer_preprocess", "gs3d_viewer_sort",
+    "gs3d_viewer_draw", "gs3d_viewer_draw_standalone", "gs3d_viewer_set_blend_mode", "gs3d_viewer_set_tile_size",
+    "gs3d_viewer_set_profiling", "gs3d_viewer_stage_times", "gs3d_viewer_gaussian_count",
+    "gs3d_viewer_read_indirect_args", "gs3d_viewer_read_radix_sort_indirect_args",
+    "gs3d_viewer_read_indirect_indices", "gs3d_viewer_read_gaussians_depth", "gs3d_depth_buffer_len",
+    "gs3d_radix_sorter_new", "gs3d_radix_sorter_destroy", "gs3d_radix_sorter_write", "gs3d_radix_sorter_sort",
+    "gs3d_radix_sorter_read", "gs3d_camera_pod_new", "gs3d_camera_pod_look_at", "gs3d_gaussian_cov3d",
+    "gs3d_pack_half", "gs3d_synth_gaussians",
+)
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        msg = lib().gs3d_last_error().decode(errors="replace")
+        if status == 2:
+            raise ModelSizeExceedsDeviceLimit(status, msg)
+        if status == 5:
+            raise ModelNotFound(status, msg)
+        raise Gs3dError(status, msg)
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+# ---------------------------------------------------------------------------
+# host helpers
+# ---------------------------------------------------------------------------
+
+def synth_gaussians(config: int, n: int, seed: int, first: int = 0) -> np.ndarray:
+    """Synthetic scene records (see gs3d_synth_gaussians)."""
+    out = np.zeros(n, GAUSSIAN_DTYPE)
+    if n:
+        _check(lib().gs3d_synth_gaussians(int(config), int(seed), int(first), int(n), _ptr(out)))
+    return out
+
+
+def gaussians(pos, rot, scale, color, sh=None) -> np.ndarray:
+    """Build gs3d_gaussian records from arrays."""
+    pos = np.asarray(pos, np.float32).reshape(-1, 3)
+    n = pos.shape[0]
+    g = np.zeros(n, GAUSSIAN_DTYPE)
+    g["pos"] = pos
+    g["rot"] = np.asarray(rot, np.float32).reshape(-1, 4)
+    g["scale"] = np.asarray(scale, np.float32).reshape(-1, 3)
+    g["color"] = np.asarray(color, np.uint8).reshape(-1, 4)
+    if sh is not None:
+        g["sh"] = np.asarray(sh, np.float32).reshape(-1, 15, 3)
+    return g
+
+
+def cov3d(g: np.ndarray) -> np.ndarray:
+    """cov3d of each record as the f32 pod stores it."""
+    g = np.ascontiguousarray(g, GAUSSIAN_DTYPE)
+    out = np.zeros((g.shape[0], 6), np.float32)
+    for i in range(g.shape[0]):
+        _check(lib().gs3d_gaussian_cov3d(_ptr(g[i:i + 1]), _ptr(out[i])))
+    return out
+
+
+def pack_half(g: np.ndarray):
+    """(rot_scale f16 (n,8), sh f16 (n,48)) as the f16 pod uploads them."""
+    g = np.ascontiguousarray(g, GAUSSIAN_DTYPE)
+    n = g.shape[0]
+    rs = np.zeros((n, 8), np.uint16)
+    sh = np.zeros((n, 48), np.uint16)
+    if n:
+        _check(lib().gs3d_pack_half(_ptr(g), n, _ptr(rs), _ptr(sh)))
+    return rs.view(np.float16), sh.view(np.float16)
+
+
+def depth_buffer_len(n: int) -> int:
+    return int(lib().gs3d_depth_buffer_len(int(n)))
+
+
+# ---------------------------------------------------------------------------
+# device objects
+# ---------------------------------------------------------------------------
+
+class Device:
+    """wgpu::Device + wgpu::Queue: one HIP device and its stream."""
+
+    def __init__(self, hip_device: int = 0):
+        h = C.c_void_p()
+        _check(lib().gs3d_device_create(int(hip_device), C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def poll(self) -> None:
+        """queue.submit + device.poll(wait)."""
+        _check(lib().gs3d_device_poll(self._h))
+
+    def set_buffer_limit(self, nbytes: int) -> None:
+        _check(lib().gs3d_device_set_buffer_limit(self._h, int(nbytes)))
+
+    def compute_units(self) -> int:
+        n = C.c_int()
+        _check(lib().gs3d_device_compute_units(self._h, C.byref(n)))
+        return n.value
+
+    def close(self) -> None:
+        if self._h:
+            lib().gs3d_device_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Texture:
+    """Render target (wgpu::Texture)."""
+
+    def __init__(self, device: Device, width: int, height: int, fmt: TextureFormat = TextureFormat.RGBA8_UNORM):
+        h = C.c_void_p()
+        _check(lib().gs3d_texture_create(device.handle, int(width), int(height), int(fmt), C.byref(h)))
+        self._h, self.device, self.width, self.height, self.format = h, device, int(width), int(height), fmt
+
+    @property
+    def handle(self):
+        return self._h
+
+    def read(self) -> np.ndarray:
+        """(height, width, 4) uint8 or float32, top row first."""
+        dt = np.uint8 if self.format == TextureFormat.RGBA8_UNORM else np.float32
+        out = np.zeros((self.height, self.width, 4), dt)
+        _check(lib().gs3d_texture_read(self._h, _ptr(out), out.nbytes))
+        return out
+
+    def close(self) -> None:
+        if self._h:
+            lib().gs3d_texture_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Viewer:
+    """Viewer (src/lib.rs:65-276): buffers + Preprocessor + RadixSorter + Renderer."""
+
+    def __init__(self, device: Device, texture_format: TextureFormat, gaussians: np.ndarray,
+                 pod: GaussianPod = GaussianPod.SINGLE):
+        g = np.ascontiguousarray(gaussians, GAUSSIAN_DTYPE)
+        h = C.c_void_p()
+        _check(lib().gs3d_viewer_new(device.handle, int(texture_format), _ptr(g) if g.shape[0] else None,
+                                     int(g.shape[0]), int(pod), C.byref(h)))
+        self._h, self.device, self.count, self.format, self.pod = h, device, int(g.shape[0]), texture_format, pod
+
+    @property
+    def handle(self):
+        return self._h
+
+    # -- Viewer::update_* (src/lib.rs:202-263)
+    def update_camera(self, camera: Camera, texture_size) -> None:
+        c = camera._c()
+        _check(lib().gs3d_viewer_update_camera(self._h, C.byref(c), int(texture_size[0]), int(texture_size[1])))
+
+    def update_camera_with_pod(self, pod: CameraPod) -> None:
+        _check(lib().gs3d_viewer_update_camera_with_pod(self._h, C.byref(pod)))
+
+    def update_model_transform(self, pos, rot, scale) -> None:
+        p = (C.c_float * 3)(*[float(x) for x in pos])
+        r = (C.c_float * 4)(*[float(x) for x in rot])
+        s = (C.c_float * 3)(*[float(x) for x in scale])
+        _check(lib().gs3d_viewer_update_model_transform(self._h, p, r, s))
+
+    def update_gaussian_transform(self, size: float = 1.0, display_mode: DisplayMode = DisplayMode.SPLAT,
+                                  sh_deg: int = 3, no_sh0: bool = False, max_std_dev: float = 3.0) -> None:
+        _check(lib().gs3d_viewer_update_gaussian_transform(self._h, float(size), int(display_mode), int(sh_deg),
+                                                           int(bool(no_sh0)), float(max_std_dev)))
+
+    # -- recording
+    def render(self, target: Texture) -> None:
+        """Viewer::render: preprocess -> sort -> render (enqueue only)."""
+        _check(lib().gs3d_viewer_render(self._h, target.handle))
+
+    def preprocess(self) -> None:
+        _check(lib().gs3d_viewer_preprocess(self._h))
+
+    def sort(self) -> None:
+        _check(lib().gs3d_viewer_sort(self._h))
+
+    def draw(self, target: Texture, standalone: bool = False) -> None:
+        fn = lib().gs3d_viewer_draw_standalone if standalone else lib().gs3d_viewer_draw
+        _check(fn(self._h, target.handle))
+
+    # -- options
+    def set_blend_mode(self, mode: BlendMode) -> None:
+        _check(lib().gs3d_viewer_set_blend_mode(self._h, int(mode)))
+
+    def set_tile_size(self, tw: int, th: int) -> None:
+        _check(lib().gs3d_viewer_set_tile_size(self._h, int(tw), int(th)))
+
+    def set_profiling(self, on: bool) -> None:
+        _check(lib().gs3d_viewer_set_profiling(self._h, int(bool(on))))
+
+    def stage_times(self) -> np.ndarray:
+        out = np.zeros(5, np.float32)
+        _check(lib().gs3d_viewer_stage_times(self._h, _ptr(out)))
+        return out
+
+    # -- buffer downloads
+    def indirect_args(self) -> np.ndarray:
+        out = np.zeros(4, np.uint32)
+        _check(lib().gs3d_viewer_read_indirect_args(self._h, _ptr(out)))
+        return out
+
+    def radix_sort_indirect_args(self) -> np.ndarray:
+        out = np.zeros(3, np.uint32)
+        _check(lib().gs3d_viewer_read_radix_sort_indirect_args(self._h, _ptr(out)))
+        return out
+
+    def indirect_indices(self, count: Optional[int] = None) -> np.ndarray:
+        count = self.count if count is None else int(count)
+        out = np.zeros(max(count, 1), np.uint32)
+        _check(lib().gs3d_viewer_read_indirect_indices(self._h, _ptr(out), count))
+        return out[:count]
+
+    def gaussians_depth(self, count: Optional[int] = None) -> np.ndarray:
+        count = depth_buffer_len(self.count) if count is None else int(count)
+        out = np.zeros(max(count, 1), np.float32)
+        _check(lib().gs3d_viewer_read_gaussians_depth(self._h, _ptr(out), count))
+        return out[:count]
+
+    def close(self) -> None:
+        if self._h:
+            lib().gs3d_viewer_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class RadixSorter:
+    """RadixSorter (src/radix_sorter.rs:9-96) on its own key/payload buffers."""
+
+    def __init__(self, device: Device, capacity: int):
+        h = C.c_void_p()
+        _check(lib().gs3d_radix_sorter_new(device.handle, int(capacity), C.byref(h)))
+        self._h, self.device, self.capacity, self.n = h, device, int(capacity), 0
+
+    def write(self, keys: np.ndarray, vals: np.ndarray) -> None:
+        k = np.ascontiguousarray(keys, np.uint32)
+        v = np.ascontiguousarray(vals, np.uint32)
+        if k.shape != v.shape:
+            raise ValueError("keys and values differ in length")
+        _check(lib().gs3d_radix_sorter_write(self._h, _ptr(k), _ptr(v), int(k.shape[0])))
+        self.n = int(k.shape[0])
+
+    def sort(self) -> None:
+        _check(lib().gs3d_radix_sorter_sort(self._h))
+
+    def read(self):
+        k = np.zeros(max(self.n, 1), np.uint32)
+        v = np.zeros(max(self.n, 1), np.uint32)
+        _check(lib().gs3d_radix_sorter_read(self._h, _ptr(k), _ptr(v), self.n))
+        return k[:self.n], v[:self.n]
+
+    def close(self) -> None:
+        if self._h:
+            lib().gs3d_radix_sorter_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,228 @@
+// Preprocessor kernels: the per-frame constants ("pre", preprocess.wesl:52-56)
+// and the fused projection kernel ("main" + "post", preprocess.wesl:60-126,
+// with the per-instance vertex-stage work of render.wesl:75-130 folded in).
+//
+// Projection is HBM-bound: every Gaussian's position record is streamed once
+// (coalesced, 16 B/lane); the covariance stream is read only for off-screen
+// centres and visible splats, the SH stream only for visible splats. Visible
+// Gaussians are compacted in ascending index order with a wave ballot, a
+// workgroup scan and a decoupled look-back across workgroups, so the output
+// order is deterministic (the reference appends with an atomic counter,
+// preprocess.wesl:101, whose order is not).
+#include <hip/hip_runtime.h>
+
+#include "device_util.hpp"
+#include "gaussian_device.hpp"
+#include "kernels.hpp"
+#include "math.hpp"
+#include "types.hpp"
+
+namespace gs3d {
+
+constexpr int kPreThreads = 256;
+constexpr int kPreWaves = kPreThreads / kWave;
+constexpr int kPreItems = 4;
+constexpr uint32_t kPreTile = kPreThreads * kPreItems;
+
+__global__ __launch_bounds__(256) void k_frame_pre(const CameraPod* __restrict__ cam,
+                                                   const ModelTransformPod* __restrict__ mt,
+                                                   const GaussianTransformPod* __restrict__ gt,
+                                                   FrameConsts* __restrict__ fc, uint32_t* indirect_args,
+                                                   uint32_t* dispatch_args, uint32_t* status,
+                                                   uint32_t num_tiles, uint32_t* ticket) {
+  if (threadIdx.x == 0) {
+    FrameConsts c;
+    derive_frame_consts(*cam, *mt, *gt, c);
+    *fc = c;
+    // preprocess.wesl:55 (instance_count = 0); an empty model leaves the
+    // sort dispatch at zero workgroups (preprocess.wesl:114 with V = 0).
+    indirect_args[1] = 0u;
+    dispatch_args[0] = 0u;
+    dispatch_args[1] = 1u;
+    dispatch_args[2] = 1u;
+    *ticket = 0u;
+  }
+  for (uint32_t t = threadIdx.x; t < num_tiles; t += blockDim.x) status[t] = 0u;
+}
+
+// Frame constants only (a Renderer driven without the preprocessor).
+__global__ __launch_bounds__(64) void k_frame_consts(const CameraPod* __restrict__ cam,
+                                                     const ModelTransformPod* __restrict__ mt,
+                                                     const GaussianTransformPod* __restrict__ gt,
+                                                     FrameConsts* __restrict__ fc) {
+  if (threadIdx.x == 0) {
+    FrameConsts c;
+    derive_frame_consts(*cam, *mt, *gt, c);
+    *fc = c;
+  }
+}
+
+template <int POD>
+__global__ __launch_bounds__(kPreThreads) void k_preprocess(PreprocessArgs a) {
+  __shared__ uint32_t s_cnt[kPreItems][kPreWaves];
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_base;
+  __shared__ uint32_t s_total;
+
+  const FrameConsts& fc = *a.fc;
+  const uint32_t tile = take_ticket(a.ticket, &s_tile);
+  const uint32_t tid = threadIdx.x;
+  const uint32_t w = tid / kWave;
+  const uint32_t base = tile * kPreTile;
+
+  bool vis[kPreItems];
+  float key[kPreItems];
+  uint64_t masks[kPreItems];
+
+#pragma unroll
+  for (int k = 0; k < kPreItems; ++k) {
+    const uint32_t i = base + uint32_t(k) * kPreThreads + tid;
+    vis[k] = false;
+    key[k] = 0.0f;
+    if (i < a.g.n) {
+      const float4 pc = a.g.pc[i];
+      const float pos[3] = {pc.x, pc.y, pc.z};
+      const float p4[4] = {pc.x, pc.y, pc.z, 1.0f};
+      float world[4], clip[4];
+      m4v(fc.model, p4, world);  // model_to_world
+      m4v(fc.pv, world, clip);   // world_to_camera: (proj * view) * world
+      const float nx = clip[0] / clip[3];
+      const float ny = clip[1] / clip[3];
+      const float nz = clip[2] / clip[3];
+      bool visible = true;
+      float c[6];
+      bool have_cov = false;
+      if (cull(nx, ny, nz)) {
+        // preprocess.wesl:87-98: re-test the centre moved towards the
+        // screen by the projected major axis.
+        load_cov3d<POD>(a.g, i, c);
+        have_cov = true;
+        const float sd = fc.max_std_dev;
+        float ax[4];
+        cov2d_axes(fc, pos, c, sd * fc.gsize, ax);
+        const float mx = (ax[0] * sd) / fc.size[0];
+        const float my = (ax[1] * sd) / fc.size[1];
+        const float mlen = sqrtf(mx * mx + my * my);
+        const float nlen = sqrtf((-nx) * (-nx) + (-ny) * (-ny));
+        const float dx = (-nx) / nlen;
+        const float dy = (-ny) / nlen;
+        const float t = fminf(mlen, nlen);
+        visible = !cull(nx + t * dx, ny + t * dy, nz);
+      }
+      if (visible) {
+        key[k] = 1.0f - nz;  // preprocess.wesl:105
+        if (!have_cov) load_cov3d<POD>(a.g, i, c);
+        compute_splat<POD>(fc, a.g, i, pos, __float_as_uint(pc.w), world, c, a.splats + i);
+      }
+      vis[k] = visible;
+    }
+    masks[k] = __ballot(vis[k]);
+    if (lane_id() == 0) s_cnt[k][w] = uint32_t(__popcll(masks[k]));
+  }
+  __syncthreads();
+
+  if (tid == 0) {
+    // exclusive scan over (item, wave) = ascending Gaussian index order
+    uint32_t run = 0;
+    for (int k = 0; k < kPreItems; ++k)
+      for (int ww = 0; ww < kPreWaves; ++ww) {
+        const uint32_t c = s_cnt[k][ww];
+        s_cnt[k][ww] = run;
+        run += c;
+      }
+    uint32_t excl = 0;
+    if (tile == 0) {
+      status_store(a.status, kFlagInclusive | run);
+    } else {
+      status_store(a.status + tile, kFlagAggregate | run);
+      excl = lookback_exclusive(a.status, tile, a.err);
+      status_store(a.status + tile, kFlagInclusive | (excl + run));
+    }
+    s_base = excl;
+    s_total = excl + run;
+    if (tile == a.num_tiles - 1) {
+      // preprocess.wesl:108-116 (post): visible count and sort dispatch size
+      const uint32_t v = excl + run;
+      a.indirect_args[1] = v;
+      a.dispatch_args[0] = (v + kHistoBlockKvs - 1u) / kHistoBlockKvs;
+      a.dispatch_args[1] = 1u;
+      a.dispatch_args[2] = 1u;
+    }
+  }
+  __syncthreads();
+
+  const uint32_t out_base = s_base;
+#pragma unroll
+  for (int k = 0; k < kPreItems; ++k) {
+    if (vis[k]) {
+      const uint32_t i = base + uint32_t(k) * kPreThreads + tid;
+      const uint32_t p = out_base + s_cnt[k][w] + mask_rank(masks[k]);
+      a.indices[p] = i;   // preprocess.wesl:102
+      a.depth[p] = key[k];  // preprocess.wesl:105
+    }
+  }
+
+  if (tile == a.num_tiles - 1) {
+    // preprocess.wesl:118-125: pad keys up to the sort block size with 2.0
+    const uint32_t v = s_total;
+    const uint32_t padded = ((v + kHistoBlockKvs - 1u) / kHistoBlockKvs) * kHistoBlockKvs;
+    const uint32_t end = padded < a.depth_len ? padded : a.depth_len;
+    for (uint32_t j = v + tid; j < end; j += kPreThreads) a.depth[j] = __uint_as_float(kPadKeyBits);
+  }
+}
+
+// Standalone vertex stage for a Renderer driven without the fused
+// preprocessor: one thread per drawn instance (render.wesl:82-83).
+template <int POD>
+__global__ __launch_bounds__(256) void k_vertex(const FrameConsts* __restrict__ fcp, GaussianStreams g,
+                                                const uint32_t* __restrict__ indirect_args,
+                                                const uint32_t* __restrict__ indices, Splat2D* splats) {
+  const FrameConsts& fc = *fcp;
+  const uint32_t v = indirect_args[1];
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < v; s += gridDim.x * blockDim.x) {
+    const uint32_t i = indices[s];
+    if (i >= g.n) continue;
+    const float4 pc = g.pc[i];
+    const float pos[3] = {pc.x, pc.y, pc.z};
+    const float p4[4] = {pc.x, pc.y, pc.z, 1.0f};
+    float world[4], c[6];
+    m4v(fc.model, p4, world);
+    load_cov3d<POD>(g, i, c);
+    compute_splat<POD>(fc, g, i, pos, __float_as_uint(pc.w), world, c, splats + i);
+  }
+}
+
+uint32_t preprocess_num_tiles(uint32_t n) { return (n + kPreTile - 1) / kPreTile; }
+
+void launch_frame_pre(hipStream_t st, const CameraPod* cam, const ModelTransformPod* mt,
+                      const GaussianTransformPod* gt, FrameConsts* fc, uint32_t* indirect_args,
+                      uint32_t* dispatch_args, uint32_t* status, uint32_t num_tiles, uint32_t* ticket) {
+  hipLaunchKernelGGL(k_frame_pre, dim3(1), dim3(256), 0, st, cam, mt, gt, fc, indirect_args, dispatch_args,
+                     status, num_tiles, ticket);
+}
+
+void launch_frame_consts(hipStream_t st, const CameraPod* cam, const ModelTransformPod* mt,
+                         const GaussianTransformPod* gt, FrameConsts* fc) {
+  hipLaunchKernelGGL(k_frame_consts, dim3(1), dim3(64), 0, st, cam, mt, gt, fc);
+}
+
+void launch_preprocess(hipStream_t st, int pod, const PreprocessArgs& a) {
+  if (a.num_tiles == 0) return;
+  if (pod == kPodSingle)
+    hipLaunchKernelGGL(k_preprocess<kPodSingle>, dim3(a.num_tiles), dim3(kPreThreads), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_preprocess<kPodHalf>, dim3(a.num_tiles), dim3(kPreThreads), 0, st, a);
+}
+
+void launch_vertex(hipStream_t st, int pod, const FrameConsts* fc, const GaussianStreams& g,
+                   const uint32_t* indirect_args, const uint32_t* indices, Splat2D* splats, uint32_t max_v) {
+  if (max_v == 0) return;
+  uint32_t blocks = (max_v + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  if (pod == kPodSingle)
+    hipLaunchKernelGGL(k_vertex<kPodSingle>, dim3(blocks), dim3(256), 0, st, fc, g, indirect_args, indices, splats);
+  else
+    hipLaunchKernelGGL(k_vertex<kPodHalf>, dim3(blocks), dim3(256), 0, st, fc, g, indirect_args, indices, splats);
+}
+
+}  // namespace gs3d

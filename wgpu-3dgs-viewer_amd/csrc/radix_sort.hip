@@ -1,0 +1,262 @@
+// Stable LSD radix sort of (u32 key, u32 value) pairs, 8-bit digits, one
+// onesweep pass per digit (Adinets & Merrill 2022) with decoupled look-back
+// (Merrill & Garland 2016). Replaces the reference's vendored wgpu_sort
+// (src/radix_sorter.rs:98-940, src/shader/radix_sort.wgsl): same result (a
+// stable ascending sort), different machinery:
+//   * one upsweep reads the keys once and builds every pass's 256-bin
+//     histogram (LDS atomics, then one global atomic per bin per workgroup);
+//   * each pass is a persistent kernel: a workgroup takes 4096-key tiles by
+//     ticket, ranks them with wave64 ballot matching (8 ballots per key give
+//     the lanes sharing its digit), publishes per-digit counts, looks back
+//     over its predecessors for the global offset, reorders the tile in LDS
+//     and writes runs of equal digits contiguously.
+// Count comes from device memory (GPU-driven, like the reference's indirect
+// dispatch): n = min(*count * count_mul, count_max).
+#include <hip/hip_runtime.h>
+
+#include "device_util.hpp"
+#include "kernels.hpp"
+#include "types.hpp"
+
+namespace gs3d {
+
+constexpr int kRsThreads = 256;
+constexpr int kRsWaves = kRsThreads / kWave;
+constexpr int kRsItems = 16;
+constexpr uint32_t kRsTile = kRsThreads * kRsItems;  // 4096 keys
+
+__device__ __forceinline__ uint32_t sort_n(const SortCount& c) {
+  const uint64_t v = uint64_t(*c.count) * uint64_t(c.count_mul);
+  return v < uint64_t(c.count_max) ? uint32_t(v) : c.count_max;
+}
+
+// Inclusive scan of one value per thread across a 256-thread workgroup.
+// `s_wsum` holds kRsWaves words of LDS. Contains __syncthreads.
+__device__ __forceinline__ uint32_t block_inclusive_scan_256(uint32_t x, uint32_t* s_wsum) {
+  const uint32_t lane = threadIdx.x % kWave;
+  const uint32_t w = threadIdx.x / kWave;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, kWave);
+    if (lane >= uint32_t(off)) x += y;
+  }
+  if (lane == kWave - 1) s_wsum[w] = x;
+  __syncthreads();
+  uint32_t pre = 0;
+  for (uint32_t q = 0; q < w; ++q) pre += s_wsum[q];
+  __syncthreads();
+  return x + pre;
+}
+
+__global__ __launch_bounds__(256) void k_rs_zero(SortCount cnt, SortScratch sc, int passes, uint32_t* pad_keys,
+                                                 uint32_t pad_begin, uint32_t pad_end) {
+  const uint32_t n = sort_n(cnt);
+  const uint32_t tiles = (n + kRsTile - 1) / kRsTile;
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t j = gid; j < uint32_t(passes) * 256u; j += stride) sc.hist[j] = 0u;
+  if (gid < uint32_t(passes)) sc.tickets[gid] = 0u;
+  const uint64_t per_pass = uint64_t(tiles) * 256u;
+  for (uint64_t j = gid; j < per_pass * uint64_t(passes); j += stride) {
+    const uint64_t p = j / per_pass;
+    const uint64_t r = j - p * per_pass;
+    sc.lookback[p * uint64_t(sc.max_tiles) * 256u + r] = 0u;
+  }
+  // radix_sort.wgsl:92-110: keys past the sorted range are 0xFFFFFFFF
+  if (pad_keys)
+    for (uint32_t j = pad_begin + gid; j < pad_end; j += stride) pad_keys[j] = 0xFFFFFFFFu;
+}
+
+__global__ __launch_bounds__(256) void k_rs_upsweep(SortCount cnt, const uint32_t* __restrict__ keys,
+                                                    uint32_t* __restrict__ hist, int passes) {
+  __shared__ uint32_t s_h[4][256];
+  const uint32_t n = sort_n(cnt);
+  for (int p = 0; p < 4; ++p) s_h[p][threadIdx.x] = 0u;
+  __syncthreads();
+  const uint32_t stride = gridDim.x * blockDim.x * 4u;
+  for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) * 4u; i < n; i += stride) {
+    uint32_t k4[4];
+    if (i + 3u < n) {
+      const uint4 v = *reinterpret_cast<const uint4*>(keys + i);
+      k4[0] = v.x;
+      k4[1] = v.y;
+      k4[2] = v.z;
+      k4[3] = v.w;
+    } else {
+      for (int q = 0; q < 4; ++q) k4[q] = (i + q < n) ? keys[i + q] : 0u;
+    }
+    const int m = (i + 3u < n) ? 4 : int(n - i);
+    for (int q = 0; q < m; ++q)
+      for (int p = 0; p < passes; ++p) atomicAdd(&s_h[p][(k4[q] >> (8 * p)) & 0xFFu], 1u);
+  }
+  __syncthreads();
+  for (int p = 0; p < passes; ++p) {
+    const uint32_t c = s_h[p][threadIdx.x];
+    if (c) atomicAdd(&hist[p * 256 + threadIdx.x], c);
+  }
+}
+
+__global__ __launch_bounds__(kRsThreads) void k_rs_onesweep(SortCount cnt, const uint32_t* __restrict__ kin,
+                                                           const uint32_t* __restrict__ vin,
+                                                           uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                           const uint32_t* __restrict__ hist, uint32_t* lookback,
+                                                           uint32_t* ticket, int shift, uint32_t* err) {
+  __shared__ uint32_t s_keys[kRsTile];
+  __shared__ uint32_t s_vals[kRsTile];
+  __shared__ uint32_t s_wcnt[kRsWaves][256];
+  __shared__ uint32_t s_dbase[256];
+  __shared__ uint32_t s_gbase[256];
+  __shared__ uint32_t s_hscan[256];
+  __shared__ uint32_t s_wsum[kRsWaves];
+  __shared__ uint32_t s_tile;
+
+  const uint32_t n = sort_n(cnt);
+  const uint32_t num_tiles = (n + kRsTile - 1) / kRsTile;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = tid % kWave;
+  const uint32_t w = tid / kWave;
+
+  {
+    const uint32_t h = hist[tid];
+    s_hscan[tid] = block_inclusive_scan_256(h, s_wsum) - h;
+  }
+
+  for (;;) {
+    const uint32_t t = take_ticket(ticket, &s_tile);
+    if (t >= num_tiles) break;
+    const uint32_t base = t * kRsTile;
+    const uint32_t valid = (n - base) < kRsTile ? (n - base) : kRsTile;
+
+#pragma unroll
+    for (int q = 0; q < kRsWaves; ++q) s_wcnt[q][tid] = 0u;
+    __syncthreads();
+
+    uint32_t key[kRsItems], val[kRsItems], lrank[kRsItems];
+#pragma unroll
+    for (int k = 0; k < kRsItems; ++k) {
+      const uint32_t li = w * (kWave * kRsItems) + uint32_t(k) * kWave + lane;
+      const bool ok = li < valid;
+      key[k] = ok ? kin[base + li] : 0xFFFFFFFFu;
+      val[k] = ok ? vin[base + li] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < kRsItems; ++k) {
+      const uint32_t d = (key[k] >> shift) & 0xFFu;
+      uint64_t m = ~0ull;
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bb = __ballot(bit);
+        m &= bit ? bb : ~bb;
+      }
+      const uint32_t r = mask_rank(m);
+      const uint32_t prev = s_wcnt[w][d];
+      if (r == 0u) s_wcnt[w][d] = prev + uint32_t(__popcll(m));
+      lrank[k] = prev + r;
+    }
+    __syncthreads();
+
+    // per digit (thread = digit): wave offsets, tile total, publish
+    const uint32_t d = tid;
+    uint32_t tot = 0;
+#pragma unroll
+    for (int q = 0; q < kRsWaves; ++q) {
+      const uint32_t c = s_wcnt[q][d];
+      s_wcnt[q][d] = tot;
+      tot += c;
+    }
+    const uint32_t pub = tot - ((d == 255u) ? (kRsTile - valid) : 0u);
+    status_store(lookback + size_t(t) * 256u + d, (t == 0 ? kFlagInclusive : kFlagAggregate) | pub);
+    const uint32_t dbase = block_inclusive_scan_256(tot, s_wsum) - tot;
+    s_dbase[d] = dbase;
+    __syncthreads();
+
+    // reorder the tile in LDS (stable: ranks follow input order)
+#pragma unroll
+    for (int k = 0; k < kRsItems; ++k) {
+      const uint32_t dk = (key[k] >> shift) & 0xFFu;
+      const uint32_t slot = s_dbase[dk] + s_wcnt[w][dk] + lrank[k];
+      s_keys[slot] = key[k];
+      s_vals[slot] = val[k];
+    }
+
+    // global offset of this tile's run of digit d
+    uint32_t excl = 0;
+    if (t > 0) {
+      int64_t j = int64_t(t) - 1;
+      uint32_t spins = 0;
+      while (j >= 0) {
+        const uint32_t s = status_load(lookback + size_t(j) * 256u + d);
+        const uint32_t flag = s & kFlagMask;
+        if (flag == 0u) {
+          __builtin_amdgcn_s_sleep(1);
+          if (++spins > kSpinLimit) {
+            atomicOr(err, kErrLookbackSpin);
+            break;
+          }
+          continue;
+        }
+        excl += s & kValueMask;
+        if (flag == kFlagInclusive) break;
+        --j;
+      }
+      status_store(lookback + size_t(t) * 256u + d, kFlagInclusive | (excl + pub));
+    }
+    s_gbase[d] = s_hscan[d] + excl - dbase;
+    __syncthreads();
+
+    for (uint32_t j = tid; j < valid; j += kRsThreads) {
+      const uint32_t k = s_keys[j];
+      const uint32_t pos = s_gbase[(k >> shift) & 0xFFu] + j;
+      if (pos < n) {  // always true for a consistent histogram; guards the write
+        kout[pos] = k;
+        vout[pos] = s_vals[j];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+uint32_t sort_tile_items() { return kRsTile; }
+
+uint32_t sort_max_tiles(uint32_t count_max) { return (count_max + kRsTile - 1) / kRsTile; }
+
+size_t sort_lookback_words(uint32_t count_max, int passes) {
+  return size_t(sort_max_tiles(count_max)) * 256u * size_t(passes);
+}
+
+static void launch_zero(hipStream_t st, int num_cus, SortCount cnt, const SortScratch& sc, int passes,
+                        uint32_t* pad_keys, uint32_t pad_begin, uint32_t pad_end) {
+  uint32_t blocks = (sc.max_tiles * 256u * uint32_t(passes) + 255u) / 256u;
+  const uint32_t cap = uint32_t(num_cus) * 4u;
+  if (blocks > cap) blocks = cap;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_rs_zero, dim3(blocks), dim3(256), 0, st, cnt, sc, passes, pad_keys, pad_begin, pad_end);
+}
+
+int launch_radix_sort(hipStream_t st, int num_cus, SortCount cnt, uint32_t* k0, uint32_t* v0, uint32_t* k1,
+                      uint32_t* v1, int passes, const SortScratch& sc, uint32_t* pad_keys, uint32_t pad_begin,
+                      uint32_t pad_end) {
+  if (sc.max_tiles == 0 || passes <= 0) return 0;
+  launch_zero(st, num_cus, cnt, sc, passes, pad_keys, pad_begin, pad_end);
+  uint32_t up_blocks = (sc.max_tiles * kRsTile + 4095u) / 4096u;
+  if (up_blocks > uint32_t(num_cus) * 4u) up_blocks = uint32_t(num_cus) * 4u;
+  if (up_blocks < 1) up_blocks = 1;
+  hipLaunchKernelGGL(k_rs_upsweep, dim3(up_blocks), dim3(256), 0, st, cnt, k0, sc.hist, passes);
+  uint32_t grid = sc.max_tiles;
+  if (grid > uint32_t(num_cus) * 4u) grid = uint32_t(num_cus) * 4u;
+  uint32_t *ks = k0, *vs = v0, *kd = k1, *vd = v1;
+  for (int p = 0; p < passes; ++p) {
+    hipLaunchKernelGGL(k_rs_onesweep, dim3(grid), dim3(kRsThreads), 0, st, cnt, ks, vs, kd, vd, sc.hist + p * 256,
+                       sc.lookback + size_t(p) * sc.max_tiles * 256u, sc.tickets + p, 8 * p, sc.err);
+    uint32_t* t = ks;
+    ks = kd;
+    kd = t;
+    t = vs;
+    vs = vd;
+    vd = t;
+  }
+  return passes & 1;
+}
+
+}  // namespace gs3d

@@ -1,0 +1,120 @@
+// Device-visible record layouts shared by the HIP kernels and the host library.
+//
+// Everything here is plain data: the C ABI in include/gs3d.h exposes only the
+// host-side records (gs3d_gaussian, gs3d_camera_pod, ...); these are the
+// HBM layouts behind it.
+#pragma once
+
+#include <cstdint>
+
+namespace gs3d {
+
+// wgpu_sort::HISTO_BLOCK_KVS (src/radix_sorter.rs:175-178): the reference's
+// sort dispatch granularity. Kept only to reproduce the indirect-dispatch
+// arguments and the depth-buffer padding the reference writes
+// (preprocess.wesl:113-125, radix_sort.wgsl:92-110).
+constexpr uint32_t kHistoBlockKvs = 3840;
+
+// f32 bits of the padding key the reference writes after the visible keys
+// (preprocess.wesl:124, `2.0`).
+constexpr uint32_t kPadKeyBits = 0x40000000u;
+
+enum PodKind : int32_t {
+  kPodSingle = 0,  // f32 SH + f32 cov3d (reference default pod, src/lib.rs:44)
+  kPodHalf = 1,    // f16 SH + f16 rotation/scale, cov3d built on the GPU
+};
+
+enum DisplayMode : int32_t { kModeSplat = 0, kModeEllipse = 1, kModePoint = 2 };
+
+enum TextureFormat : int32_t {
+  kFormatRgba8Unorm = 0,
+  kFormatRgba32Float = 1,
+};
+
+enum BlendMode : int32_t {
+  // front-to-back in f32 with early termination, one quantisation at the end
+  kBlendFast = 0,
+  // back-to-front, src clamped, destination stored as unorm8 after every
+  // splat: the fixed-function ALPHA_BLENDING into an Rgba8Unorm target
+  kBlendExactUnorm8 = 1,
+};
+
+// CameraPod (src/buffer/camera.rs:61-79): 144 bytes.
+struct CameraPod {
+  float view[16];
+  float proj[16];
+  float size[2];
+  uint32_t padding[2];
+};
+static_assert(sizeof(CameraPod) == 144, "CameraPod is 144 bytes");
+
+// Model transform uniform: position, rotation quaternion (x,y,z,w), scale.
+struct ModelTransformPod {
+  float pos[3];
+  float pad0;
+  float rot[4];
+  float scale[3];
+  float pad1;
+};
+static_assert(sizeof(ModelTransformPod) == 48, "ModelTransformPod is 48 bytes");
+
+// Gaussian transform uniform: size, display mode / SH degree / no_sh0 flags,
+// max standard deviation.
+struct GaussianTransformPod {
+  float size;
+  uint32_t flags;  // bits 0-1 display mode, 2-3 sh degree, 4 no_sh0
+  float max_std_dev;
+  uint32_t pad;
+};
+static_assert(sizeof(GaussianTransformPod) == 16, "GaussianTransformPod is 16 bytes");
+
+inline uint32_t pack_gaussian_flags(int32_t mode, uint32_t sh_deg, bool no_sh0) {
+  return (uint32_t(mode) & 3u) | ((sh_deg & 3u) << 2) | (no_sh0 ? 16u : 0u);
+}
+
+// Per-frame constants derived on the GPU from the three uniform buffers at the
+// start of every frame (the reference recomputes them in every invocation).
+struct FrameConsts {
+  float view[16];
+  float proj[16];
+  float pv[16];     // proj * view           (camera.wesl:14)
+  float vm[16];     // view * model          (utils.wesl:37)
+  float model[16];  // model_transform_mat
+  float sr[9];      // model_scale_rot_mat
+  float inv_sr[9];  // model_transform_inv_sr_mat
+  float campos[3];  // render.wesl:59-63
+  float size[2];
+  float focal[2];   // utils.wesl:35
+  float gsize;
+  float max_std_dev;
+  int32_t display_mode;
+  int32_t sh_deg;
+  int32_t no_sh0;
+};
+
+// Per visible Gaussian, the 2D splat the compositor draws (the pixel-space
+// restatement of what vert_main hands the rasterizer, render.wesl:75-130).
+// Indexed by Gaussian index; 64 bytes so that one record is one aligned
+// 64-byte HBM sector.
+struct alignas(16) Splat2D {
+  float cx, cy;      // centre in pixels (y down)
+  float ux, uy;      // qo.x = dot(p - c, u)
+  float vx, vy;      // qo.y = dot(p - c, v)
+  float r, g, b, a;  // view_color(), alpha = opacity
+  float hx, hy;      // half extents in pixels (binning bound)
+  uint32_t drawable;
+  uint32_t pad[3];
+};
+static_assert(sizeof(Splat2D) == 64, "Splat2D is 64 bytes");
+
+// Look-back status words: 2-bit flag + 30-bit value.
+constexpr uint32_t kFlagAggregate = 1u << 30;
+constexpr uint32_t kFlagInclusive = 2u << 30;
+constexpr uint32_t kFlagMask = 3u << 30;
+constexpr uint32_t kValueMask = (1u << 30) - 1u;
+
+// Error bits the kernels raise in the per-viewer status word.
+constexpr uint32_t kErrLookbackSpin = 1u;
+constexpr uint32_t kErrTileOverflow = 2u;
+
+}  // namespace gs3d

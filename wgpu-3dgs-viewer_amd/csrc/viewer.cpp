@@ -1,0 +1,669 @@
+// Host side of the C++ mirror (see viewer.hpp).
+#include "viewer.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "math.hpp"
+
+namespace gs3d {
+
+void hip_check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return;
+  const gs3d_status code = (e == hipErrorOutOfMemory) ? GS3D_ERROR_OUT_OF_MEMORY : GS3D_ERROR_DEVICE;
+  throw Error(code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ---- Device -----------------------------------------------------------------
+
+Device::Device(int ordinal) : ordinal_(ordinal) {
+  int count = 0;
+  hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+  if (ordinal < 0 || ordinal >= count) throw Error(GS3D_ERROR_INVALID_ARGUMENT, "no such HIP device");
+  hip_check(hipSetDevice(ordinal), "hipSetDevice");
+  hipDeviceProp_t prop;
+  hip_check(hipGetDeviceProperties(&prop, ordinal), "hipGetDeviceProperties");
+  num_cus_ = prop.multiProcessorCount;
+  buffer_limit_ = prop.totalGlobalMem;
+  hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+  hip_check(hipMalloc(&err_, 256), "hipMalloc(err)");
+  hip_check(hipMemset(err_, 0, 256), "hipMemset(err)");
+}
+
+Device::~Device() {
+  if (stream_) {
+    hipStreamSynchronize(stream_);
+    hipStreamDestroy(stream_);
+  }
+  if (err_) hipFree(err_);
+}
+
+void Device::make_current() const { hip_check(hipSetDevice(ordinal_), "hipSetDevice"); }
+
+void Device::wait() { hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
+
+void Device::note_render(Renderer* r) {
+  if (std::find(pending_.begin(), pending_.end(), r) == pending_.end()) pending_.push_back(r);
+}
+
+void Device::forget_renderer(Renderer* r) { pending_.erase(std::remove(pending_.begin(), pending_.end(), r), pending_.end()); }
+
+void Device::poll() {
+  make_current();
+  for (int attempt = 0;; ++attempt) {
+    wait();
+    uint32_t e = 0;
+    hip_check(hipMemcpy(&e, err_, 4, hipMemcpyDeviceToHost), "read error word");
+    if (e == 0) break;
+    hip_check(hipMemset(err_, 0, 4), "clear error word");
+    if (e & kErrLookbackSpin) {
+      pending_.clear();
+      throw Error(GS3D_ERROR_KERNEL, "a kernel exceeded its look-back wait limit");
+    }
+    if ((e & kErrTileOverflow) && attempt < 4) {
+      bool any = false;
+      for (Renderer* r : pending_)
+        if (r->grow_if_overflowed()) {
+          r->rerender();
+          any = true;
+        }
+      if (any) continue;
+    }
+    break;
+  }
+  pending_.clear();
+}
+
+// ---- DeviceBuffer -------------------------------------------------------------
+
+DeviceBuffer::DeviceBuffer(Device& dev, size_t bytes) : dev_(&dev), bytes_(bytes) {
+  dev.make_current();
+  const size_t alloc = bytes ? bytes : 4;
+  hip_check(hipMalloc(&ptr_, alloc), "hipMalloc");
+  hip_check(hipMemsetAsync(ptr_, 0, alloc, dev.stream()), "hipMemsetAsync");
+  hip_check(hipStreamSynchronize(dev.stream()), "hipStreamSynchronize");
+}
+
+DeviceBuffer::~DeviceBuffer() {
+  if (ptr_) hipFree(ptr_);
+}
+
+DeviceBuffer::DeviceBuffer(DeviceBuffer&& o) noexcept : dev_(o.dev_), ptr_(o.ptr_), bytes_(o.bytes_) {
+  o.ptr_ = nullptr;
+  o.bytes_ = 0;
+}
+
+DeviceBuffer& DeviceBuffer::operator=(DeviceBuffer&& o) noexcept {
+  if (this != &o) {
+    if (ptr_) hipFree(ptr_);
+    dev_ = o.dev_;
+    ptr_ = o.ptr_;
+    bytes_ = o.bytes_;
+    o.ptr_ = nullptr;
+    o.bytes_ = 0;
+  }
+  return *this;
+}
+
+void DeviceBuffer::upload(const void* src, size_t bytes, size_t offset) {
+  if (offset + bytes > bytes_) throw Error(GS3D_ERROR_INVALID_ARGUMENT, "upload past the end of a buffer");
+  if (bytes == 0) return;
+  // stream-ordered like queue.write_buffer; pageable sources are staged
+  // before the call returns
+  hip_check(hipMemcpyAsync(static_cast<char*>(ptr_) + offset, src, bytes, hipMemcpyHostToDevice, dev_->stream()),
+            "hipMemcpyAsync(H2D)");
+}
+
+void DeviceBuffer::download(void* dst, size_t bytes, size_t offset) const {
+  if (offset + bytes > bytes_) throw Error(GS3D_ERROR_INVALID_ARGUMENT, "download past the end of a buffer");
+  if (bytes == 0) return;
+  hip_check(hipMemcpyAsync(dst, static_cast<const char*>(ptr_) + offset, bytes, hipMemcpyDeviceToHost,
+                           dev_->stream()),
+            "hipMemcpyAsync(D2H)");
+  hip_check(hipStreamSynchronize(dev_->stream()), "hipStreamSynchronize");
+}
+
+// ---- UniformBuffer ---------------------------------------------------------------
+
+UniformBuffer::UniformBuffer(Device& dev, size_t bytes) : device_(dev), buf_(dev, bytes), bytes_(bytes) {
+  hip_check(hipHostMalloc(&staging_, bytes, hipHostMallocDefault), "hipHostMalloc");
+  hip_check(hipEventCreateWithFlags(&done_, hipEventDisableTiming), "hipEventCreate");
+  hip_check(hipEventRecord(done_, dev.stream()), "hipEventRecord");
+}
+
+UniformBuffer::~UniformBuffer() {
+  if (done_) {
+    hipEventSynchronize(done_);
+    hipEventDestroy(done_);
+  }
+  if (staging_) hipHostFree(staging_);
+}
+
+void UniformBuffer::update(const void* src) {
+  // the previous copy out of the staging memory must have finished
+  hip_check(hipEventSynchronize(done_), "hipEventSynchronize");
+  std::memcpy(staging_, src, bytes_);
+  hip_check(hipMemcpyAsync(buf_.get(), staging_, bytes_, hipMemcpyHostToDevice, device_.stream()),
+            "hipMemcpyAsync(uniform)");
+  hip_check(hipEventRecord(done_, device_.stream()), "hipEventRecord");
+}
+
+// ---- camera (src/camera.rs:71-92, glam 0.30) --------------------------------------
+
+static void cross3(const float a[3], const float b[3], float o[3]) {
+  o[0] = a[1] * b[2] - b[1] * a[2];
+  o[1] = a[2] * b[0] - b[2] * a[0];
+  o[2] = a[0] * b[1] - b[0] * a[1];
+}
+
+static float dot3(const float a[3], const float b[3]) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+
+void look_to_rh(const float eye[3], const float dir[3], const float up[3], float out[16]) {
+  float s0[3], s[3], u[3];
+  cross3(dir, up, s0);
+  const float inv = 1.0f / std::sqrt(dot3(s0, s0));
+  for (int i = 0; i < 3; ++i) s[i] = s0[i] * inv;
+  cross3(s, dir, u);
+  const float m[16] = {s[0], u[0], -dir[0], 0.0f, s[1], u[1], -dir[1], 0.0f,
+                       s[2], u[2], -dir[2], 0.0f, -dot3(eye, s), -dot3(eye, u), dot3(eye, dir), 1.0f};
+  std::memcpy(out, m, sizeof m);
+}
+
+void perspective_rh(float fov_y, float aspect, float z_near, float z_far, float out[16]) {
+  const float half = 0.5f * fov_y;
+  const float h = std::cos(half) / std::sin(half);
+  const float w = h / aspect;
+  const float r = z_far / (z_near - z_far);
+  const float m[16] = {w, 0, 0, 0, 0, h, 0, 0, 0, 0, r, -1.0f, 0, 0, r * z_near, 0};
+  std::memcpy(out, m, sizeof m);
+}
+
+void Camera::forward(float out[3]) const {
+  out[0] = std::cos(pitch) * std::sin(yaw);
+  out[1] = std::sin(pitch);
+  out[2] = std::cos(pitch) * std::cos(yaw);
+}
+
+void Camera::view(float out[16]) const {
+  float f[3];
+  forward(f);
+  const float up[3] = {0.0f, 1.0f, 0.0f};
+  look_to_rh(pos, f, up, out);
+}
+
+void Camera::projection(float aspect, float out[16]) const { perspective_rh(vertical_fov, aspect, z_start, z_end, out); }
+
+CameraPod camera_pod_new(const Camera& c, uint32_t w, uint32_t h) {
+  CameraPod p{};
+  c.view(p.view);
+  c.projection(float(w) / float(h), p.proj);
+  p.size[0] = float(w);
+  p.size[1] = float(h);
+  return p;
+}
+
+// ---- buffers -------------------------------------------------------------------
+
+CameraBuffer::CameraBuffer(Device& dev) : buf_(dev, sizeof(CameraPod)) {}
+
+void CameraBuffer::update(const Camera& c, uint32_t w, uint32_t h) { update_with_pod(camera_pod_new(c, w, h)); }
+
+void CameraBuffer::update_with_pod(const CameraPod& pod) {
+  pod_ = pod;
+  buf_.update(&pod_);
+}
+
+ModelTransformBuffer::ModelTransformBuffer(Device& dev) : buf_(dev, sizeof(ModelTransformPod)) {
+  const float p[3] = {0, 0, 0}, r[4] = {0, 0, 0, 1}, s[3] = {1, 1, 1};
+  update(p, r, s);
+}
+
+void ModelTransformBuffer::update(const float pos[3], const float rot[4], const float scale[3]) {
+  ModelTransformPod t{};
+  for (int i = 0; i < 3; ++i) {
+    t.pos[i] = pos[i];
+    t.scale[i] = scale[i];
+  }
+  for (int i = 0; i < 4; ++i) t.rot[i] = rot[i];
+  buf_.update(&t);
+}
+
+GaussianTransformBuffer::GaussianTransformBuffer(Device& dev) : buf_(dev, sizeof(GaussianTransformPod)) {
+  update(1.0f, kModeSplat, 3, false, 3.0f);
+}
+
+void GaussianTransformBuffer::update(float size, int32_t mode, uint32_t sh_deg, bool no_sh0, float max_std_dev) {
+  if (mode < kModeSplat || mode > kModePoint) throw Error(GS3D_ERROR_INVALID_ARGUMENT, "unknown display mode");
+  if (sh_deg > 3) throw Error(GS3D_ERROR_INVALID_ARGUMENT, "SH degree must be 0..3");
+  if (!(max_std_dev > 0.0f)) throw Error(GS3D_ERROR_INVALID_ARGUMENT, "max standard deviation must be positive");
+  GaussianTransformPod t{};
+  t.size = size;
+  t.flags = pack_gaussian_flags(mode, sh_deg, no_sh0);
+  t.max_std_dev = max_std_dev;
+  buf_.update(&t);
+}
+
+uint16_t float_to_half(float f) {
+  const _Float16 h = static_cast<_Float16>(f);
+  uint16_t u;
+  std::memcpy(&u, &h, 2);
+  return u;
+}
+
+uint32_t pack_color(const uint8_t c[4]) {
+  return uint32_t(c[0]) | (uint32_t(c[1]) << 8) | (uint32_t(c[2]) << 16) | (uint32_t(c[3]) << 24);
+}
+
+void gaussian_cov3d(const gs3d_gaussian& g, float out[6]) { cov3d_from_scale_rot(g.scale, g.rot, out); }
+
+uint64_t GaussiansBuffer::bytes_per_gaussian(PodKind pod) {
+  return pod == kPodSingle ? (16 + 24 + 192) : (16 + 16 + 96);
+}
+
+uint64_t GaussiansBuffer::size_bytes() const { return uint64_t(n_) * bytes_per_gaussian(pod_); }
+
+GaussiansBuffer::GaussiansBuffer(Device& dev, const gs3d_gaussian* g, uint32_t n, PodKind pod) : pod_(pod), n_(n) {
+  if (pod != kPodSingle && pod != kPodHalf) throw Error(GS3D_ERROR_INVALID_ARGUMENT, "unknown gaussian pod");
+  const uint64_t total = uint64_t(n) * bytes_per_gaussian(pod);
+  if (total > dev.buffer_limit())
+    throw Error(GS3D_ERROR_MODEL_SIZE_EXCEEDS_DEVICE_LIMIT,
+                "model size exceeds the device limit: " + std::to_string(total) + " > " +
+                    std::to_string(dev.buffer_limit()) + ", try smaller model or more aggressive compression");
+  pc_ = DeviceBuffer(dev, size_t(n) * 16);
+  if (pod == kPodSingle) {
+    cov_ = DeviceBuffer(dev, size_t(n) * 24);
+    sh_ = DeviceBuffer(dev, size_t(n) * 192);
+  } else {
+    rs_ = DeviceBuffer(dev, size_t(n) * 16);
+    sh_ = DeviceBuffer(dev, size_t(n) * 96);
+  }
+  // pack and upload in chunks to bound host memory
+  const uint32_t chunk = 1u << 18;
+  std::vector<float> pc, cov, sh32;
+  std::vector<uint16_t> rs, sh16;
+  for (uint32_t b = 0; b < n; b += chunk) {
+    const uint32_t m = std::min(chunk, n - b);
+    pc.assign(size_t(m) * 4, 0.0f);
+    for (uint32_t j = 0; j < m; ++j) {
+      const gs3d_gaussian& q = g[b + j];
+      pc[4 * j + 0] = q.pos[0];
+      pc[4 * j + 1] = q.pos[1];
+      pc[4 * j + 2] = q.pos[2];
+      const uint32_t cb = pack_color(q.color);
+      std::memcpy(&pc[4 * j + 3], &cb, 4);
+    }
+    pc_.upload(pc.data(), size_t(m) * 16, size_t(b) * 16);
+    if (pod == kPodSingle) {
+      cov.assign(size_t(m) * 6, 0.0f);
+      sh32.assign(size_t(m) * 48, 0.0f);
+      for (uint32_t j = 0; j < m; ++j) {
+        const gs3d_gaussian& q = g[b + j];
+        gaussian_cov3d(q, &cov[6 * j]);
+        std::memcpy(&sh32[48 * size_t(j)], &q.sh[0][0], 45 * sizeof(float));
+      }
+      cov_.upload(cov.data(), size_t(m) * 24, size_t(b) * 24);
+      sh_.upload(sh32.data(), size_t(m) * 192, size_t(b) * 192);
+    } else {
+      rs.assign(size_t(m) * 8, 0);
+      sh16.assign(size_t(m) * 48, 0);
+      for (uint32_t j = 0; j < m; ++j) {
+        const gs3d_gaussian& q = g[b + j];
+        for (int k = 0; k < 4; ++k) rs[8 * size_t(j) + k] = float_to_half(q.rot[k]);
+        for (int k = 0; k < 3; ++k) rs[8 * size_t(j) + 4 + k] = float_to_half(q.scale[k]);
+        for (int k = 0; k < 45; ++k) sh16[48 * size_t(j) + k] = float_to_half((&q.sh[0][0])[k]);
+      }
+      rs_.upload(rs.data(), size_t(m) * 16, size_t(b) * 16);
+      sh_.upload(sh16.data(), size_t(m) * 96, size_t(b) * 96);
+    }
+    // the staging vectors are reused: wait for the copies of this chunk
+    dev.wait();
+  }
+}
+
+GaussianStreams GaussiansBuffer::streams() const {
+  GaussianStreams s{};
+  s.pc = pc_.as<float4>();
+  s.cov = pod_ == kPodSingle ? cov_.as<float2>() : nullptr;
+  s.rs = pod_ == kPodHalf ? rs_.as<uint4>() : nullptr;
+  s.sh32 = pod_ == kPodSingle ? sh_.as<float4>() : nullptr;
+  s.sh16 = pod_ == kPodHalf ? sh_.as<uint4>() : nullptr;
+  s.n = n_;
+  return s;
+}
+
+IndirectArgsBuffer::IndirectArgsBuffer(Device& dev) : buf_(dev, 16) {
+  const uint32_t init[4] = {6, 0, 0, 0};  // src/buffer/indirect_args.rs:16-21
+  buf_.upload(init, 16);
+  dev.wait();
+}
+
+RadixSortIndirectArgsBuffer::RadixSortIndirectArgsBuffer(Device& dev) : buf_(dev, 12) {
+  const uint32_t init[3] = {1, 1, 1};  // src/buffer/indirect_args.rs:67
+  buf_.upload(init, 12);
+  dev.wait();
+}
+
+IndirectIndicesBuffer::IndirectIndicesBuffer(Device& dev, uint32_t n) : n_(n), buf_(dev, size_t(n) * 4) {}
+
+uint32_t depth_buffer_len(uint32_t n) { return ((n + kHistoBlockKvs - 1) / kHistoBlockKvs) * kHistoBlockKvs; }
+
+GaussiansDepthBuffer::GaussiansDepthBuffer(Device& dev, uint32_t n)
+    : len_(depth_buffer_len(n)), buf_(dev, size_t(depth_buffer_len(n)) * 4) {}
+
+SplatsBuffer::SplatsBuffer(Device& dev, uint32_t n) : buf_(dev, size_t(n) * sizeof(Splat2D)) {}
+
+Texture::Texture(Device& dev, uint32_t w, uint32_t h, TextureFormat fmt)
+    : device_(&dev), w_(w), h_(h), fmt_(fmt) {
+  if (w == 0 || h == 0 || w > 16384 || h > 16384) throw Error(GS3D_ERROR_INVALID_ARGUMENT, "bad texture size");
+  if (fmt != kFormatRgba8Unorm && fmt != kFormatRgba32Float)
+    throw Error(GS3D_ERROR_INVALID_ARGUMENT, "unknown texture format");
+  buf_ = DeviceBuffer(dev, size_t(w) * h * (fmt == kFormatRgba8Unorm ? 4 : 16));
+}
+
+void Texture::read(void* dst, size_t bytes) const {
+  if (bytes != buf_.size()) throw Error(GS3D_ERROR_INVALID_ARGUMENT, "texture read size mismatch");
+  device_->poll();
+  buf_.download(dst, bytes);
+}
+
+// ---- StageTimer ----------------------------------------------------------------
+
+StageTimer::StageTimer(Device& dev) : dev_(&dev) {
+  for (int s = 0; s < GS3D_STAGE_COUNT; ++s) {
+    hip_check(hipEventCreate(&ev_[s][0]), "hipEventCreate");
+    hip_check(hipEventCreate(&ev_[s][1]), "hipEventCreate");
+    used_[s] = false;
+  }
+}
+
+StageTimer::~StageTimer() {
+  for (int s = 0; s < GS3D_STAGE_COUNT; ++s) {
+    hipEventDestroy(ev_[s][0]);
+    hipEventDestroy(ev_[s][1]);
+  }
+}
+
+void StageTimer::mark(int stage, bool end) {
+  if (!enabled_) return;
+  hip_check(hipEventRecord(ev_[stage][end ? 1 : 0], dev_->stream()), "hipEventRecord");
+  if (end) used_[stage] = true;
+}
+
+void StageTimer::read(float ms[GS3D_STAGE_COUNT]) {
+  dev_->wait();
+  for (int s = 0; s < GS3D_STAGE_COUNT; ++s) {
+    ms[s] = 0.0f;
+    if (used_[s]) hip_check(hipEventElapsedTime(&ms[s], ev_[s][0], ev_[s][1]), "hipEventElapsedTime");
+  }
+}
+
+// ---- Preprocessor ----------------------------------------------------------------
+
+Preprocessor::Preprocessor(Device& dev, const CameraBuffer& camera, const ModelTransformBuffer& model,
+                           const GaussianTransformBuffer& gt, const GaussiansBuffer& gaussians,
+                           IndirectArgsBuffer& indirect_args, RadixSortIndirectArgsBuffer& rs_args,
+                           IndirectIndicesBuffer& indices, GaussiansDepthBuffer& depth, SplatsBuffer& splats)
+    : dev_(dev),
+      camera_(camera),
+      model_(model),
+      gt_(gt),
+      gaussians_(gaussians),
+      indirect_args_(indirect_args),
+      rs_args_(rs_args),
+      indices_(indices),
+      depth_(depth),
+      splats_(splats),
+      consts_(dev, sizeof(FrameConsts)),
+      status_(dev, size_t(std::max(preprocess_num_tiles(gaussians.len()), 1u)) * 4),
+      ticket_(dev, 64),
+      max_tiles_(preprocess_num_tiles(gaussians.len())) {
+  // src/preprocessor.rs:239-246
+  if (gaussians.size_bytes() > dev.buffer_limit())
+    throw Error(GS3D_ERROR_MODEL_SIZE_EXCEEDS_DEVICE_LIMIT, "model size exceeds the device limit");
+}
+
+void Preprocessor::preprocess(uint32_t gaussian_count) {
+  const uint32_t n = std::min(gaussian_count, gaussians_.len());
+  const uint32_t tiles = preprocess_num_tiles(n);
+  launch_frame_pre(dev_.stream(), camera_.dev(), model_.dev(), gt_.dev(), consts_.as<FrameConsts>(),
+                   indirect_args_.dev(), rs_args_.dev(), status_.as<uint32_t>(), tiles, ticket_.as<uint32_t>());
+  PreprocessArgs a{};
+  a.fc = consts_.as<FrameConsts>();
+  a.g = gaussians_.streams();
+  a.g.n = n;
+  a.indices = indices_.dev();
+  a.depth = depth_.dev();
+  a.depth_len = depth_.len();
+  a.splats = splats_.dev();
+  a.indirect_args = indirect_args_.dev();
+  a.dispatch_args = rs_args_.dev();
+  a.status = status_.as<uint32_t>();
+  a.ticket = ticket_.as<uint32_t>();
+  a.num_tiles = tiles;
+  a.err = dev_.err();
+  launch_preprocess(dev_.stream(), gaussians_.pod(), a);
+  hip_check(hipGetLastError(), "launch preprocess");
+}
+
+// ---- sort ----------------------------------------------------------------------
+
+SortScratchBuffers::SortScratchBuffers(Device& dev, uint32_t capacity, int passes)
+    : dev_(dev),
+      capacity_(capacity),
+      passes_(passes),
+      hist_(dev, size_t(passes) * 256 * 4),
+      lookback_(dev, std::max<size_t>(sort_lookback_words(capacity, passes), 1) * 4),
+      tickets_(dev, 64) {}
+
+SortScratch SortScratchBuffers::get() const {
+  SortScratch s{};
+  s.hist = hist_.as<uint32_t>();
+  s.lookback = lookback_.as<uint32_t>();
+  s.tickets = tickets_.as<uint32_t>();
+  s.max_tiles = sort_max_tiles(capacity_);
+  s.err = dev_.err();
+  return s;
+}
+
+RadixSorter::RadixSorter(Device& dev, GaussiansDepthBuffer& depth, IndirectIndicesBuffer& indices)
+    : dev_(dev),
+      depth_(depth),
+      indices_(indices),
+      keys_b_(dev, size_t(indices.len()) * 4),
+      vals_b_(dev, size_t(indices.len()) * 4),
+      scratch_(dev, indices.len(), 4) {}
+
+void RadixSorter::sort(const RadixSortIndirectArgsBuffer& rs_args) {
+  // the reference sorts dispatch.x * 3840 entries, payload clamped to N
+  // (src/radix_sorter.rs:658-669, 741)
+  sort_count(rs_args.dev(), kHistoBlockKvs);
+}
+
+void RadixSorter::sort_count(const uint32_t* count_dev, uint32_t count_mul) {
+  const uint32_t n = indices_.len();
+  if (n == 0) return;
+  SortCount c{count_dev, count_mul, n};
+  uint32_t* keys = reinterpret_cast<uint32_t*>(depth_.dev());
+  launch_radix_sort(dev_.stream(), dev_.num_cus(), c, keys, indices_.dev(), keys_b_.as<uint32_t>(),
+                    vals_b_.as<uint32_t>(), 4, scratch_.get(), keys, n, depth_.len());
+  hip_check(hipGetLastError(), "launch sort");
+}
+
+// ---- Renderer ----------------------------------------------------------------------
+
+Renderer::Renderer(Device& dev, TextureFormat format, const CameraBuffer& camera, const ModelTransformBuffer& model,
+                   const GaussianTransformBuffer& gt, const GaussiansBuffer& gaussians,
+                   const IndirectIndicesBuffer& indices, SplatsBuffer& splats, const FrameConsts* shared_consts)
+    : dev_(dev),
+      format_(format),
+      camera_(camera),
+      model_(model),
+      gt_(gt),
+      gaussians_(gaussians),
+      indices_(indices),
+      splats_(splats),
+      shared_consts_(shared_consts),
+      own_consts_(dev, sizeof(FrameConsts)),
+      bin_status_(dev, size_t(std::max(bin_max_tiles(indices.len()), 1u)) * 4),
+      bin_ticket_(dev, 64),
+      pair_count_(dev, 64),
+      timer_(dev) {
+  if (format != kFormatRgba8Unorm && format != kFormatRgba32Float)
+    throw Error(GS3D_ERROR_INVALID_ARGUMENT, "unknown texture format");
+  // src/renderer.rs:130-135
+  if (gaussians.size_bytes() > dev.buffer_limit())
+    throw Error(GS3D_ERROR_MODEL_SIZE_EXCEEDS_DEVICE_LIMIT, "model size exceeds the device limit");
+  ensure_capacity(std::max<uint32_t>(1u << 20, std::min<uint64_t>(uint64_t(indices.len()) * 2, 0x3FFFFFFFu)));
+}
+
+Renderer::~Renderer() { dev_.forget_renderer(this); }
+
+void Renderer::set_tile_size(uint32_t w, uint32_t h) {
+  const bool ok = (w == 8 && h == 8) || (w == 16 && h == 16) || (w == 32 && h == 8);
+  if (!ok) throw Error(GS3D_ERROR_INVALID_ARGUMENT, "tile size must be 8x8, 16x16 or 32x8");
+  tile_w_ = w;
+  tile_h_ = h;
+}
+
+void Renderer::ensure_capacity(uint32_t pairs) {
+  if (pairs <= capacity_) return;
+  dev_.wait();
+  for (int b = 0; b < 2; ++b) {
+    pk_[b] = DeviceBuffer(dev_, size_t(pairs) * 4);
+    pv_[b] = DeviceBuffer(dev_, size_t(pairs) * 4);
+  }
+  scratch_ = std::make_unique<SortScratchBuffers>(dev_, pairs, 3);
+  capacity_ = pairs;
+}
+
+bool Renderer::grow_if_overflowed() {
+  uint32_t total = 0;
+  pair_count_.download(&total, 4);
+  if (total <= capacity_) return false;
+  const uint64_t want = std::min<uint64_t>(uint64_t(total) + total / 4 + 1024, 0x3FFFFFFFu);
+  ensure_capacity(uint32_t(want));
+  return true;
+}
+
+void Renderer::rerender() {
+  if (last_target_ && last_args_) render(*last_target_, *last_args_, last_standalone_);
+}
+
+static uint32_t ceil_log2(uint32_t x) {
+  uint32_t b = 0;
+  while ((1u << b) < x && b < 31) ++b;
+  return b;
+}
+
+void Renderer::render(Texture& target, const IndirectArgsBuffer& indirect_args, bool standalone) {
+  if (target.format() != format_)
+    throw Error(GS3D_ERROR_INVALID_ARGUMENT, "render target format differs from the renderer's format");
+  hipStream_t st = dev_.stream();
+  StageTimer& tm = active_timer();
+  const FrameConsts* fc = shared_consts_;
+  if (standalone || fc == nullptr) {
+    launch_frame_consts(st, camera_.dev(), model_.dev(), gt_.dev(), own_consts_.as<FrameConsts>());
+    fc = own_consts_.as<FrameConsts>();
+  }
+  if (standalone)
+    launch_vertex(st, gaussians_.pod(), fc, gaussians_.streams(), indirect_args.dev(), indices_.dev(), splats_.dev(),
+                  indices_.len());
+
+  TileGrid g{};
+  g.width = target.width();
+  g.height = target.height();
+  g.tile_w = tile_w_;
+  g.tile_h = tile_h_;
+  g.tiles_x = (g.width + tile_w_ - 1) / tile_w_;
+  g.tiles_y = (g.height + tile_h_ - 1) / tile_h_;
+  const uint32_t num_tiles = g.tiles_x * g.tiles_y;
+  if (ranges_words_ < num_tiles * 2) {
+    dev_.wait();
+    ranges_ = DeviceBuffer(dev_, size_t(num_tiles) * 8);
+    ranges_words_ = num_tiles * 2;
+  }
+
+  tm.mark(GS3D_STAGE_BIN, false);
+  const uint32_t bin_tiles = bin_max_tiles(indices_.len());
+  launch_bin_reset(st, ranges_.as<uint32_t>(), num_tiles, bin_status_.as<uint32_t>(), bin_tiles,
+                   bin_ticket_.as<uint32_t>(), pair_count_.as<uint32_t>());
+  BinArgs b{};
+  b.indirect_args = indirect_args.dev();
+  b.indices = indices_.dev();
+  b.splats = splats_.dev();
+  b.grid = g;
+  b.pair_keys = pk_[0].as<uint32_t>();
+  b.pair_vals = pv_[0].as<uint32_t>();
+  b.capacity = capacity_;
+  b.pair_count = pair_count_.as<uint32_t>();
+  b.status = bin_status_.as<uint32_t>();
+  b.ticket = bin_ticket_.as<uint32_t>();
+  b.max_tiles = bin_tiles;
+  b.err = dev_.err();
+  launch_bin(st, b);
+  tm.mark(GS3D_STAGE_BIN, true);
+
+  tm.mark(GS3D_STAGE_TILE_SORT, false);
+  const int passes = int((ceil_log2(num_tiles) + 7) / 8);
+  SortCount c{pair_count_.as<uint32_t>(), 1u, capacity_};
+  const int which = launch_radix_sort(st, dev_.num_cus(), c, pk_[0].as<uint32_t>(), pv_[0].as<uint32_t>(),
+                                      pk_[1].as<uint32_t>(), pv_[1].as<uint32_t>(), passes, scratch_->get());
+  launch_ranges(st, pair_count_.as<uint32_t>(), capacity_, pk_[which].as<uint32_t>(), ranges_.as<uint32_t>(),
+                dev_.num_cus());
+  tm.mark(GS3D_STAGE_TILE_SORT, true);
+
+  tm.mark(GS3D_STAGE_COMPOSITE, false);
+  CompositeArgs ca{};
+  ca.fc = fc;
+  ca.pair_vals = pv_[which].as<uint32_t>();
+  ca.ranges = ranges_.as<uint32_t>();
+  ca.splats = splats_.dev();
+  ca.grid = g;
+  ca.format = format_;
+  ca.blend = blend_;
+  ca.out = target.dev();
+  launch_composite(st, ca);
+  tm.mark(GS3D_STAGE_COMPOSITE, true);
+  hip_check(hipGetLastError(), "launch render");
+
+  last_target_ = &target;
+  last_args_ = &indirect_args;
+  last_standalone_ = standalone;
+  dev_.note_render(this);
+}
+
+// ---- Viewer --------------------------------------------------------------------------
+
+Viewer::Viewer(Device& dev, TextureFormat format, const gs3d_gaussian* g, uint32_t n, PodKind pod)
+    : device(dev),
+      camera_buffer(dev),
+      model_transform_buffer(dev),
+      gaussian_transform_buffer(dev),
+      gaussians_buffer(dev, g, n, pod),
+      indirect_args_buffer(dev),
+      radix_sort_indirect_args_buffer(dev),
+      indirect_indices_buffer(dev, n),
+      gaussians_depth_buffer(dev, n),
+      splats_buffer(dev, n),
+      preprocessor(dev, camera_buffer, model_transform_buffer, gaussian_transform_buffer, gaussians_buffer,
+                   indirect_args_buffer, radix_sort_indirect_args_buffer, indirect_indices_buffer,
+                   gaussians_depth_buffer, splats_buffer),
+      radix_sorter(dev, gaussians_depth_buffer, indirect_indices_buffer),
+      renderer(dev, format, camera_buffer, model_transform_buffer, gaussian_transform_buffer, gaussians_buffer,
+               indirect_indices_buffer, splats_buffer, preprocessor.frame_consts()),
+      timer(dev) {
+  renderer.set_timer(&timer);
+}
+
+void Viewer::render(Texture& target) {
+  timer.mark(GS3D_STAGE_PREPROCESS, false);
+  preprocessor.preprocess(gaussians_buffer.len());
+  timer.mark(GS3D_STAGE_PREPROCESS, true);
+  timer.mark(GS3D_STAGE_SORT, false);
+  radix_sorter.sort(radix_sort_indirect_args_buffer);
+  timer.mark(GS3D_STAGE_SORT, true);
+  renderer.render(target, indirect_args_buffer, false);
+}
+
+}  // namespace gs3d

@@ -1,0 +1,416 @@
+// C++ host mirror of the reference's Viewer / Preprocessor / RadixSorter /
+// Renderer and their buffers (src/lib.rs, src/preprocessor.rs,
+// src/radix_sorter.rs, src/renderer.rs, src/buffer/*). Type names and call
+// shapes follow the reference; wgpu handles become a Device (HIP device +
+// stream) and device buffers. The C ABI in include/gs3d.h wraps these.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/gs3d.h"
+#include "kernels.hpp"
+#include "types.hpp"
+
+namespace gs3d {
+
+// Error carrying a gs3d_status (the reference's error enums, src/error.rs).
+struct Error : std::runtime_error {
+  gs3d_status code;
+  Error(gs3d_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what);
+
+class Renderer;
+
+// wgpu::Device + wgpu::Queue: one HIP device and one stream.
+class Device {
+ public:
+  explicit Device(int ordinal);
+  ~Device();
+  Device(const Device&) = delete;
+  Device& operator=(const Device&) = delete;
+
+  hipStream_t stream() const { return stream_; }
+  int ordinal() const { return ordinal_; }
+  int num_cus() const { return num_cus_; }
+  uint64_t buffer_limit() const { return buffer_limit_; }
+  void set_buffer_limit(uint64_t b) { buffer_limit_ = b; }
+  // Device-side error word every kernel may OR bits into.
+  uint32_t* err() const { return err_; }
+  void make_current() const;
+
+  // queue.submit + device.poll(wait). Re-renders frames whose tile lists
+  // overflowed; throws on kernel-reported errors.
+  void poll();
+  void wait();
+  void note_render(Renderer* r);
+  void forget_renderer(Renderer* r);
+
+ private:
+  int ordinal_ = 0;
+  int num_cus_ = 0;
+  uint64_t buffer_limit_ = 0;
+  hipStream_t stream_ = nullptr;
+  uint32_t* err_ = nullptr;
+  std::vector<Renderer*> pending_;
+};
+
+// Owning device allocation.
+class DeviceBuffer {
+ public:
+  DeviceBuffer() = default;
+  DeviceBuffer(Device& dev, size_t bytes);
+  ~DeviceBuffer();
+  DeviceBuffer(DeviceBuffer&& o) noexcept;
+  DeviceBuffer& operator=(DeviceBuffer&& o) noexcept;
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+
+  void* get() const { return ptr_; }
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(ptr_);
+  }
+  size_t size() const { return bytes_; }
+  void upload(const void* src, size_t bytes, size_t offset = 0);
+  void download(void* dst, size_t bytes, size_t offset = 0) const;
+
+ private:
+  Device* dev_ = nullptr;
+  void* ptr_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+// Small uniform buffer written from the host in stream order (the
+// reference's queue.write_buffer): the bytes are copied into pinned staging
+// memory and from there to the device on the stream, so the caller's memory
+// may change as soon as update() returns.
+class UniformBuffer {
+ public:
+  UniformBuffer(Device& dev, size_t bytes);
+  ~UniformBuffer();
+  UniformBuffer(const UniformBuffer&) = delete;
+  UniformBuffer& operator=(const UniformBuffer&) = delete;
+  void update(const void* src);
+  void* dev() const { return buf_.get(); }
+
+ private:
+  Device& device_;
+  DeviceBuffer buf_;
+  size_t bytes_;
+  void* staging_ = nullptr;
+  hipEvent_t done_ = nullptr;
+};
+
+// ---- host camera (src/camera.rs) -------------------------------------------
+
+struct Camera {
+  float pos[3] = {0, 0, 0};
+  float z_start = 0.1f, z_end = 1e4f;
+  float vertical_fov = 1.0471976f;
+  float pitch = 0, yaw = 0;
+  void forward(float out[3]) const;
+  void view(float out[16]) const;
+  void projection(float aspect, float out[16]) const;
+};
+
+void look_to_rh(const float eye[3], const float dir[3], const float up[3], float out[16]);
+void perspective_rh(float fov_y, float aspect, float z_near, float z_far, float out[16]);
+CameraPod camera_pod_new(const Camera& c, uint32_t w, uint32_t h);  // CameraPod::new
+
+// ---- buffers (src/buffer/*) --------------------------------------------------
+
+class CameraBuffer {
+ public:
+  explicit CameraBuffer(Device& dev);
+  void update(const Camera& c, uint32_t w, uint32_t h);
+  void update_with_pod(const CameraPod& pod);
+  const CameraPod* dev() const { return static_cast<const CameraPod*>(buf_.dev()); }
+  const CameraPod& host() const { return pod_; }
+
+ private:
+  UniformBuffer buf_;
+  CameraPod pod_{};
+};
+
+class ModelTransformBuffer {
+ public:
+  explicit ModelTransformBuffer(Device& dev);
+  void update(const float pos[3], const float rot[4], const float scale[3]);
+  const ModelTransformPod* dev() const { return static_cast<const ModelTransformPod*>(buf_.dev()); }
+
+ private:
+  UniformBuffer buf_;
+};
+
+class GaussianTransformBuffer {
+ public:
+  explicit GaussianTransformBuffer(Device& dev);
+  void update(float size, int32_t mode, uint32_t sh_deg, bool no_sh0, float max_std_dev);
+  const GaussianTransformPod* dev() const { return static_cast<const GaussianTransformPod*>(buf_.dev()); }
+
+ private:
+  UniformBuffer buf_;
+};
+
+// GaussiansBuffer<G>: the packed model in HBM (structure of streams).
+class GaussiansBuffer {
+ public:
+  GaussiansBuffer(Device& dev, const gs3d_gaussian* g, uint32_t n, PodKind pod);
+  uint32_t len() const { return n_; }
+  PodKind pod() const { return pod_; }
+  uint64_t size_bytes() const;
+  GaussianStreams streams() const;
+  static uint64_t bytes_per_gaussian(PodKind pod);
+
+ private:
+  PodKind pod_;
+  uint32_t n_;
+  DeviceBuffer pc_, cov_, rs_, sh_;
+};
+
+class IndirectArgsBuffer {
+ public:
+  explicit IndirectArgsBuffer(Device& dev);
+  uint32_t* dev() const { return buf_.as<uint32_t>(); }
+  void read(uint32_t out[4]) const { buf_.download(out, 16); }
+
+ private:
+  DeviceBuffer buf_;
+};
+
+class RadixSortIndirectArgsBuffer {
+ public:
+  explicit RadixSortIndirectArgsBuffer(Device& dev);
+  uint32_t* dev() const { return buf_.as<uint32_t>(); }
+  void read(uint32_t out[3]) const { buf_.download(out, 12); }
+
+ private:
+  DeviceBuffer buf_;
+};
+
+class IndirectIndicesBuffer {
+ public:
+  IndirectIndicesBuffer(Device& dev, uint32_t n);
+  uint32_t* dev() const { return buf_.as<uint32_t>(); }
+  uint32_t len() const { return n_; }
+  void read(uint32_t* out, uint32_t count) const { buf_.download(out, size_t(count) * 4); }
+
+ private:
+  uint32_t n_;
+  DeviceBuffer buf_;
+};
+
+uint32_t depth_buffer_len(uint32_t n);
+
+class GaussiansDepthBuffer {
+ public:
+  GaussiansDepthBuffer(Device& dev, uint32_t n);
+  float* dev() const { return buf_.as<float>(); }
+  uint32_t len() const { return len_; }
+  void read(float* out, uint32_t count) const { buf_.download(out, size_t(count) * 4); }
+
+ private:
+  uint32_t len_;
+  DeviceBuffer buf_;
+};
+
+// Per-Gaussian 2D splat records (no counterpart in the reference: its vertex
+// shader recomputes them for each of the 6 vertices of every instance).
+class SplatsBuffer {
+ public:
+  SplatsBuffer(Device& dev, uint32_t n);
+  Splat2D* dev() const { return buf_.as<Splat2D>(); }
+
+ private:
+  DeviceBuffer buf_;
+};
+
+// Render target (wgpu::Texture).
+class Texture {
+ public:
+  Texture(Device& dev, uint32_t w, uint32_t h, TextureFormat fmt);
+  uint32_t width() const { return w_; }
+  uint32_t height() const { return h_; }
+  TextureFormat format() const { return fmt_; }
+  void* dev() const { return buf_.get(); }
+  size_t bytes() const { return buf_.size(); }
+  void read(void* dst, size_t bytes) const;
+
+ private:
+  Device* device_;
+  uint32_t w_, h_;
+  TextureFormat fmt_;
+  DeviceBuffer buf_;
+};
+
+// ---- operators -------------------------------------------------------------
+
+// Stage timing with HIP events on the device stream.
+class StageTimer {
+ public:
+  explicit StageTimer(Device& dev);
+  ~StageTimer();
+  void set_enabled(bool on) { enabled_ = on; }
+  bool enabled() const { return enabled_; }
+  void mark(int stage, bool end);
+  void read(float ms[GS3D_STAGE_COUNT]);
+
+ private:
+  Device* dev_;
+  bool enabled_ = false;
+  hipEvent_t ev_[GS3D_STAGE_COUNT][2];
+  bool used_[GS3D_STAGE_COUNT];
+};
+
+// Preprocessor (src/preprocessor.rs): per-frame constants ("pre"), the fused
+// projection ("main" + "post" + the vertex stage's per-instance work).
+class Preprocessor {
+ public:
+  Preprocessor(Device& dev, const CameraBuffer& camera, const ModelTransformBuffer& model,
+               const GaussianTransformBuffer& gt, const GaussiansBuffer& gaussians, IndirectArgsBuffer& indirect_args,
+               RadixSortIndirectArgsBuffer& rs_args, IndirectIndicesBuffer& indices, GaussiansDepthBuffer& depth,
+               SplatsBuffer& splats);
+  void preprocess(uint32_t gaussian_count);
+  const FrameConsts* frame_consts() const { return consts_.as<FrameConsts>(); }
+
+ private:
+  Device& dev_;
+  const CameraBuffer& camera_;
+  const ModelTransformBuffer& model_;
+  const GaussianTransformBuffer& gt_;
+  const GaussiansBuffer& gaussians_;
+  IndirectArgsBuffer& indirect_args_;
+  RadixSortIndirectArgsBuffer& rs_args_;
+  IndirectIndicesBuffer& indices_;
+  GaussiansDepthBuffer& depth_;
+  SplatsBuffer& splats_;
+  DeviceBuffer consts_, status_, ticket_;
+  uint32_t max_tiles_;
+};
+
+// Reusable sort scratch for up to `capacity` pairs and `passes` passes.
+class SortScratchBuffers {
+ public:
+  SortScratchBuffers(Device& dev, uint32_t capacity, int passes);
+  SortScratch get() const;
+  uint32_t capacity() const { return capacity_; }
+
+ private:
+  Device& dev_;
+  uint32_t capacity_;
+  int passes_;
+  DeviceBuffer hist_, lookback_, tickets_;
+};
+
+// RadixSorter (src/radix_sorter.rs:9-96): stable ascending (depth key,
+// index) sort in place, count from the RadixSortIndirectArgsBuffer.
+class RadixSorter {
+ public:
+  RadixSorter(Device& dev, GaussiansDepthBuffer& depth, IndirectIndicesBuffer& indices);
+  void sort(const RadixSortIndirectArgsBuffer& rs_args);
+  // Sort an explicit device count (count = *count_dev, at most len).
+  void sort_count(const uint32_t* count_dev, uint32_t count_mul);
+
+ private:
+  Device& dev_;
+  GaussiansDepthBuffer& depth_;
+  IndirectIndicesBuffer& indices_;
+  DeviceBuffer keys_b_, vals_b_;
+  SortScratchBuffers scratch_;
+};
+
+// Renderer (src/renderer.rs): tile binning + compositing into a Texture.
+class Renderer {
+ public:
+  Renderer(Device& dev, TextureFormat format, const CameraBuffer& camera, const ModelTransformBuffer& model,
+           const GaussianTransformBuffer& gt, const GaussiansBuffer& gaussians, const IndirectIndicesBuffer& indices,
+           SplatsBuffer& splats, const FrameConsts* shared_consts);
+  ~Renderer();
+  // Renderer::render; `standalone` recomputes the splats (vertex stage).
+  void render(Texture& target, const IndirectArgsBuffer& indirect_args, bool standalone);
+  void set_blend(BlendMode b) { blend_ = b; }
+  void set_tile_size(uint32_t w, uint32_t h);
+  StageTimer& timer() { return timer_; }
+  void set_timer(StageTimer* t) { ext_timer_ = t; }
+  // Called by Device::poll: true if the last frame overflowed its tile
+  // lists (capacity is grown, the frame must be rendered again).
+  bool grow_if_overflowed();
+  void rerender();
+
+ private:
+  void ensure_capacity(uint32_t pairs);
+  StageTimer& active_timer() { return ext_timer_ ? *ext_timer_ : timer_; }
+
+  Device& dev_;
+  TextureFormat format_;
+  const CameraBuffer& camera_;
+  const ModelTransformBuffer& model_;
+  const GaussianTransformBuffer& gt_;
+  const GaussiansBuffer& gaussians_;
+  const IndirectIndicesBuffer& indices_;
+  SplatsBuffer& splats_;
+  const FrameConsts* shared_consts_;
+  DeviceBuffer own_consts_;
+  BlendMode blend_ = kBlendFast;
+  uint32_t tile_w_ = 16, tile_h_ = 16;
+  uint32_t capacity_ = 0;
+  DeviceBuffer pk_[2], pv_[2];
+  std::unique_ptr<SortScratchBuffers> scratch_;
+  DeviceBuffer ranges_, bin_status_, bin_ticket_, pair_count_;
+  uint32_t ranges_words_ = 0;
+  StageTimer timer_;
+  StageTimer* ext_timer_ = nullptr;
+  // last frame, for re-rendering after an overflow
+  Texture* last_target_ = nullptr;
+  const IndirectArgsBuffer* last_args_ = nullptr;
+  bool last_standalone_ = false;
+};
+
+// Viewer (src/lib.rs:65-276).
+class Viewer {
+ public:
+  Viewer(Device& dev, TextureFormat format, const gs3d_gaussian* g, uint32_t n, PodKind pod);
+  void update_camera(const Camera& c, uint32_t w, uint32_t h) { camera_buffer.update(c, w, h); }
+  void update_camera_with_pod(const CameraPod& pod) { camera_buffer.update_with_pod(pod); }
+  void update_model_transform(const float pos[3], const float rot[4], const float scale[3]) {
+    model_transform_buffer.update(pos, rot, scale);
+  }
+  void update_gaussian_transform(float size, int32_t mode, uint32_t sh_deg, bool no_sh0, float max_std_dev) {
+    gaussian_transform_buffer.update(size, mode, sh_deg, no_sh0, max_std_dev);
+  }
+  void render(Texture& target);
+
+  Device& device;
+  CameraBuffer camera_buffer;
+  ModelTransformBuffer model_transform_buffer;
+  GaussianTransformBuffer gaussian_transform_buffer;
+  GaussiansBuffer gaussians_buffer;
+  IndirectArgsBuffer indirect_args_buffer;
+  RadixSortIndirectArgsBuffer radix_sort_indirect_args_buffer;
+  IndirectIndicesBuffer indirect_indices_buffer;
+  GaussiansDepthBuffer gaussians_depth_buffer;
+  SplatsBuffer splats_buffer;
+  Preprocessor preprocessor;
+  RadixSorter radix_sorter;
+  Renderer renderer;
+  StageTimer timer;
+};
+
+// ---- host-side packing helpers ---------------------------------------------
+
+uint16_t float_to_half(float f);  // round to nearest even
+void gaussian_cov3d(const gs3d_gaussian& g, float out[6]);
+uint32_t pack_color(const uint8_t c[4]);
+
+// Synthetic scenes (synth.cpp).
+void synth_gaussians(int config, uint64_t seed, uint32_t first, uint32_t count, gs3d_gaussian* out);
+
+}  // namespace gs3d
