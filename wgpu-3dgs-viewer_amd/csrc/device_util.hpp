@@ -60,6 +60,7 @@ __device__ __forceinline__ uint32_t lookback_exclusive(uint32_t* status, uint32_
 // Ticket: the order in which workgroups start, so that every predecessor a
 // workgroup waits on is already resident.
 __device__ __forceinline__ uint32_t take_ticket(uint32_t* ticket, uint32_t* s_slot) {
+  __syncthreads();  // every lane has read the previous ticket from s_slot
   if (threadIdx.x == 0) *s_slot = atomicAdd(ticket, 1u);
   __syncthreads();
   const uint32_t t = *s_slot;

@@ -18,8 +18,8 @@ namespace gs3d {
 //               sh   : float4[12] 45 SH floats + 3 pad          192 B
 //   kPodHalf:   rs   : uint4  rot xyzw f16, scale xyz f16, pad   16 B
 //               sh   : uint4[6] 45 SH halves + 3 pad             96 B
-// The culled majority only reads `pc` (and the covariance stream when the
-// centre is off screen).
+// The culled majority only reads `pc` and the covariance stream; the SH
+// stream is read for visible splats only.
 struct GaussianStreams {
   const float4* pc;
   const float2* cov;  // kPodSingle
@@ -32,21 +32,39 @@ struct GaussianStreams {
 __device__ __forceinline__ float half_lo(uint32_t w) { return __half2float(__ushort_as_half((unsigned short)(w & 0xFFFFu))); }
 __device__ __forceinline__ float half_hi(uint32_t w) { return __half2float(__ushort_as_half((unsigned short)(w >> 16))); }
 
+// The covariance stream's raw words for one Gaussian (so that loads can be
+// issued before they are needed).
+struct CovWords {
+  float2 a, b, c;  // kPodSingle
+  uint4 rs;        // kPodHalf
+};
+
 template <int POD>
-__device__ __forceinline__ void load_cov3d(const GaussianStreams& g, uint32_t i, float c[6]) {
+__device__ __forceinline__ CovWords load_cov_words(const GaussianStreams& g, uint32_t i) {
+  CovWords w{};
   if constexpr (POD == kPodSingle) {
     const float2* p = g.cov + size_t(i) * 3;
-    const float2 a = p[0], b = p[1], d = p[2];
-    c[0] = a.x;
-    c[1] = a.y;
-    c[2] = b.x;
-    c[3] = b.y;
-    c[4] = d.x;
-    c[5] = d.y;
+    w.a = p[0];
+    w.b = p[1];
+    w.c = p[2];
   } else {
-    const uint4 w = g.rs[i];
-    const float q[4] = {half_lo(w.x), half_hi(w.x), half_lo(w.y), half_hi(w.y)};
-    const float s[3] = {half_lo(w.z), half_hi(w.z), half_lo(w.w)};
+    w.rs = g.rs[i];
+  }
+  return w;
+}
+
+template <int POD>
+__device__ __forceinline__ void cov3d_of(const CovWords& w, float c[6]) {
+  if constexpr (POD == kPodSingle) {
+    c[0] = w.a.x;
+    c[1] = w.a.y;
+    c[2] = w.b.x;
+    c[3] = w.b.y;
+    c[4] = w.c.x;
+    c[5] = w.c.y;
+  } else {
+    const float q[4] = {half_lo(w.rs.x), half_hi(w.rs.x), half_lo(w.rs.y), half_hi(w.rs.y)};
+    const float s[3] = {half_lo(w.rs.z), half_hi(w.rs.z), half_lo(w.rs.w)};
     cov3d_from_scale_rot(s, q, c);
   }
 }
@@ -82,10 +100,19 @@ __device__ __forceinline__ void load_sh(const GaussianStreams& g, uint32_t i, fl
 
 // render.wesl:75-130 for one instance, in pixel space. `world` is
 // model_transform_mat * (pos, 1); `c` the cov3d.
+//
+// Unlike the cull and depth-key path (math.hpp, bit-exact with the f32
+// oracle), nothing here feeds a bit-exact output: the record only shapes
+// pixels, which are compared within a tolerance. It is therefore evaluated
+// with contraction, hardware reciprocal / square root (1 ulp), and the
+// sparsity of the Jacobian J (utils.wesl:38-42), whose third row and two
+// entries are zero: T = J W SR reduces to two rows, cov2d to three dot
+// products.
 template <int POD>
 __device__ __forceinline__ void compute_splat(const FrameConsts& fc, const GaussianStreams& g, uint32_t i,
                                               const float pos[3], uint32_t color_bits, const float world[4],
                                               const float c[6], Splat2D* out) {
+#pragma clang fp contract(fast)
   float view[4], proj[4];
   m4v(fc.view, world, view);
   m4v(fc.proj, view, proj);
@@ -94,23 +121,22 @@ __device__ __forceinline__ void compute_splat(const FrameConsts& fc, const Gauss
   const float wvd[3] = {fc.campos[0] - world[0], fc.campos[1] - world[1], fc.campos[2] - world[2]};
   float mvd[3];
   m3v(fc.inv_sr, wvd, mvd);
-  const float len = sqrtf((mvd[0] * mvd[0] + mvd[1] * mvd[1]) + mvd[2] * mvd[2]);
-  const float x = -(mvd[0] / len), y = -(mvd[1] / len), z = -(mvd[2] / len);
+  const float il = __builtin_amdgcn_rsqf(mvd[0] * mvd[0] + mvd[1] * mvd[1] + mvd[2] * mvd[2]);
+  const float x = -mvd[0] * il, y = -mvd[1] * il, z = -mvd[2] * il;
   float sh[48];
   if (fc.sh_deg >= 1) load_sh<POD>(g, i, sh);
   float rgb[3];
 #pragma unroll
   for (int ch = 0; ch < 3; ++ch) {
-    const float base = fc.no_sh0 ? 0.5f : float((color_bits >> (8 * ch)) & 0xFFu) / 255.0f;
+    const float base = fc.no_sh0 ? 0.5f : float((color_bits >> (8 * ch)) & 0xFFu) * (1.0f / 255.0f);
     rgb[ch] = view_color_channel(base, x, y, z, fc.sh_deg, [&](int k) { return sh[k * 3 + ch]; });
   }
-  float alpha = float(color_bits >> 24) / 255.0f;
+  float alpha = float(color_bits >> 24) * (1.0f / 255.0f);
 
-  const float ndx = proj[0] / proj[3];
-  const float ndy = proj[1] / proj[3];
+  const float iw = __builtin_amdgcn_rcpf(proj[3]);
   Splat2D s;
-  s.cx = ((ndx + 1.0f) * 0.5f) * fc.size[0];
-  s.cy = ((1.0f - ndy) * 0.5f) * fc.size[1];
+  s.cx = (proj[0] * iw + 1.0f) * 0.5f * fc.size[0];
+  s.cy = (1.0f - proj[1] * iw) * 0.5f * fc.size[1];
   s.ux = s.uy = s.vx = s.vy = 0.0f;
   s.hx = s.hy = 0.0f;
   s.pad[0] = s.pad[1] = s.pad[2] = 0u;
@@ -120,28 +146,68 @@ __device__ __forceinline__ void compute_splat(const FrameConsts& fc, const Gauss
   if (fc.display_mode == kModePoint) {
     // render.wesl:92-104
     const float aspect = fc.size[1] / fc.size[0];
-    const float vlen = sqrtf((view[0] * view[0] + view[1] * view[1]) + view[2] * view[2]);
+    const float ivl = __builtin_amdgcn_rsqf(view[0] * view[0] + view[1] * view[1] + view[2] * view[2]);
     const float q = 0.01f * fc.gsize;
-    s.hx = (((q * aspect) / vlen) * 0.5f) * fc.size[0];
-    s.hy = ((q / vlen) * 0.5f) * fc.size[1];
+    s.hx = q * aspect * ivl * 0.5f * fc.size[0];
+    s.hy = q * ivl * 0.5f * fc.size[1];
     alpha = 1.0f;
     s.drawable = zclip_ok ? 1u : 0u;
   } else {
-    const float sd = fc.max_std_dev;
-    float ax[4];
-    cov2d_axes(fc, pos, c, sd * fc.gsize, ax);
-    const float mm = ax[0] * ax[0] + ax[1] * ax[1];
-    const float nn = ax[2] * ax[2] + ax[3] * ax[3];
-    const bool ok = zclip_ok && (mm > 0.0f) && (nn > 0.0f);
-    s.drawable = ok ? 1u : 0u;
-    if (ok) {
-      s.ux = (2.0f * ax[0]) / mm;
-      s.uy = -((2.0f * ax[1]) / mm);
-      s.vx = (2.0f * ax[2]) / nn;
-      s.vy = -((2.0f * ax[3]) / nn);
-      s.hx = (0.5f * sd) * sqrtf(ax[0] * ax[0] + ax[2] * ax[2]);
-      s.hy = (0.5f * sd) * sqrtf(ax[1] * ax[1] + ax[3] * ax[3]);
+    // cov2d (utils.wesl:25-48) with the zero entries of J dropped
+    const float p4[4] = {pos[0], pos[1], pos[2], 1.0f};
+    float t[4];
+    m4v(fc.vm, p4, t);
+    const float itz = __builtin_amdgcn_rcpf(t[2]);
+    const float j00 = fc.focal[0] * itz, j11 = fc.focal[1] * itz;
+    const float j20 = -fc.focal[0] * t[0] * itz * itz, j21 = -fc.focal[1] * t[1] * itz * itz;
+    // rows of T = J * (W SR): T0 = j00 * R0 + j20 * R2, T1 = j11 * R1 + j21 * R2
+    float t0[3], t1[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      t0[k] = j00 * fc.wsr[k * 3 + 0] + j20 * fc.wsr[k * 3 + 2];
+      t1[k] = j11 * fc.wsr[k * 3 + 1] + j21 * fc.wsr[k * 3 + 2];
     }
+    // Sigma T0^T and Sigma T1^T (Sigma symmetric: xx xy xz yy yz zz)
+    const float s0x = c[0] * t0[0] + c[1] * t0[1] + c[2] * t0[2];
+    const float s0y = c[1] * t0[0] + c[3] * t0[1] + c[4] * t0[2];
+    const float s0z = c[2] * t0[0] + c[4] * t0[1] + c[5] * t0[2];
+    const float s1x = c[0] * t1[0] + c[1] * t1[1] + c[2] * t1[2];
+    const float s1y = c[1] * t1[0] + c[3] * t1[1] + c[4] * t1[2];
+    const float s1z = c[2] * t1[0] + c[4] * t1[1] + c[5] * t1[2];
+    const float ca = t0[0] * s0x + t0[1] * s0y + t0[2] * s0z;
+    const float cb = t1[0] * s0x + t1[1] * s0y + t1[2] * s0z;
+    const float cc = t1[0] * s1x + t1[1] * s1y + t1[2] * s1z;
+    // cov2d_axes (utils.wesl:54-79)
+    const float sd = fc.max_std_dev;
+    const float sds = sd * fc.gsize;
+    const float mid = 0.5f * (ca + cc);
+    const float hd = 0.5f * (ca - cc);
+    const float radius = __builtin_amdgcn_sqrtf(hd * hd + cb * cb);
+    const float lmaj = mid + radius, lmin = mid - radius;
+    bool ok = zclip_ok && lmin >= 0.0f;
+    if (ok) {
+      const float dx0 = cb, dy0 = lmaj - ca;
+      float dx = 0.0f, dy = 1.0f;
+      if (!(dx0 == 0.0f && dy0 == 0.0f)) {
+        const float inv = __builtin_amdgcn_rsqf(dx0 * dx0 + dy0 * dy0);
+        dx = dx0 * inv;
+        dy = dy0 * inv;
+      }
+      const float maj = fminf(sds * __builtin_amdgcn_sqrtf(lmaj), 1024.0f);
+      const float mnr = fminf(sds * __builtin_amdgcn_sqrtf(lmin), 1024.0f);
+      ok = (maj > 0.0f) && (mnr > 0.0f);
+      if (ok) {
+        // qo = 2 (d . (M.x, -M.y)) / |M|^2 with M = maj (dx, dy), m = mnr (dy, -dx)
+        const float im = 2.0f * __builtin_amdgcn_rcpf(maj), in = 2.0f * __builtin_amdgcn_rcpf(mnr);
+        s.ux = dx * im;
+        s.uy = -dy * im;
+        s.vx = dy * in;
+        s.vy = dx * in;
+        s.hx = 0.5f * sd * __builtin_amdgcn_sqrtf(maj * maj * dx * dx + mnr * mnr * dy * dy);
+        s.hy = 0.5f * sd * __builtin_amdgcn_sqrtf(maj * maj * dy * dy + mnr * mnr * dx * dx);
+      }
+    }
+    s.drawable = ok ? 1u : 0u;
   }
   s.r = rgb[0];
   s.g = rgb[1];

@@ -98,6 +98,14 @@ struct CompositeArgs {
   int format;
   int blend;
   void* out;
+  // fast mode work list (see composite.hip)
+  uint32_t capacity;   // pair capacity: bounds the work list
+  uint32_t seg_len;    // splats per segment of a long tile list
+  uint32_t* item_tile; // [composite_max_items]
+  uint32_t* item_seg;  // [composite_max_items]
+  uint32_t* seg_base;  // [tiles] first item of a multi-segment tile
+  uint32_t* plan;      // {items, multi-segment items, work ticket}
+  float4* partial;     // [composite_max_heavy_items][tile pixels]
 };
 
 uint32_t bin_max_tiles(uint32_t max_instances);
@@ -106,6 +114,8 @@ void launch_bin_reset(hipStream_t st, uint32_t* ranges, uint32_t num_tiles, uint
 void launch_bin(hipStream_t st, const BinArgs& a);
 void launch_ranges(hipStream_t st, const uint32_t* pair_count, uint32_t capacity, const uint32_t* keys,
                    uint32_t* ranges, int num_cus);
-void launch_composite(hipStream_t st, const CompositeArgs& a);
+uint32_t composite_max_items(uint32_t num_tiles, uint32_t capacity, uint32_t seg_len);
+uint32_t composite_max_heavy_items(uint32_t capacity, uint32_t seg_len);
+void launch_composite(hipStream_t st, const CompositeArgs& a, int num_cus);
 
 }  // namespace gs3d

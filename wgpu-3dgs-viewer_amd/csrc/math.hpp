@@ -114,6 +114,9 @@ GS3D_HD void derive_frame_consts(const CameraPod& cam, const ModelTransformPod& 
   model_mats(mt, fc.model, fc.sr, fc.inv_sr);
   m4m4(fc.proj, fc.view, fc.pv);
   m4m4(fc.view, fc.model, fc.vm);
+  const float w3[9] = {cam.view[0], cam.view[1], cam.view[2], cam.view[4], cam.view[5],
+                       cam.view[6], cam.view[8], cam.view[9], cam.view[10]};
+  m3m3(w3, fc.sr, fc.wsr);
   for (int i = 0; i < 3; ++i)
     fc.campos[i] = -((cam.view[i * 4 + 0] * cam.view[12] + cam.view[i * 4 + 1] * cam.view[13]) +
                      cam.view[i * 4 + 2] * cam.view[14]);

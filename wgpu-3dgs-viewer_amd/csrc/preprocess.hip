@@ -74,13 +74,27 @@ __global__ __launch_bounds__(kPreThreads) void k_preprocess(PreprocessArgs a) {
   float key[kPreItems];
   uint64_t masks[kPreItems];
 
+  // issue every item's position and covariance loads first: both are needed
+  // by every Gaussian in range (the covariance for the bound re-test of
+  // off-screen centres or for the splat of visible ones)
+  float4 pcs[kPreItems];
+  CovWords cws[kPreItems];
+#pragma unroll
+  for (int k = 0; k < kPreItems; ++k) {
+    const uint32_t i = base + uint32_t(k) * kPreThreads + tid;
+    if (i < a.g.n) {
+      pcs[k] = a.g.pc[i];
+      cws[k] = load_cov_words<POD>(a.g, i);
+    }
+  }
+
 #pragma unroll
   for (int k = 0; k < kPreItems; ++k) {
     const uint32_t i = base + uint32_t(k) * kPreThreads + tid;
     vis[k] = false;
     key[k] = 0.0f;
     if (i < a.g.n) {
-      const float4 pc = a.g.pc[i];
+      const float4 pc = pcs[k];
       const float pos[3] = {pc.x, pc.y, pc.z};
       const float p4[4] = {pc.x, pc.y, pc.z, 1.0f};
       float world[4], clip[4];
@@ -91,12 +105,10 @@ __global__ __launch_bounds__(kPreThreads) void k_preprocess(PreprocessArgs a) {
       const float nz = clip[2] / clip[3];
       bool visible = true;
       float c[6];
-      bool have_cov = false;
+      cov3d_of<POD>(cws[k], c);
       if (cull(nx, ny, nz)) {
         // preprocess.wesl:87-98: re-test the centre moved towards the
         // screen by the projected major axis.
-        load_cov3d<POD>(a.g, i, c);
-        have_cov = true;
         const float sd = fc.max_std_dev;
         float ax[4];
         cov2d_axes(fc, pos, c, sd * fc.gsize, ax);
@@ -111,7 +123,6 @@ __global__ __launch_bounds__(kPreThreads) void k_preprocess(PreprocessArgs a) {
       }
       if (visible) {
         key[k] = 1.0f - nz;  // preprocess.wesl:105
-        if (!have_cov) load_cov3d<POD>(a.g, i, c);
         compute_splat<POD>(fc, a.g, i, pos, __float_as_uint(pc.w), world, c, a.splats + i);
       }
       vis[k] = visible;
@@ -187,7 +198,7 @@ __global__ __launch_bounds__(256) void k_vertex(const FrameConsts* __restrict__ 
     const float p4[4] = {pc.x, pc.y, pc.z, 1.0f};
     float world[4], c[6];
     m4v(fc.model, p4, world);
-    load_cov3d<POD>(g, i, c);
+    cov3d_of<POD>(load_cov_words<POD>(g, i), c);
     compute_splat<POD>(fc, g, i, pos, __float_as_uint(pc.w), world, c, splats + i);
   }
 }

@@ -1,7 +1,6 @@
-// Renderer kernels: tile binning and tile compositing. They replace the
-// reference's draw_indirect of 6 vertices per instance through the
-// fixed-function rasterizer and ALPHA_BLENDING (src/renderer.rs:163-195,
-// 284-308; render.wesl:143-181).
+// Renderer kernels, part 1: tile binning. Together with composite.hip they
+// replace the reference's draw_indirect of 6 vertices per instance through
+// the fixed-function rasterizer (src/renderer.rs:163-195, 284-308).
 //
 //   bin       one thread per drawn instance, in sorted (back-to-front) order:
 //             count the screen tiles its footprint touches, exclusive-scan the
@@ -9,12 +8,6 @@
 //             (tile id, Gaussian index) pairs. Pairs come out in draw order, so
 //             a stable sort by tile id keeps every tile's list in draw order.
 //   ranges    start/end of every tile's run in the tile-sorted pairs.
-//   composite one workgroup per tile, one lane per pixel; splat records are
-//             staged through LDS in batches of one per lane and evaluated by
-//             every pixel. Fast mode walks front to back with transmittance
-//             and stops once every pixel is saturated; exact mode walks back to
-//             front and stores unorm8 after every splat, as blending into an
-//             Rgba8Unorm target does.
 #include <hip/hip_runtime.h>
 
 #include "device_util.hpp"
@@ -26,10 +19,6 @@ namespace gs3d {
 constexpr int kBinThreads = 256;
 constexpr int kBinItems = 4;
 constexpr uint32_t kBinTile = kBinThreads * kBinItems;
-
-// Early-termination threshold on transmittance for the fast mode: what is
-// left behind contributes at most 1e-4 of a splat colour.
-constexpr float kMinTransmittance = 1e-4f;
 
 uint32_t bin_max_tiles(uint32_t max_instances) { return (max_instances + kBinTile - 1) / kBinTile; }
 
@@ -165,155 +154,6 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ pai
   }
 }
 
-__device__ __forceinline__ uint32_t unorm8(float x) {
-  x = fminf(fmaxf(x, 0.0f), 1.0f);
-  return uint32_t(int(x * 255.0f + 0.5f));
-}
-
-__device__ __forceinline__ float clamp01(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
-
-// Fragment stage (render.wesl:143-181) for one staged splat at pixel centre
-// (px, py): the fragment alpha, or -1 when discarded. r0 = (cx, cy, ux, uy),
-// r1 = (vx, vy, r, g), r2 = (b, a, hx, hy).
-__device__ __forceinline__ float frag_alpha(int mode, float sd2, float ell2, const float4& r0, const float4& r1,
-                                            const float4& r2, float px, float py) {
-  const float dx = px - r0.x;
-  const float dy = py - r0.y;
-  if (mode == kModePoint) return (dx >= -r2.z && dx < r2.z && dy >= -r2.w && dy < r2.w) ? 1.0f : -1.0f;
-  const float qx = dx * r0.z + dy * r0.w;
-  const float qy = dx * r1.x + dy * r1.y;
-  const float r2q = qx * qx + qy * qy;
-  if (!(r2q <= sd2)) return -1.0f;
-  if (mode == kModeEllipse) return r2.y + (1.0f - r2.y) * ((r2q > ell2) ? 1.0f : 0.0f);
-  return r2.y * expf(-r2q);
-}
-
-constexpr int kMaxCompositeThreads = 256;
-
-__global__ __launch_bounds__(kMaxCompositeThreads) void k_composite(CompositeArgs a) {
-  __shared__ float4 s_r0[kMaxCompositeThreads];
-  __shared__ float4 s_r1[kMaxCompositeThreads];
-  __shared__ float4 s_r2[kMaxCompositeThreads];
-
-  const uint32_t nt = blockDim.x;
-  const uint32_t tid = threadIdx.x;
-  const uint32_t tile = blockIdx.x;
-  const uint32_t tx = tile % a.grid.tiles_x, ty = tile / a.grid.tiles_x;
-  const uint32_t px = tx * a.grid.tile_w + tid % a.grid.tile_w;
-  const uint32_t py = ty * a.grid.tile_h + tid / a.grid.tile_w;
-  const bool inside = px < a.grid.width && py < a.grid.height;
-  const float fpx = float(px) + 0.5f, fpy = float(py) + 0.5f;
-  const uint32_t start = a.ranges[2 * tile], end = a.ranges[2 * tile + 1];
-  const FrameConsts& fc = *a.fc;
-  const int mode = fc.display_mode;
-  const float sd = fc.max_std_dev;
-  const float sd2 = sd * sd;
-  const float e = sd - 0.1f;
-  const float ell2 = e * e;
-  const bool clamp_src = a.format == kFormatRgba8Unorm;
-
-  if (a.blend == kBlendFast) {
-    float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
-    bool done = !inside;
-    for (uint32_t bend = end; bend > start;) {
-      const uint32_t cnt = (bend - start) < nt ? (bend - start) : nt;
-      __syncthreads();
-      if (tid < cnt) {
-        const uint32_t k = a.pair_vals[bend - 1 - tid];  // nearest first
-        const float4* rp = reinterpret_cast<const float4*>(a.splats + k);
-        s_r0[tid] = rp[0];
-        s_r1[tid] = rp[1];
-        s_r2[tid] = rp[2];
-      }
-      __syncthreads();
-      if (!done) {
-        for (uint32_t q = 0; q < cnt; ++q) {
-          const float4 r0 = s_r0[q], r1 = s_r1[q], r2 = s_r2[q];
-          float al = frag_alpha(mode, sd2, ell2, r0, r1, r2, fpx, fpy);
-          if (al < 0.0f) continue;
-          float sr = r1.z, sg = r1.w, sb = r2.x;
-          if (clamp_src) {
-            sr = clamp01(sr);
-            sg = clamp01(sg);
-            sb = clamp01(sb);
-            al = clamp01(al);
-          }
-          const float wgt = T * al;
-          cr = cr + wgt * sr;
-          cg = cg + wgt * sg;
-          cb = cb + wgt * sb;
-          T = T * (1.0f - al);
-          if (T < kMinTransmittance) {
-            done = true;
-            break;
-          }
-        }
-      }
-      bend -= cnt;
-      if (__syncthreads_and(done ? 1 : 0)) break;
-    }
-    if (inside) {
-      const size_t o = size_t(py) * a.grid.width + px;
-      if (a.format == kFormatRgba8Unorm) {
-        const uint32_t pk = unorm8(cr) | (unorm8(cg) << 8) | (unorm8(cb) << 16) | (255u << 24);
-        reinterpret_cast<uint32_t*>(a.out)[o] = pk;
-      } else {
-        reinterpret_cast<float4*>(a.out)[o] = make_float4(cr, cg, cb, 1.0f);
-      }
-    }
-  } else {
-    // exact: back to front, destination rounded to the target after each splat
-    float dr = 0.0f, dg = 0.0f, db = 0.0f, da = 1.0f;
-    const bool q8 = a.format == kFormatRgba8Unorm;
-    for (uint32_t b0 = start; b0 < end; b0 += nt) {
-      const uint32_t cnt = (end - b0) < nt ? (end - b0) : nt;
-      __syncthreads();
-      if (tid < cnt) {
-        const uint32_t k = a.pair_vals[b0 + tid];  // farthest first
-        const float4* rp = reinterpret_cast<const float4*>(a.splats + k);
-        s_r0[tid] = rp[0];
-        s_r1[tid] = rp[1];
-        s_r2[tid] = rp[2];
-      }
-      __syncthreads();
-      if (inside) {
-        for (uint32_t q = 0; q < cnt; ++q) {
-          const float4 r0 = s_r0[q], r1 = s_r1[q], r2 = s_r2[q];
-          float al = frag_alpha(mode, sd2, ell2, r0, r1, r2, fpx, fpy);
-          if (al < 0.0f) continue;
-          float sr = r1.z, sg = r1.w, sb = r2.x;
-          if (q8) {
-            sr = clamp01(sr);
-            sg = clamp01(sg);
-            sb = clamp01(sb);
-            al = clamp01(al);
-          }
-          const float om = 1.0f - al;
-          dr = sr * al + dr * om;
-          dg = sg * al + dg * om;
-          db = sb * al + db * om;
-          da = al + da * om;
-          if (q8) {
-            dr = float(unorm8(dr)) / 255.0f;
-            dg = float(unorm8(dg)) / 255.0f;
-            db = float(unorm8(db)) / 255.0f;
-            da = float(unorm8(da)) / 255.0f;
-          }
-        }
-      }
-    }
-    if (inside) {
-      const size_t o = size_t(py) * a.grid.width + px;
-      if (q8) {
-        const uint32_t pk = unorm8(dr) | (unorm8(dg) << 8) | (unorm8(db) << 16) | (unorm8(da) << 24);
-        reinterpret_cast<uint32_t*>(a.out)[o] = pk;
-      } else {
-        reinterpret_cast<float4*>(a.out)[o] = make_float4(dr, dg, db, da);
-      }
-    }
-  }
-}
-
 void launch_bin_reset(hipStream_t st, uint32_t* ranges, uint32_t num_tiles, uint32_t* status, uint32_t max_bin_tiles,
                       uint32_t* ticket, uint32_t* pair_count) {
   uint32_t words = num_tiles * 2u;
@@ -337,13 +177,6 @@ void launch_ranges(hipStream_t st, const uint32_t* pair_count, uint32_t capacity
   if (blocks > cap) blocks = cap;
   if (blocks < 1u) blocks = 1u;
   hipLaunchKernelGGL(k_ranges, dim3(blocks), dim3(256), 0, st, pair_count, capacity, keys, ranges);
-}
-
-void launch_composite(hipStream_t st, const CompositeArgs& a) {
-  const uint32_t tiles = a.grid.tiles_x * a.grid.tiles_y;
-  const uint32_t threads = a.grid.tile_w * a.grid.tile_h;
-  if (tiles == 0) return;
-  hipLaunchKernelGGL(k_composite, dim3(tiles), dim3(threads), 0, st, a);
 }
 
 }  // namespace gs3d

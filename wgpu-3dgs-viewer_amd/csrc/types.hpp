@@ -82,6 +82,7 @@ struct FrameConsts {
   float model[16];  // model_transform_mat
   float sr[9];      // model_scale_rot_mat
   float inv_sr[9];  // model_transform_inv_sr_mat
+  float wsr[9];     // mat3(view) * model_scale_rot_mat
   float campos[3];  // render.wesl:59-63
   float size[2];
   float focal[2];   // utils.wesl:35

@@ -614,6 +614,20 @@ void Renderer::render(Texture& target, const IndirectArgsBuffer& indirect_args, 
   tm.mark(GS3D_STAGE_TILE_SORT, true);
 
   tm.mark(GS3D_STAGE_COMPOSITE, false);
+  const uint32_t tile_px = tile_w_ * tile_h_;
+  const size_t items = composite_max_items(num_tiles, capacity_, seg_len_);
+  const size_t heavy = composite_max_heavy_items(capacity_, seg_len_);
+  auto grow = [&](DeviceBuffer& b, size_t bytes) {
+    if (b.size() < bytes) {
+      dev_.wait();
+      b = DeviceBuffer(dev_, bytes);
+    }
+  };
+  grow(item_tile_, items * 4);
+  grow(item_seg_, items * 4);
+  grow(seg_base_, size_t(num_tiles) * 4);
+  grow(plan_, 16);
+  grow(partial_, heavy * tile_px * sizeof(float4));
   CompositeArgs ca{};
   ca.fc = fc;
   ca.pair_vals = pv_[which].as<uint32_t>();
@@ -623,7 +637,14 @@ void Renderer::render(Texture& target, const IndirectArgsBuffer& indirect_args, 
   ca.format = format_;
   ca.blend = blend_;
   ca.out = target.dev();
-  launch_composite(st, ca);
+  ca.capacity = capacity_;
+  ca.seg_len = seg_len_;
+  ca.item_tile = item_tile_.as<uint32_t>();
+  ca.item_seg = item_seg_.as<uint32_t>();
+  ca.seg_base = seg_base_.as<uint32_t>();
+  ca.plan = plan_.as<uint32_t>();
+  ca.partial = partial_.as<float4>();
+  launch_composite(st, ca, dev_.num_cus());
   tm.mark(GS3D_STAGE_COMPOSITE, true);
   hip_check(hipGetLastError(), "launch render");
 

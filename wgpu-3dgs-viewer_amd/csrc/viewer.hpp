@@ -366,6 +366,9 @@ class Renderer {
   std::unique_ptr<SortScratchBuffers> scratch_;
   DeviceBuffer ranges_, bin_status_, bin_ticket_, pair_count_;
   uint32_t ranges_words_ = 0;
+  // compositing work list (fast blend)
+  uint32_t seg_len_ = 512;
+  DeviceBuffer item_tile_, item_seg_, seg_base_, plan_, partial_;
   StageTimer timer_;
   StageTimer* ext_timer_ = nullptr;
   // last frame, for re-rendering after an overflow
