@@ -214,6 +214,112 @@ gs3d_status gs3d_radix_sorter_write(gs3d_radix_sorter* s, const uint32_t* keys, 
 gs3d_status gs3d_radix_sorter_sort(gs3d_radix_sorter* s);
 gs3d_status gs3d_radix_sorter_read(gs3d_radix_sorter* s, uint32_t* keys, uint32_t* vals, uint32_t n);
 
+/* ---- resident models ------------------------------------------------------ */
+
+/* A GaussiansBuffer (core's GaussiansBuffer<G>) kept in HBM and shared by
+ * batched views and served requests. */
+typedef struct gs3d_model gs3d_model;
+gs3d_status gs3d_model_new(gs3d_device* dev, const gs3d_gaussian* gaussians, uint32_t count, gs3d_gaussian_pod pod,
+                           gs3d_model** out);
+void gs3d_model_destroy(gs3d_model* m);
+
+/* ---- MultiModelViewer (src/multi_model.rs:289-531) ------------------------ */
+
+typedef struct gs3d_multi_model_viewer gs3d_multi_model_viewer;
+/* MultiModelViewer::new (src/multi_model.rs:307-317). */
+gs3d_status gs3d_multi_model_viewer_new(gs3d_device* dev, gs3d_texture_format format, gs3d_multi_model_viewer** out);
+void gs3d_multi_model_viewer_destroy(gs3d_multi_model_viewer* v);
+/* MultiModelViewer::insert_model (src/multi_model.rs:353-391). */
+gs3d_status gs3d_multi_model_viewer_insert_model(gs3d_multi_model_viewer* v, const char* key,
+                                                 const gs3d_gaussian* gaussians, uint32_t count,
+                                                 gs3d_gaussian_pod pod);
+/* MultiModelViewer::remove_model (src/multi_model.rs:393-396);
+ * GS3D_ERROR_MODEL_NOT_FOUND for an unknown key. */
+gs3d_status gs3d_multi_model_viewer_remove_model(gs3d_multi_model_viewer* v, const char* key);
+/* MultiModelViewer::update_camera / _with_pod (src/multi_model.rs:398-412). */
+gs3d_status gs3d_multi_model_viewer_update_camera(gs3d_multi_model_viewer* v, const gs3d_camera* camera,
+                                                  uint32_t width, uint32_t height);
+gs3d_status gs3d_multi_model_viewer_update_camera_with_pod(gs3d_multi_model_viewer* v, const gs3d_camera_pod* pod);
+/* MultiModelViewer::update_model_transform (src/multi_model.rs:414-429);
+ * GS3D_ERROR_MODEL_NOT_FOUND for an unknown key. */
+gs3d_status gs3d_multi_model_viewer_update_model_transform(gs3d_multi_model_viewer* v, const char* key,
+                                                           const float pos[3], const float rot[4],
+                                                           const float scale[3]);
+/* MultiModelViewer::update_gaussian_transform (src/multi_model.rs:446-464). */
+gs3d_status gs3d_multi_model_viewer_update_gaussian_transform(gs3d_multi_model_viewer* v, float size,
+                                                              gs3d_display_mode display_mode, uint32_t sh_deg,
+                                                              int no_sh0, float max_std_dev);
+gs3d_status gs3d_multi_model_viewer_set_blend_mode(gs3d_multi_model_viewer* v, gs3d_blend_mode mode);
+/* MultiModelViewer::render (src/multi_model.rs:476-530): the models named by
+ * keys[0..count), drawn in that order into one pass cleared to black. */
+gs3d_status gs3d_multi_model_viewer_render(gs3d_multi_model_viewer* v, gs3d_texture* target,
+                                           const char* const* keys, uint32_t count);
+
+/* ---- batched views (no counterpart in the reference: the batched form) --- */
+
+typedef struct gs3d_view_spec {
+    uint32_t model;  /* index into the batch's model list */
+    uint32_t width;
+    uint32_t height;
+} gs3d_view_spec;
+
+typedef struct gs3d_batch gs3d_batch;
+/* A batch renderer over resident models, spreading its views over `queues`
+ * device queues. */
+gs3d_status gs3d_batch_new(gs3d_device* dev, gs3d_model* const* models, uint32_t model_count, int queues,
+                           gs3d_batch** out);
+void gs3d_batch_destroy(gs3d_batch* b);
+/* One view slot (viewer + RGBA8 target) per view. */
+gs3d_status gs3d_batch_set_layout(gs3d_batch* b, const gs3d_view_spec* views, uint32_t count);
+/* Picks the fastest compositing tile (8x8, 16x16, 32x8) for every view. */
+gs3d_status gs3d_batch_autotune(gs3d_batch* b, const gs3d_camera_pod* pods, int reps);
+/* Renders every view with its camera pod. use_graph: the first call records
+ * the whole batch as one HIP graph, later calls replay it. Enqueue only. */
+gs3d_status gs3d_batch_render(gs3d_batch* b, const gs3d_camera_pod* pods, int use_graph);
+/* Waits for the batch (re-rendering views whose tile lists overflowed). */
+gs3d_status gs3d_batch_finish(gs3d_batch* b);
+gs3d_status gs3d_batch_read_target(gs3d_batch* b, uint32_t view, void* dst, size_t bytes);
+gs3d_status gs3d_batch_tile_size(gs3d_batch* b, uint32_t view, uint32_t* tile_w, uint32_t* tile_h);
+
+/* ---- serving (no counterpart in the reference: the serving form) --------- */
+
+typedef struct gs3d_request {
+    uint32_t model;
+    uint32_t width;
+    uint32_t height;
+    uint32_t padding;
+    double arrival_s;  /* seconds from the start of the run */
+    gs3d_camera_pod pod;
+} gs3d_request;
+
+typedef struct gs3d_serve_stats {
+    uint64_t requests;
+    uint64_t completed;
+    double seconds;       /* first arrival to last completion */
+    double frames_per_s;
+    double p50_ms;        /* latency: arrival to completion */
+    double p99_ms;
+    double mean_ms;
+    double max_ms;
+    double mean_in_flight;
+    uint64_t peak_hbm_bytes;
+    uint64_t deferred_for_memory;
+} gs3d_serve_stats;
+
+typedef struct gs3d_server gs3d_server;
+/* A continuous-batching server over resident models: up to 2 requests per
+ * queue in flight, per-request viewers and targets drawn from pools bounded
+ * by hbm_budget bytes. */
+gs3d_status gs3d_server_new(gs3d_device* dev, gs3d_model* const* models, uint32_t model_count, int queues,
+                            uint64_t hbm_budget, gs3d_server** out);
+void gs3d_server_destroy(gs3d_server* s);
+/* Serves requests[0..count) (sorted by arrival) in real time. images may be
+ * null; otherwise images[i], when non-null, receives request i's RGBA8 frame. */
+gs3d_status gs3d_server_run(gs3d_server* s, const gs3d_request* requests, uint32_t count, uint8_t* const* images,
+                            gs3d_serve_stats* out);
+/* One request with nothing else in flight; its RGBA8 frame into out. */
+gs3d_status gs3d_server_render_alone(gs3d_server* s, const gs3d_request* request, uint8_t* out);
+
 /* ---- host helpers -------------------------------------------------------- */
 
 /* CameraPod::new (src/buffer/camera.rs:72-79) from a Camera. */

@@ -89,7 +89,10 @@ def test_render_config0(gs, orc, device, scene0, blend):
     d = max_abs_diff(img, oimg)
     p = psnr(img, oimg)
     print(f"config0 {blend}: max diff {d}/255, PSNR {p:.2f} dB", flush=True)
-    assert d <= (2 if blend == "fast" else 1)
+    # both modes: 2/255 (the splat records are computed with hardware
+    # reciprocal / square root; in exact mode a 1-ulp change of a fragment
+    # alpha can move one per-step unorm8 rounding)
+    assert d <= 2
     assert p >= 45.0
     assert np.all(img[..., 3] == 255)  # clear alpha 1 stays 1 under ALPHA_BLENDING
 
@@ -110,7 +113,8 @@ def test_render_config0_rgba32f(gs, orc, device):
         _, _, oimg = orc.full_frame(frame, pos, color, sh, cov, w, h, orc.TARGET_RGBA32F, threads=8)
         d = max_abs_diff(img[..., :3], oimg[..., :3])
         print(f"rgba32f blend={int(blend)}: max diff {d:.3e}", flush=True)
-        assert d <= 1e-3
+        # colours are stored as f16 in the splat records (2^-11 relative)
+        assert d <= 2e-3
 
 
 def test_standalone_renderer_matches_fused(gs, orc, device, scene0):

@@ -194,6 +194,30 @@ def _declare(L):
         "gs3d_gaussian_cov3d": ([vp, vp], i32),
         "gs3d_pack_half": ([vp, u32, vp, vp], i32),
         "gs3d_synth_gaussians": ([i32, u64, u32, u32, vp], i32),
+        "gs3d_model_new": ([vp, vp, u32, i32, pp], i32),
+        "gs3d_model_destroy": ([vp], None),
+        "gs3d_multi_model_viewer_new": ([vp, i32, pp], i32),
+        "gs3d_multi_model_viewer_destroy": ([vp], None),
+        "gs3d_multi_model_viewer_insert_model": ([vp, C.c_char_p, vp, u32, i32], i32),
+        "gs3d_multi_model_viewer_remove_model": ([vp, C.c_char_p], i32),
+        "gs3d_multi_model_viewer_update_camera": ([vp, vp, u32, u32], i32),
+        "gs3d_multi_model_viewer_update_camera_with_pod": ([vp, vp], i32),
+        "gs3d_multi_model_viewer_update_model_transform": ([vp, C.c_char_p, vp, vp, vp], i32),
+        "gs3d_multi_model_viewer_update_gaussian_transform": ([vp, f32, i32, u32, i32, f32], i32),
+        "gs3d_multi_model_viewer_set_blend_mode": ([vp, i32], i32),
+        "gs3d_multi_model_viewer_render": ([vp, vp, C.POINTER(C.c_char_p), u32], i32),
+        "gs3d_batch_new": ([vp, C.POINTER(C.c_void_p), u32, i32, pp], i32),
+        "gs3d_batch_destroy": ([vp], None),
+        "gs3d_batch_set_layout": ([vp, vp, u32], i32),
+        "gs3d_batch_autotune": ([vp, vp, i32], i32),
+        "gs3d_batch_render": ([vp, vp, i32], i32),
+        "gs3d_batch_finish": ([vp], i32),
+        "gs3d_batch_read_target": ([vp, u32, vp, C.c_size_t], i32),
+        "gs3d_batch_tile_size": ([vp, u32, vp, vp], i32),
+        "gs3d_server_new": ([vp, C.POINTER(C.c_void_p), u32, i32, u64, pp], i32),
+        "gs3d_server_destroy": ([vp], None),
+        "gs3d_server_run": ([vp, vp, u32, C.POINTER(C.c_void_p), vp], i32),
+        "gs3d_server_render_alone": ([vp, vp, vp], i32),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -213,8 +237,32 @@ EXPORTED_SYMBOLS = (
     "gs3d_viewer_read_indirect_indices", "gs3d_viewer_read_gaussians_depth", "gs3d_depth_buffer_len",
     "gs3d_radix_sorter_new", "gs3d_radix_sorter_destroy", "gs3d_radix_sorter_write", "gs3d_radix_sorter_sort",
     "gs3d_radix_sorter_read", "gs3d_camera_pod_new", "gs3d_camera_pod_look_at", "gs3d_gaussian_cov3d",
-    "gs3d_pack_half", "gs3d_synth_gaussians",
+    "gs3d_pack_half", "gs3d_synth_gaussians", "gs3d_model_new", "gs3d_model_destroy",
+    "gs3d_multi_model_viewer_new", "gs3d_multi_model_viewer_destroy", "gs3d_multi_model_viewer_insert_model",
+    "gs3d_multi_model_viewer_remove_model", "gs3d_multi_model_viewer_update_camera",
+    "gs3d_multi_model_viewer_update_camera_with_pod", "gs3d_multi_model_viewer_update_model_transform",
+    "gs3d_multi_model_viewer_update_gaussian_transform", "gs3d_multi_model_viewer_set_blend_mode",
+    "gs3d_multi_model_viewer_render", "gs3d_batch_new", "gs3d_batch_destroy", "gs3d_batch_set_layout",
+    "gs3d_batch_autotune", "gs3d_batch_render", "gs3d_batch_finish", "gs3d_batch_read_target",
+    "gs3d_batch_tile_size", "gs3d_server_new", "gs3d_server_destroy", "gs3d_server_run",
+    "gs3d_server_render_alone",
 )
+
+# gs3d_request: model, width, height, padding, arrival_s, CameraPod (144 B)
+REQUEST_DTYPE = np.dtype([("model", "<u4"), ("width", "<u4"), ("height", "<u4"), ("padding", "<u4"),
+                          ("arrival_s", "<f8"), ("pod", "u1", (144,))])
+assert REQUEST_DTYPE.itemsize == 168
+VIEW_DTYPE = np.dtype([("model", "<u4"), ("width", "<u4"), ("height", "<u4")])
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("requests", C.c_uint64), ("completed", C.c_uint64), ("seconds", C.c_double),
+                ("frames_per_s", C.c_double), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
+                ("mean_ms", C.c_double), ("max_ms", C.c_double), ("mean_in_flight", C.c_double),
+                ("peak_hbm_bytes", C.c_uint64), ("deferred_for_memory", C.c_uint64)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 def _check(status: int) -> None:
@@ -479,6 +527,193 @@ class RadixSorter:
     def close(self) -> None:
         if self._h:
             lib().gs3d_radix_sorter_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def pods_array(pods) -> np.ndarray:
+    """CameraPods as one contiguous (n, 144) byte array."""
+    return np.frombuffer(b"".join(p.to_bytes() for p in pods), np.uint8).reshape(-1, 144).copy()
+
+
+class Model:
+    """A resident model (GaussiansBuffer) shared by batched views and served requests."""
+
+    def __init__(self, device: Device, gaussians: np.ndarray, pod: GaussianPod = GaussianPod.HALF):
+        g = np.ascontiguousarray(gaussians, GAUSSIAN_DTYPE)
+        h = C.c_void_p()
+        _check(lib().gs3d_model_new(device.handle, _ptr(g) if g.shape[0] else None, int(g.shape[0]), int(pod),
+                                    C.byref(h)))
+        self._h, self.device, self.count, self.pod = h, device, int(g.shape[0]), pod
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib().gs3d_model_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _handles(objs):
+    arr = (C.c_void_p * max(len(objs), 1))(*[o.handle.value if isinstance(o.handle, C.c_void_p) else o.handle
+                                              for o in objs])
+    return arr
+
+
+class MultiModelViewer:
+    """MultiModelViewer (src/multi_model.rs:289-531)."""
+
+    def __init__(self, device: Device, texture_format: TextureFormat = TextureFormat.RGBA8_UNORM):
+        h = C.c_void_p()
+        _check(lib().gs3d_multi_model_viewer_new(device.handle, int(texture_format), C.byref(h)))
+        self._h, self.device = h, device
+
+    def insert_model(self, key: str, gaussians: np.ndarray, pod: GaussianPod = GaussianPod.SINGLE):
+        g = np.ascontiguousarray(gaussians, GAUSSIAN_DTYPE)
+        _check(lib().gs3d_multi_model_viewer_insert_model(self._h, key.encode(), _ptr(g) if g.shape[0] else None,
+                                                          int(g.shape[0]), int(pod)))
+
+    def remove_model(self, key: str):
+        _check(lib().gs3d_multi_model_viewer_remove_model(self._h, key.encode()))
+
+    def update_camera(self, camera: Camera, texture_size):
+        c = camera._c()
+        _check(lib().gs3d_multi_model_viewer_update_camera(self._h, C.byref(c), int(texture_size[0]),
+                                                           int(texture_size[1])))
+
+    def update_camera_with_pod(self, pod: CameraPod):
+        _check(lib().gs3d_multi_model_viewer_update_camera_with_pod(self._h, C.byref(pod)))
+
+    def update_model_transform(self, key: str, pos, rot, scale):
+        p = (C.c_float * 3)(*[float(x) for x in pos])
+        r = (C.c_float * 4)(*[float(x) for x in rot])
+        s = (C.c_float * 3)(*[float(x) for x in scale])
+        _check(lib().gs3d_multi_model_viewer_update_model_transform(self._h, key.encode(), p, r, s))
+
+    def update_gaussian_transform(self, size=1.0, display_mode=DisplayMode.SPLAT, sh_deg=3, no_sh0=False,
+                                  max_std_dev=3.0):
+        _check(lib().gs3d_multi_model_viewer_update_gaussian_transform(self._h, float(size), int(display_mode),
+                                                                       int(sh_deg), int(bool(no_sh0)),
+                                                                       float(max_std_dev)))
+
+    def set_blend_mode(self, mode: BlendMode):
+        _check(lib().gs3d_multi_model_viewer_set_blend_mode(self._h, int(mode)))
+
+    def render(self, target: Texture, keys: Sequence[str]):
+        arr = (C.c_char_p * max(len(keys), 1))(*[k.encode() for k in keys])
+        _check(lib().gs3d_multi_model_viewer_render(self._h, target.handle, arr, len(keys)))
+
+    def close(self):
+        if self._h:
+            lib().gs3d_multi_model_viewer_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Batched views over resident models, recorded as one HIP graph."""
+
+    def __init__(self, device: Device, models: Sequence[Model], queues: int = 4):
+        self._models = list(models)
+        h = C.c_void_p()
+        _check(lib().gs3d_batch_new(device.handle, _handles(self._models), len(self._models), int(queues),
+                                    C.byref(h)))
+        self._h, self.device, self.views = h, device, 0
+        self._sizes = []
+
+    def set_layout(self, views) -> None:
+        """views: sequence of (model index, width, height)."""
+        arr = np.array([tuple(v) for v in views], VIEW_DTYPE)
+        _check(lib().gs3d_batch_set_layout(self._h, _ptr(arr) if len(arr) else None, len(arr)))
+        self.views = len(arr)
+        self._sizes = [(int(v[1]), int(v[2])) for v in views]
+
+    def autotune(self, pods, reps: int = 3) -> None:
+        a = pods_array(pods)
+        _check(lib().gs3d_batch_autotune(self._h, _ptr(a), int(reps)))
+
+    def render(self, pods, use_graph: bool = True) -> None:
+        a = pods_array(pods) if not isinstance(pods, np.ndarray) else np.ascontiguousarray(pods)
+        _check(lib().gs3d_batch_render(self._h, _ptr(a), int(bool(use_graph))))
+
+    def finish(self) -> None:
+        _check(lib().gs3d_batch_finish(self._h))
+
+    def read(self, view: int) -> np.ndarray:
+        w, h = self._sizes[view]
+        out = np.zeros((h, w, 4), np.uint8)
+        _check(lib().gs3d_batch_read_target(self._h, int(view), _ptr(out), out.nbytes))
+        return out
+
+    def tile_size(self, view: int):
+        tw, th = C.c_uint32(), C.c_uint32()
+        _check(lib().gs3d_batch_tile_size(self._h, int(view), C.byref(tw), C.byref(th)))
+        return tw.value, th.value
+
+    def close(self):
+        if self._h:
+            lib().gs3d_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching request loop over resident models."""
+
+    def __init__(self, device: Device, models: Sequence[Model], queues: int = 8, hbm_budget: int = 64 << 30):
+        self._models = list(models)
+        h = C.c_void_p()
+        _check(lib().gs3d_server_new(device.handle, _handles(self._models), len(self._models), int(queues),
+                                     int(hbm_budget), C.byref(h)))
+        self._h, self.device = h, device
+
+    def run(self, requests: np.ndarray, capture=None) -> dict:
+        """requests: REQUEST_DTYPE array sorted by arrival. capture: request
+        indices whose frames to return. Returns (stats dict, {index: image})."""
+        r = np.ascontiguousarray(requests, REQUEST_DTYPE)
+        images = {}
+        ptrs = None
+        if capture:
+            ptrs = (C.c_void_p * len(r))()
+            for i in capture:
+                images[int(i)] = np.zeros((int(r[i]["height"]), int(r[i]["width"]), 4), np.uint8)
+                ptrs[int(i)] = images[int(i)].ctypes.data
+        st = ServeStats()
+        _check(lib().gs3d_server_run(self._h, _ptr(r) if len(r) else None, len(r), ptrs, C.byref(st)))
+        return st.as_dict(), images
+
+    def render_alone(self, request) -> np.ndarray:
+        r = np.ascontiguousarray(np.asarray(request, REQUEST_DTYPE).reshape(1))
+        out = np.zeros((int(r[0]["height"]), int(r[0]["width"]), 4), np.uint8)
+        _check(lib().gs3d_server_render_alone(self._h, _ptr(r), _ptr(out)))
+        return out
+
+    def close(self):
+        if self._h:
+            lib().gs3d_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -9,8 +9,29 @@
 #include <string>
 
 #include "../../include/gs3d.h"
+#include "batch.hpp"
 #include "math.hpp"
 #include "viewer.hpp"
+
+struct gs3d_model {
+  std::shared_ptr<gs3d::GaussiansBuffer> buf;
+};
+
+struct gs3d_multi_model_viewer {
+  gs3d::MultiModelViewer v;
+  gs3d_multi_model_viewer(gs3d::Device& d, gs3d::TextureFormat f) : v(d, f) {}
+};
+
+struct gs3d_batch {
+  gs3d::BatchRenderer b;
+  gs3d_batch(gs3d::Device& d, std::vector<std::shared_ptr<gs3d::GaussiansBuffer>> m, int q) : b(d, std::move(m), q) {}
+};
+
+struct gs3d_server {
+  gs3d::Server s;
+  gs3d_server(gs3d::Device& d, std::vector<std::shared_ptr<gs3d::GaussiansBuffer>> m, int q, uint64_t budget)
+      : s(d, std::move(m), q, budget) {}
+};
 
 struct gs3d_device {
   gs3d::Device dev;
@@ -403,6 +424,258 @@ gs3d_status gs3d_synth_gaussians(int config, uint64_t seed, uint32_t first, uint
     need(config >= 0 && config <= 2, "unknown synthetic config");
     need(count == 0 || out != nullptr, "out is null");
     gs3d::synth_gaussians(config, seed, first, count, out);
+  });
+}
+
+gs3d_status gs3d_model_new(gs3d_device* dev, const gs3d_gaussian* gaussians, uint32_t count, gs3d_gaussian_pod pod,
+                           gs3d_model** out) {
+  return guard([&] {
+    need(dev != nullptr && out != nullptr, "null argument");
+    need(count == 0 || gaussians != nullptr, "gaussians is null");
+    need(count < (1u << 30), "more than 2^30 Gaussians");
+    need(pod == GS3D_POD_SINGLE || pod == GS3D_POD_HALF, "unknown gaussian pod");
+    *out = nullptr;
+    auto m = std::make_unique<gs3d_model>();
+    m->buf = std::make_shared<gs3d::GaussiansBuffer>(dev->dev, gaussians, count,
+                                                     pod == GS3D_POD_SINGLE ? gs3d::kPodSingle : gs3d::kPodHalf);
+    *out = m.release();
+  });
+}
+
+void gs3d_model_destroy(gs3d_model* m) { delete m; }
+
+static std::vector<std::shared_ptr<gs3d::GaussiansBuffer>> model_list(gs3d_model* const* models, uint32_t n) {
+  need(n == 0 || models != nullptr, "models is null");
+  std::vector<std::shared_ptr<gs3d::GaussiansBuffer>> v;
+  for (uint32_t i = 0; i < n; ++i) {
+    need(models[i] != nullptr, "null model");
+    v.push_back(models[i]->buf);
+  }
+  return v;
+}
+
+static gs3d::BlendMode to_blend(gs3d_blend_mode m) {
+  need(m == GS3D_BLEND_FAST || m == GS3D_BLEND_EXACT, "unknown blend mode");
+  return m == GS3D_BLEND_FAST ? gs3d::kBlendFast : gs3d::kBlendExactUnorm8;
+}
+
+static gs3d::Camera to_camera(const gs3d_camera* camera) {
+  gs3d::Camera c;
+  for (int i = 0; i < 3; ++i) c.pos[i] = camera->pos[i];
+  c.z_start = camera->z_start;
+  c.z_end = camera->z_end;
+  c.vertical_fov = camera->vertical_fov;
+  c.pitch = camera->pitch;
+  c.yaw = camera->yaw;
+  return c;
+}
+
+gs3d_status gs3d_multi_model_viewer_new(gs3d_device* dev, gs3d_texture_format format, gs3d_multi_model_viewer** out) {
+  return guard([&] {
+    need(dev != nullptr && out != nullptr, "null argument");
+    *out = nullptr;
+    *out = new gs3d_multi_model_viewer(dev->dev, to_format(format));
+  });
+}
+
+void gs3d_multi_model_viewer_destroy(gs3d_multi_model_viewer* v) { delete v; }
+
+gs3d_status gs3d_multi_model_viewer_insert_model(gs3d_multi_model_viewer* v, const char* key,
+                                                 const gs3d_gaussian* gaussians, uint32_t count,
+                                                 gs3d_gaussian_pod pod) {
+  return guard([&] {
+    need(v != nullptr && key != nullptr, "null argument");
+    need(count == 0 || gaussians != nullptr, "gaussians is null");
+    need(pod == GS3D_POD_SINGLE || pod == GS3D_POD_HALF, "unknown gaussian pod");
+    v->v.insert_model(key, gaussians, count, pod == GS3D_POD_SINGLE ? gs3d::kPodSingle : gs3d::kPodHalf);
+  });
+}
+
+gs3d_status gs3d_multi_model_viewer_remove_model(gs3d_multi_model_viewer* v, const char* key) {
+  return guard([&] {
+    need(v != nullptr && key != nullptr, "null argument");
+    if (!v->v.remove_model(key))
+      throw gs3d::Error(GS3D_ERROR_MODEL_NOT_FOUND, "model with the given key does not exist");
+  });
+}
+
+gs3d_status gs3d_multi_model_viewer_update_camera(gs3d_multi_model_viewer* v, const gs3d_camera* camera,
+                                                  uint32_t width, uint32_t height) {
+  return guard([&] {
+    need(v != nullptr && camera != nullptr, "null argument");
+    need(width > 0 && height > 0, "empty target size");
+    v->v.update_camera(to_camera(camera), width, height);
+  });
+}
+
+gs3d_status gs3d_multi_model_viewer_update_camera_with_pod(gs3d_multi_model_viewer* v, const gs3d_camera_pod* pod) {
+  return guard([&] {
+    need(v != nullptr && pod != nullptr, "null argument");
+    gs3d::CameraPod p;
+    std::memcpy(&p, pod, sizeof p);
+    v->v.update_camera_with_pod(p);
+  });
+}
+
+gs3d_status gs3d_multi_model_viewer_update_model_transform(gs3d_multi_model_viewer* v, const char* key,
+                                                           const float pos[3], const float rot[4],
+                                                           const float scale[3]) {
+  return guard([&] {
+    need(v != nullptr && key != nullptr && pos && rot && scale, "null argument");
+    v->v.update_model_transform(key, pos, rot, scale);
+  });
+}
+
+gs3d_status gs3d_multi_model_viewer_update_gaussian_transform(gs3d_multi_model_viewer* v, float size,
+                                                              gs3d_display_mode display_mode, uint32_t sh_deg,
+                                                              int no_sh0, float max_std_dev) {
+  return guard([&] {
+    need(v != nullptr, "viewer is null");
+    need(display_mode >= GS3D_DISPLAY_SPLAT && display_mode <= GS3D_DISPLAY_POINT, "unknown display mode");
+    need(sh_deg <= 3, "SH degree must be 0..3");
+    need(max_std_dev > 0.0f, "max standard deviation must be positive");
+    v->v.update_gaussian_transform(size, int32_t(display_mode), sh_deg, no_sh0 != 0, max_std_dev);
+  });
+}
+
+gs3d_status gs3d_multi_model_viewer_set_blend_mode(gs3d_multi_model_viewer* v, gs3d_blend_mode mode) {
+  return guard([&] {
+    need(v != nullptr, "viewer is null");
+    v->v.set_blend(to_blend(mode));
+  });
+}
+
+gs3d_status gs3d_multi_model_viewer_render(gs3d_multi_model_viewer* v, gs3d_texture* target,
+                                           const char* const* keys, uint32_t count) {
+  return guard([&] {
+    need(v != nullptr && target != nullptr, "null argument");
+    need(count == 0 || keys != nullptr, "keys is null");
+    std::vector<std::string> k;
+    for (uint32_t i = 0; i < count; ++i) {
+      need(keys[i] != nullptr, "null key");
+      k.emplace_back(keys[i]);
+    }
+    v->v.render(target->tex, k);
+  });
+}
+
+gs3d_status gs3d_batch_new(gs3d_device* dev, gs3d_model* const* models, uint32_t model_count, int queues,
+                           gs3d_batch** out) {
+  return guard([&] {
+    need(dev != nullptr && out != nullptr, "null argument");
+    need(queues >= 1 && queues <= 16, "queues must be 1..16");
+    *out = nullptr;
+    *out = new gs3d_batch(dev->dev, model_list(models, model_count), queues);
+  });
+}
+
+void gs3d_batch_destroy(gs3d_batch* b) { delete b; }
+
+gs3d_status gs3d_batch_set_layout(gs3d_batch* b, const gs3d_view_spec* views, uint32_t count) {
+  return guard([&] {
+    need(b != nullptr && (count == 0 || views != nullptr), "null argument");
+    std::vector<gs3d::ViewSpec> v;
+    for (uint32_t i = 0; i < count; ++i) {
+      need(views[i].width > 0 && views[i].height > 0, "empty view size");
+      v.push_back(gs3d::ViewSpec{views[i].model, views[i].width, views[i].height});
+    }
+    b->b.set_layout(v);
+  });
+}
+
+gs3d_status gs3d_batch_autotune(gs3d_batch* b, const gs3d_camera_pod* pods, int reps) {
+  return guard([&] {
+    need(b != nullptr && (b->b.views() == 0 || pods != nullptr), "null argument");
+    b->b.autotune(reinterpret_cast<const gs3d::CameraPod*>(pods), reps);
+  });
+}
+
+gs3d_status gs3d_batch_render(gs3d_batch* b, const gs3d_camera_pod* pods, int use_graph) {
+  return guard([&] {
+    need(b != nullptr && (b->b.views() == 0 || pods != nullptr), "null argument");
+    b->b.render(reinterpret_cast<const gs3d::CameraPod*>(pods), use_graph != 0);
+  });
+}
+
+gs3d_status gs3d_batch_finish(gs3d_batch* b) {
+  return guard([&] {
+    need(b != nullptr, "batch is null");
+    b->b.finish();
+  });
+}
+
+gs3d_status gs3d_batch_read_target(gs3d_batch* b, uint32_t view, void* dst, size_t bytes) {
+  return guard([&] {
+    need(b != nullptr && dst != nullptr, "null argument");
+    need(view < b->b.views(), "no such view");
+    b->b.finish();
+    b->b.target(view).read(dst, bytes);
+  });
+}
+
+gs3d_status gs3d_batch_tile_size(gs3d_batch* b, uint32_t view, uint32_t* tile_w, uint32_t* tile_h) {
+  return guard([&] {
+    need(b != nullptr && tile_w && tile_h, "null argument");
+    need(view < b->b.views(), "no such view");
+    *tile_w = b->b.viewer(view).renderer.tile_w();
+    *tile_h = b->b.viewer(view).renderer.tile_h();
+  });
+}
+
+static gs3d::ServeRequest to_request(const gs3d_request& r) {
+  gs3d::ServeRequest q;
+  q.model = r.model;
+  q.width = r.width;
+  q.height = r.height;
+  q.arrival_s = r.arrival_s;
+  std::memcpy(&q.pod, &r.pod, sizeof q.pod);
+  return q;
+}
+
+gs3d_status gs3d_server_new(gs3d_device* dev, gs3d_model* const* models, uint32_t model_count, int queues,
+                            uint64_t hbm_budget, gs3d_server** out) {
+  return guard([&] {
+    need(dev != nullptr && out != nullptr, "null argument");
+    need(queues >= 1 && queues <= 16, "queues must be 1..16");
+    *out = nullptr;
+    *out = new gs3d_server(dev->dev, model_list(models, model_count), queues, hbm_budget);
+  });
+}
+
+void gs3d_server_destroy(gs3d_server* s) { delete s; }
+
+gs3d_status gs3d_server_run(gs3d_server* s, const gs3d_request* requests, uint32_t count, uint8_t* const* images,
+                            gs3d_serve_stats* out) {
+  return guard([&] {
+    need(s != nullptr && out != nullptr && (count == 0 || requests != nullptr), "null argument");
+    std::vector<gs3d::ServeRequest> r;
+    r.reserve(count);
+    for (uint32_t i = 0; i < count; ++i) {
+      need(requests[i].width > 0 && requests[i].height > 0, "empty request size");
+      need(i == 0 || requests[i].arrival_s >= requests[i - 1].arrival_s, "requests must be sorted by arrival");
+      r.push_back(to_request(requests[i]));
+    }
+    std::vector<uint8_t*> img;
+    if (images) img.assign(images, images + count);
+    const gs3d::ServeStats st = s->s.run(r, images ? &img : nullptr);
+    out->requests = st.requests;
+    out->completed = st.completed;
+    out->seconds = st.seconds;
+    out->frames_per_s = st.frames_per_s;
+    out->p50_ms = st.p50_ms;
+    out->p99_ms = st.p99_ms;
+    out->mean_ms = st.mean_ms;
+    out->max_ms = st.max_ms;
+    out->mean_in_flight = st.mean_in_flight;
+    out->peak_hbm_bytes = st.peak_hbm_bytes;
+    out->deferred_for_memory = st.deferred_for_memory;
+  });
+}
+
+gs3d_status gs3d_server_render_alone(gs3d_server* s, const gs3d_request* request, uint8_t* out) {
+  return guard([&] {
+    need(s != nullptr && request != nullptr && out != nullptr, "null argument");
+    s->s.render_alone(to_request(*request), out);
   });
 }
 

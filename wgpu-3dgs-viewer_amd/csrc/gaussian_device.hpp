@@ -32,6 +32,25 @@ struct GaussianStreams {
 __device__ __forceinline__ float half_lo(uint32_t w) { return __half2float(__ushort_as_half((unsigned short)(w & 0xFFFFu))); }
 __device__ __forceinline__ float half_hi(uint32_t w) { return __half2float(__ushort_as_half((unsigned short)(w >> 16))); }
 
+__device__ __forceinline__ uint32_t pack_half2(float lo, float hi) {
+  return uint32_t(__half_as_ushort(__float2half(lo))) | (uint32_t(__half_as_ushort(__float2half(hi))) << 16);
+}
+
+// f16 bits of a non-negative extent rounded towards +inf (a conservative
+// bound); NaN or values past the f16 range saturate at 65504.
+__device__ __forceinline__ uint32_t half_up_bits(float x) {
+  if (!(x >= 0.0f)) x = 65504.0f;
+  x = fminf(x, 65504.0f);
+  uint32_t h = __half_as_ushort(__float2half(x));
+  if (__half2float(__ushort_as_half((unsigned short)h)) < x) ++h;
+  if (h == 0u) h = 1u;  // smallest subnormal: a drawable splat never encodes as 0
+  return h;
+}
+
+__device__ __forceinline__ uint32_t pack_half2_up(float lo, float hi) {
+  return half_up_bits(lo) | (half_up_bits(hi) << 16);
+}
+
 // The covariance stream's raw words for one Gaussian (so that loads can be
 // issued before they are needed).
 struct CovWords {
@@ -137,9 +156,9 @@ __device__ __forceinline__ void compute_splat(const FrameConsts& fc, const Gauss
   Splat2D s;
   s.cx = (proj[0] * iw + 1.0f) * 0.5f * fc.size[0];
   s.cy = (1.0f - proj[1] * iw) * 0.5f * fc.size[1];
-  s.ux = s.uy = s.vx = s.vy = 0.0f;
-  s.hx = s.hy = 0.0f;
-  s.pad[0] = s.pad[1] = s.pad[2] = 0u;
+  s.ca = s.cb = s.cc = 0.0f;
+  float hx = 0.0f, hy = 0.0f;
+  bool drawable = false;
   // depth clipping of the rasterizer: 0 <= z_clip <= w_clip
   const bool zclip_ok = (proj[2] >= 0.0f) && (proj[2] <= proj[3]);
 
@@ -148,10 +167,12 @@ __device__ __forceinline__ void compute_splat(const FrameConsts& fc, const Gauss
     const float aspect = fc.size[1] / fc.size[0];
     const float ivl = __builtin_amdgcn_rsqf(view[0] * view[0] + view[1] * view[1] + view[2] * view[2]);
     const float q = 0.01f * fc.gsize;
-    s.hx = q * aspect * ivl * 0.5f * fc.size[0];
-    s.hy = q * ivl * 0.5f * fc.size[1];
+    hx = q * aspect * ivl * 0.5f * fc.size[0];
+    hy = q * ivl * 0.5f * fc.size[1];
+    s.ca = hx;
+    s.cb = hy;
     alpha = 1.0f;
-    s.drawable = zclip_ok ? 1u : 0u;
+    drawable = zclip_ok;
   } else {
     // cov2d (utils.wesl:25-48) with the zero entries of J dropped
     const float p4[4] = {pos[0], pos[1], pos[2], 1.0f};
@@ -197,22 +218,22 @@ __device__ __forceinline__ void compute_splat(const FrameConsts& fc, const Gauss
       const float mnr = fminf(sds * __builtin_amdgcn_sqrtf(lmin), 1024.0f);
       ok = (maj > 0.0f) && (mnr > 0.0f);
       if (ok) {
-        // qo = 2 (d . (M.x, -M.y)) / |M|^2 with M = maj (dx, dy), m = mnr (dy, -dx)
+        // qo = (d.u, d.v) with u = 2 (M.x, -M.y) / |M|^2, v = 2 (m.x, -m.y) / |m|^2,
+        // M = maj (dx, dy), m = mnr (dy, -dx); |qo|^2 as a quadratic form in d
         const float im = 2.0f * __builtin_amdgcn_rcpf(maj), in = 2.0f * __builtin_amdgcn_rcpf(mnr);
-        s.ux = dx * im;
-        s.uy = -dy * im;
-        s.vx = dy * in;
-        s.vy = dx * in;
-        s.hx = 0.5f * sd * __builtin_amdgcn_sqrtf(maj * maj * dx * dx + mnr * mnr * dy * dy);
-        s.hy = 0.5f * sd * __builtin_amdgcn_sqrtf(maj * maj * dy * dy + mnr * mnr * dx * dx);
+        const float ux = dx * im, uy = -dy * im, vx = dy * in, vy = dx * in;
+        s.ca = ux * ux + vx * vx;
+        s.cb = ux * uy + vx * vy;
+        s.cc = uy * uy + vy * vy;
+        hx = 0.5f * sd * __builtin_amdgcn_sqrtf(maj * maj * dx * dx + mnr * mnr * dy * dy);
+        hy = 0.5f * sd * __builtin_amdgcn_sqrtf(maj * maj * dy * dy + mnr * mnr * dx * dx);
       }
     }
-    s.drawable = ok ? 1u : 0u;
+    drawable = ok;
   }
-  s.r = rgb[0];
-  s.g = rgb[1];
-  s.b = rgb[2];
-  s.a = alpha;
+  s.rg = pack_half2(rgb[0], rgb[1]);
+  s.ba = pack_half2(rgb[2], alpha);
+  s.hxy = drawable ? pack_half2_up(hx, hy) : 0u;
   *out = s;
 }
 

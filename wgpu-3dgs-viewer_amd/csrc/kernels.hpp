@@ -97,6 +97,7 @@ struct CompositeArgs {
   TileGrid grid;
   int format;
   int blend;
+  int load;  // blend over the target's contents instead of clearing to black
   void* out;
   // fast mode work list (see composite.hip)
   uint32_t capacity;   // pair capacity: bounds the work list
@@ -112,10 +113,13 @@ uint32_t bin_max_tiles(uint32_t max_instances);
 void launch_bin_reset(hipStream_t st, uint32_t* ranges, uint32_t num_tiles, uint32_t* status, uint32_t max_bin_tiles,
                       uint32_t* ticket, uint32_t* pair_count);
 void launch_bin(hipStream_t st, const BinArgs& a);
+// host_total: device alias of mapped host memory that receives the pair
+// total (or null)
 void launch_ranges(hipStream_t st, const uint32_t* pair_count, uint32_t capacity, const uint32_t* keys,
-                   uint32_t* ranges, int num_cus);
+                   uint32_t* ranges, int num_cus, uint32_t* host_total);
 uint32_t composite_max_items(uint32_t num_tiles, uint32_t capacity, uint32_t seg_len);
 uint32_t composite_max_heavy_items(uint32_t capacity, uint32_t seg_len);
 void launch_composite(hipStream_t st, const CompositeArgs& a, int num_cus);
+void launch_clear(hipStream_t st, void* out, uint32_t pixels, int format);
 
 }  // namespace gs3d

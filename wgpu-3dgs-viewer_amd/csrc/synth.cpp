@@ -1,8 +1,12 @@
 // Synthetic scenes for tests and benchmarks. Counter-based: every random
 // number is splitmix64 of (seed, Gaussian index, field), so any slice of a
 // scene can be generated independently and identically on any host.
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
+#include <thread>
+#include <vector>
 
 #include "viewer.hpp"
 
@@ -57,7 +61,30 @@ void fill_common(const Rng& r, double log_mean, double log_std, gs3d_gaussian& g
 
 }  // namespace
 
+static void synth_range(int config, uint64_t seed, uint32_t first, uint32_t count, gs3d_gaussian* out);
+
 void synth_gaussians(int config, uint64_t seed, uint32_t first, uint32_t count, gs3d_gaussian* out) {
+  // record i depends only on (seed, i): split the range over host threads
+  const uint32_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const uint32_t chunk = 1u << 16;
+  if (count < 2 * chunk || hw == 1) {
+    synth_range(config, seed, first, count, out);
+    return;
+  }
+  std::atomic<uint32_t> next{0};
+  std::vector<std::thread> pool;
+  for (uint32_t t = 0; t < hw; ++t)
+    pool.emplace_back([&]() {
+      for (;;) {
+        const uint32_t b = next.fetch_add(chunk);
+        if (b >= count) break;
+        synth_range(config, seed, first + b, std::min(chunk, count - b), out + b);
+      }
+    });
+  for (auto& th : pool) th.join();
+}
+
+static void synth_range(int config, uint64_t seed, uint32_t first, uint32_t count, gs3d_gaussian* out) {
   const uint64_t key = splitmix64(seed * 0x2545F4914F6CDD1Dull + 0x1234567ull);
   // config 2: 64 anisotropic clusters, drawn from a stream of their own
   double centre[64][3], sd[64][3];

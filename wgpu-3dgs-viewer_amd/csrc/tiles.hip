@@ -39,9 +39,11 @@ __global__ __launch_bounds__(256) void k_bin_reset(uint32_t* ranges, uint32_t ra
 // cx + hx + 1] is a conservative pixel range.
 __device__ __forceinline__ bool splat_tiles(const Splat2D& s, const TileGrid& g, uint32_t& tx0, uint32_t& ty0,
                                             uint32_t& tx1, uint32_t& ty1) {
-  if (!s.drawable) return false;
-  const float fx0 = floorf(s.cx - s.hx - 1.0f), fx1 = floorf(s.cx + s.hx + 1.0f);
-  const float fy0 = floorf(s.cy - s.hy - 1.0f), fy1 = floorf(s.cy + s.hy + 1.0f);
+  if (s.hxy == 0u) return false;  // draws no pixel
+  const float hx = __half2float(__ushort_as_half((unsigned short)(s.hxy & 0xFFFFu)));
+  const float hy = __half2float(__ushort_as_half((unsigned short)(s.hxy >> 16)));
+  const float fx0 = floorf(s.cx - hx - 1.0f), fx1 = floorf(s.cx + hx + 1.0f);
+  const float fy0 = floorf(s.cy - hy - 1.0f), fy1 = floorf(s.cy + hy + 1.0f);
   const float wmax = float(g.width - 1), hmax = float(g.height - 1);
   if (!(fx1 >= 0.0f) || !(fx0 <= wmax) || !(fy1 >= 0.0f) || !(fy0 <= hmax)) return false;
   const uint32_t x0 = uint32_t(fmaxf(fx0, 0.0f)), x1 = uint32_t(fminf(fx1, wmax));
@@ -144,8 +146,11 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(BinArgs a) {
 }
 
 __global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ pair_count, uint32_t capacity,
-                                                const uint32_t* __restrict__ keys, uint32_t* __restrict__ ranges) {
+                                                const uint32_t* __restrict__ keys, uint32_t* __restrict__ ranges,
+                                                uint32_t* host_total) {
   const uint32_t total = *pair_count;
+  // the pair total, for the host to check the lists' capacity without a copy
+  if (host_total && blockIdx.x == 0 && threadIdx.x == 0) *host_total = total;
   const uint32_t p = total < capacity ? total : capacity;
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < p; j += gridDim.x * blockDim.x) {
     const uint32_t k = keys[j];
@@ -171,12 +176,12 @@ void launch_bin(hipStream_t st, const BinArgs& a) {
 }
 
 void launch_ranges(hipStream_t st, const uint32_t* pair_count, uint32_t capacity, const uint32_t* keys,
-                   uint32_t* ranges, int num_cus) {
+                   uint32_t* ranges, int num_cus, uint32_t* host_total) {
   uint32_t blocks = (capacity + 255u) / 256u;
   const uint32_t cap = uint32_t(num_cus) * 8u;
   if (blocks > cap) blocks = cap;
   if (blocks < 1u) blocks = 1u;
-  hipLaunchKernelGGL(k_ranges, dim3(blocks), dim3(256), 0, st, pair_count, capacity, keys, ranges);
+  hipLaunchKernelGGL(k_ranges, dim3(blocks), dim3(256), 0, st, pair_count, capacity, keys, ranges, host_total);
 }
 
 }  // namespace gs3d

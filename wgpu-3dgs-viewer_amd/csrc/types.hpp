@@ -95,18 +95,21 @@ struct FrameConsts {
 
 // Per visible Gaussian, the 2D splat the compositor draws (the pixel-space
 // restatement of what vert_main hands the rasterizer, render.wesl:75-130).
-// Indexed by Gaussian index; 64 bytes so that one record is one aligned
-// 64-byte HBM sector.
+// Indexed by Gaussian index; 32 bytes (one aligned 32-byte HBM sector).
+//   splat / ellipse: |qo|^2 = ca*dx^2 + 2*cb*dx*dy + cc*dy^2 for d = pixel
+//                    centre - (cx, cy), where qo is the interpolated
+//                    quad offset of render.wesl:116-121
+//   point          : ca, cb hold the half extents in pixels (f32)
+// rg, ba: colour and opacity as f16 pairs; hxy: f16 pair of the half extents
+// of the pixel bound, rounded up, 0 when the splat draws no pixel.
 struct alignas(16) Splat2D {
-  float cx, cy;      // centre in pixels (y down)
-  float ux, uy;      // qo.x = dot(p - c, u)
-  float vx, vy;      // qo.y = dot(p - c, v)
-  float r, g, b, a;  // view_color(), alpha = opacity
-  float hx, hy;      // half extents in pixels (binning bound)
-  uint32_t drawable;
-  uint32_t pad[3];
+  float cx, cy;
+  float ca, cb, cc;
+  uint32_t rg;
+  uint32_t ba;
+  uint32_t hxy;
 };
-static_assert(sizeof(Splat2D) == 64, "Splat2D is 64 bytes");
+static_assert(sizeof(Splat2D) == 32, "Splat2D is 32 bytes");
 
 // Look-back status words: 2-bit flag + 30-bit value.
 constexpr uint32_t kFlagAggregate = 1u << 30;

@@ -26,22 +26,39 @@ Device::Device(int ordinal) : ordinal_(ordinal) {
   hip_check(hipGetDeviceProperties(&prop, ordinal), "hipGetDeviceProperties");
   num_cus_ = prop.multiProcessorCount;
   buffer_limit_ = prop.totalGlobalMem;
-  hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+  add_queue();
   hip_check(hipMalloc(&err_, 256), "hipMalloc(err)");
   hip_check(hipMemset(err_, 0, 256), "hipMemset(err)");
 }
 
 Device::~Device() {
-  if (stream_) {
-    hipStreamSynchronize(stream_);
-    hipStreamDestroy(stream_);
+  for (hipStream_t s : streams_) {
+    hipStreamSynchronize(s);
+    hipStreamDestroy(s);
   }
   if (err_) hipFree(err_);
 }
 
+int Device::add_queue() {
+  make_current();
+  hipStream_t s = nullptr;
+  hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+  streams_.push_back(s);
+  return int(streams_.size()) - 1;
+}
+
 void Device::make_current() const { hip_check(hipSetDevice(ordinal_), "hipSetDevice"); }
 
-void Device::wait() { hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
+void Device::wait() {
+  for (hipStream_t s : streams_) hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+uint32_t Device::take_errors() {
+  uint32_t e = 0;
+  hip_check(hipMemcpy(&e, err_, 4, hipMemcpyDeviceToHost), "read error word");
+  if (e) hip_check(hipMemset(err_, 0, 4), "clear error word");
+  return e;
+}
 
 void Device::note_render(Renderer* r) {
   if (std::find(pending_.begin(), pending_.end(), r) == pending_.end()) pending_.push_back(r);
@@ -81,12 +98,16 @@ DeviceBuffer::DeviceBuffer(Device& dev, size_t bytes) : dev_(&dev), bytes_(bytes
   dev.make_current();
   const size_t alloc = bytes ? bytes : 4;
   hip_check(hipMalloc(&ptr_, alloc), "hipMalloc");
+  dev.track(int64_t(alloc));
   hip_check(hipMemsetAsync(ptr_, 0, alloc, dev.stream()), "hipMemsetAsync");
   hip_check(hipStreamSynchronize(dev.stream()), "hipStreamSynchronize");
 }
 
 DeviceBuffer::~DeviceBuffer() {
-  if (ptr_) hipFree(ptr_);
+  if (ptr_) {
+    hipFree(ptr_);
+    dev_->track(-int64_t(bytes_ ? bytes_ : 4));
+  }
 }
 
 DeviceBuffer::DeviceBuffer(DeviceBuffer&& o) noexcept : dev_(o.dev_), ptr_(o.ptr_), bytes_(o.bytes_) {
@@ -96,7 +117,10 @@ DeviceBuffer::DeviceBuffer(DeviceBuffer&& o) noexcept : dev_(o.dev_), ptr_(o.ptr
 
 DeviceBuffer& DeviceBuffer::operator=(DeviceBuffer&& o) noexcept {
   if (this != &o) {
-    if (ptr_) hipFree(ptr_);
+    if (ptr_) {
+      hipFree(ptr_);
+      dev_->track(-int64_t(bytes_ ? bytes_ : 4));
+    }
     dev_ = o.dev_;
     ptr_ = o.ptr_;
     bytes_ = o.bytes_;
@@ -126,10 +150,11 @@ void DeviceBuffer::download(void* dst, size_t bytes, size_t offset) const {
 
 // ---- UniformBuffer ---------------------------------------------------------------
 
-UniformBuffer::UniformBuffer(Device& dev, size_t bytes) : device_(dev), buf_(dev, bytes), bytes_(bytes) {
+UniformBuffer::UniformBuffer(Device& dev, size_t bytes, int queue)
+    : device_(dev), st_(dev.stream(queue)), buf_(dev, bytes), bytes_(bytes) {
   hip_check(hipHostMalloc(&staging_, bytes, hipHostMallocDefault), "hipHostMalloc");
   hip_check(hipEventCreateWithFlags(&done_, hipEventDisableTiming), "hipEventCreate");
-  hip_check(hipEventRecord(done_, dev.stream()), "hipEventRecord");
+  hip_check(hipEventRecord(done_, st_), "hipEventRecord");
 }
 
 UniformBuffer::~UniformBuffer() {
@@ -144,9 +169,8 @@ void UniformBuffer::update(const void* src) {
   // the previous copy out of the staging memory must have finished
   hip_check(hipEventSynchronize(done_), "hipEventSynchronize");
   std::memcpy(staging_, src, bytes_);
-  hip_check(hipMemcpyAsync(buf_.get(), staging_, bytes_, hipMemcpyHostToDevice, device_.stream()),
-            "hipMemcpyAsync(uniform)");
-  hip_check(hipEventRecord(done_, device_.stream()), "hipEventRecord");
+  hip_check(hipMemcpyAsync(buf_.get(), staging_, bytes_, hipMemcpyHostToDevice, st_), "hipMemcpyAsync(uniform)");
+  hip_check(hipEventRecord(done_, st_), "hipEventRecord");
 }
 
 // ---- camera (src/camera.rs:71-92, glam 0.30) --------------------------------------
@@ -205,7 +229,7 @@ CameraPod camera_pod_new(const Camera& c, uint32_t w, uint32_t h) {
 
 // ---- buffers -------------------------------------------------------------------
 
-CameraBuffer::CameraBuffer(Device& dev) : buf_(dev, sizeof(CameraPod)) {}
+CameraBuffer::CameraBuffer(Device& dev, int queue) : buf_(dev, sizeof(CameraPod), queue) {}
 
 void CameraBuffer::update(const Camera& c, uint32_t w, uint32_t h) { update_with_pod(camera_pod_new(c, w, h)); }
 
@@ -214,7 +238,7 @@ void CameraBuffer::update_with_pod(const CameraPod& pod) {
   buf_.update(&pod_);
 }
 
-ModelTransformBuffer::ModelTransformBuffer(Device& dev) : buf_(dev, sizeof(ModelTransformPod)) {
+ModelTransformBuffer::ModelTransformBuffer(Device& dev, int queue) : buf_(dev, sizeof(ModelTransformPod), queue) {
   const float p[3] = {0, 0, 0}, r[4] = {0, 0, 0, 1}, s[3] = {1, 1, 1};
   update(p, r, s);
 }
@@ -229,7 +253,8 @@ void ModelTransformBuffer::update(const float pos[3], const float rot[4], const 
   buf_.update(&t);
 }
 
-GaussianTransformBuffer::GaussianTransformBuffer(Device& dev) : buf_(dev, sizeof(GaussianTransformPod)) {
+GaussianTransformBuffer::GaussianTransformBuffer(Device& dev, int queue)
+    : buf_(dev, sizeof(GaussianTransformPod), queue) {
   update(1.0f, kModeSplat, 3, false, 3.0f);
 }
 
@@ -369,7 +394,7 @@ void Texture::read(void* dst, size_t bytes) const {
 
 // ---- StageTimer ----------------------------------------------------------------
 
-StageTimer::StageTimer(Device& dev) : dev_(&dev) {
+StageTimer::StageTimer(Device& dev, int queue) : dev_(&dev), st_(dev.stream(queue)) {
   for (int s = 0; s < GS3D_STAGE_COUNT; ++s) {
     hip_check(hipEventCreate(&ev_[s][0]), "hipEventCreate");
     hip_check(hipEventCreate(&ev_[s][1]), "hipEventCreate");
@@ -386,7 +411,7 @@ StageTimer::~StageTimer() {
 
 void StageTimer::mark(int stage, bool end) {
   if (!enabled_) return;
-  hip_check(hipEventRecord(ev_[stage][end ? 1 : 0], dev_->stream()), "hipEventRecord");
+  hip_check(hipEventRecord(ev_[stage][end ? 1 : 0], st_), "hipEventRecord");
   if (end) used_[stage] = true;
 }
 
@@ -403,8 +428,10 @@ void StageTimer::read(float ms[GS3D_STAGE_COUNT]) {
 Preprocessor::Preprocessor(Device& dev, const CameraBuffer& camera, const ModelTransformBuffer& model,
                            const GaussianTransformBuffer& gt, const GaussiansBuffer& gaussians,
                            IndirectArgsBuffer& indirect_args, RadixSortIndirectArgsBuffer& rs_args,
-                           IndirectIndicesBuffer& indices, GaussiansDepthBuffer& depth, SplatsBuffer& splats)
+                           IndirectIndicesBuffer& indices, GaussiansDepthBuffer& depth, SplatsBuffer& splats,
+                           int queue)
     : dev_(dev),
+      st_(dev.stream(queue)),
       camera_(camera),
       model_(model),
       gt_(gt),
@@ -426,7 +453,7 @@ Preprocessor::Preprocessor(Device& dev, const CameraBuffer& camera, const ModelT
 void Preprocessor::preprocess(uint32_t gaussian_count) {
   const uint32_t n = std::min(gaussian_count, gaussians_.len());
   const uint32_t tiles = preprocess_num_tiles(n);
-  launch_frame_pre(dev_.stream(), camera_.dev(), model_.dev(), gt_.dev(), consts_.as<FrameConsts>(),
+  launch_frame_pre(st_, camera_.dev(), model_.dev(), gt_.dev(), consts_.as<FrameConsts>(),
                    indirect_args_.dev(), rs_args_.dev(), status_.as<uint32_t>(), tiles, ticket_.as<uint32_t>());
   PreprocessArgs a{};
   a.fc = consts_.as<FrameConsts>();
@@ -442,7 +469,7 @@ void Preprocessor::preprocess(uint32_t gaussian_count) {
   a.ticket = ticket_.as<uint32_t>();
   a.num_tiles = tiles;
   a.err = dev_.err();
-  launch_preprocess(dev_.stream(), gaussians_.pod(), a);
+  launch_preprocess(st_, gaussians_.pod(), a);
   hip_check(hipGetLastError(), "launch preprocess");
 }
 
@@ -466,8 +493,9 @@ SortScratch SortScratchBuffers::get() const {
   return s;
 }
 
-RadixSorter::RadixSorter(Device& dev, GaussiansDepthBuffer& depth, IndirectIndicesBuffer& indices)
+RadixSorter::RadixSorter(Device& dev, GaussiansDepthBuffer& depth, IndirectIndicesBuffer& indices, int queue)
     : dev_(dev),
+      st_(dev.stream(queue)),
       depth_(depth),
       indices_(indices),
       keys_b_(dev, size_t(indices.len()) * 4),
@@ -485,7 +513,7 @@ void RadixSorter::sort_count(const uint32_t* count_dev, uint32_t count_mul) {
   if (n == 0) return;
   SortCount c{count_dev, count_mul, n};
   uint32_t* keys = reinterpret_cast<uint32_t*>(depth_.dev());
-  launch_radix_sort(dev_.stream(), dev_.num_cus(), c, keys, indices_.dev(), keys_b_.as<uint32_t>(),
+  launch_radix_sort(st_, dev_.num_cus(), c, keys, indices_.dev(), keys_b_.as<uint32_t>(),
                     vals_b_.as<uint32_t>(), 4, scratch_.get(), keys, n, depth_.len());
   hip_check(hipGetLastError(), "launch sort");
 }
@@ -494,8 +522,10 @@ void RadixSorter::sort_count(const uint32_t* count_dev, uint32_t count_mul) {
 
 Renderer::Renderer(Device& dev, TextureFormat format, const CameraBuffer& camera, const ModelTransformBuffer& model,
                    const GaussianTransformBuffer& gt, const GaussiansBuffer& gaussians,
-                   const IndirectIndicesBuffer& indices, SplatsBuffer& splats, const FrameConsts* shared_consts)
+                   const IndirectIndicesBuffer& indices, SplatsBuffer& splats, const FrameConsts* shared_consts,
+                   int queue)
     : dev_(dev),
+      st_(dev.stream(queue)),
       format_(format),
       camera_(camera),
       model_(model),
@@ -508,16 +538,26 @@ Renderer::Renderer(Device& dev, TextureFormat format, const CameraBuffer& camera
       bin_status_(dev, size_t(std::max(bin_max_tiles(indices.len()), 1u)) * 4),
       bin_ticket_(dev, 64),
       pair_count_(dev, 64),
-      timer_(dev) {
+      timer_(dev, queue) {
   if (format != kFormatRgba8Unorm && format != kFormatRgba32Float)
     throw Error(GS3D_ERROR_INVALID_ARGUMENT, "unknown texture format");
+  void* mapped = nullptr;
+  hip_check(hipHostMalloc(&mapped, 64, hipHostMallocMapped), "hipHostMalloc(mapped)");
+  pair_total_host_ = static_cast<uint32_t*>(mapped);
+  *pair_total_host_ = 0u;
+  void* alias = nullptr;
+  hip_check(hipHostGetDevicePointer(&alias, mapped, 0), "hipHostGetDevicePointer");
+  pair_total_dev_ = static_cast<uint32_t*>(alias);
   // src/renderer.rs:130-135
   if (gaussians.size_bytes() > dev.buffer_limit())
     throw Error(GS3D_ERROR_MODEL_SIZE_EXCEEDS_DEVICE_LIMIT, "model size exceeds the device limit");
   ensure_capacity(std::max<uint32_t>(1u << 20, std::min<uint64_t>(uint64_t(indices.len()) * 2, 0x3FFFFFFFu)));
 }
 
-Renderer::~Renderer() { dev_.forget_renderer(this); }
+Renderer::~Renderer() {
+  dev_.forget_renderer(this);
+  if (pair_total_host_) hipHostFree(pair_total_host_);
+}
 
 void Renderer::set_tile_size(uint32_t w, uint32_t h) {
   const bool ok = (w == 8 && h == 8) || (w == 16 && h == 16) || (w == 32 && h == 8);
@@ -538,8 +578,8 @@ void Renderer::ensure_capacity(uint32_t pairs) {
 }
 
 bool Renderer::grow_if_overflowed() {
-  uint32_t total = 0;
-  pair_count_.download(&total, 4);
+  // the frame has completed: k_ranges wrote its pair total to mapped memory
+  const uint32_t total = last_pair_total();
   if (total <= capacity_) return false;
   const uint64_t want = std::min<uint64_t>(uint64_t(total) + total / 4 + 1024, 0x3FFFFFFFu);
   ensure_capacity(uint32_t(want));
@@ -547,7 +587,7 @@ bool Renderer::grow_if_overflowed() {
 }
 
 void Renderer::rerender() {
-  if (last_target_ && last_args_) render(*last_target_, *last_args_, last_standalone_);
+  if (last_target_ && last_args_) render(*last_target_, *last_args_, last_standalone_, last_clear_);
 }
 
 static uint32_t ceil_log2(uint32_t x) {
@@ -556,10 +596,10 @@ static uint32_t ceil_log2(uint32_t x) {
   return b;
 }
 
-void Renderer::render(Texture& target, const IndirectArgsBuffer& indirect_args, bool standalone) {
+void Renderer::render(Texture& target, const IndirectArgsBuffer& indirect_args, bool standalone, bool clear) {
   if (target.format() != format_)
     throw Error(GS3D_ERROR_INVALID_ARGUMENT, "render target format differs from the renderer's format");
-  hipStream_t st = dev_.stream();
+  hipStream_t st = st_;
   StageTimer& tm = active_timer();
   const FrameConsts* fc = shared_consts_;
   if (standalone || fc == nullptr) {
@@ -610,7 +650,7 @@ void Renderer::render(Texture& target, const IndirectArgsBuffer& indirect_args, 
   const int which = launch_radix_sort(st, dev_.num_cus(), c, pk_[0].as<uint32_t>(), pv_[0].as<uint32_t>(),
                                       pk_[1].as<uint32_t>(), pv_[1].as<uint32_t>(), passes, scratch_->get());
   launch_ranges(st, pair_count_.as<uint32_t>(), capacity_, pk_[which].as<uint32_t>(), ranges_.as<uint32_t>(),
-                dev_.num_cus());
+                dev_.num_cus(), pair_total_dev_);
   tm.mark(GS3D_STAGE_TILE_SORT, true);
 
   tm.mark(GS3D_STAGE_COMPOSITE, false);
@@ -636,6 +676,7 @@ void Renderer::render(Texture& target, const IndirectArgsBuffer& indirect_args, 
   ca.grid = g;
   ca.format = format_;
   ca.blend = blend_;
+  ca.load = clear ? 0 : 1;
   ca.out = target.dev();
   ca.capacity = capacity_;
   ca.seg_len = seg_len_;
@@ -651,29 +692,35 @@ void Renderer::render(Texture& target, const IndirectArgsBuffer& indirect_args, 
   last_target_ = &target;
   last_args_ = &indirect_args;
   last_standalone_ = standalone;
+  last_clear_ = clear;
   dev_.note_render(this);
 }
 
 // ---- Viewer --------------------------------------------------------------------------
 
 Viewer::Viewer(Device& dev, TextureFormat format, const gs3d_gaussian* g, uint32_t n, PodKind pod)
+    : Viewer(dev, format, std::make_shared<GaussiansBuffer>(dev, g, n, pod), 0) {}
+
+Viewer::Viewer(Device& dev, TextureFormat format, std::shared_ptr<GaussiansBuffer> m, int q)
     : device(dev),
-      camera_buffer(dev),
-      model_transform_buffer(dev),
-      gaussian_transform_buffer(dev),
-      gaussians_buffer(dev, g, n, pod),
+      queue(q),
+      model(std::move(m)),
+      camera_buffer(dev, q),
+      model_transform_buffer(dev, q),
+      gaussian_transform_buffer(dev, q),
+      gaussians_buffer(*model),
       indirect_args_buffer(dev),
       radix_sort_indirect_args_buffer(dev),
-      indirect_indices_buffer(dev, n),
-      gaussians_depth_buffer(dev, n),
-      splats_buffer(dev, n),
+      indirect_indices_buffer(dev, model->len()),
+      gaussians_depth_buffer(dev, model->len()),
+      splats_buffer(dev, model->len()),
       preprocessor(dev, camera_buffer, model_transform_buffer, gaussian_transform_buffer, gaussians_buffer,
                    indirect_args_buffer, radix_sort_indirect_args_buffer, indirect_indices_buffer,
-                   gaussians_depth_buffer, splats_buffer),
-      radix_sorter(dev, gaussians_depth_buffer, indirect_indices_buffer),
+                   gaussians_depth_buffer, splats_buffer, q),
+      radix_sorter(dev, gaussians_depth_buffer, indirect_indices_buffer, q),
       renderer(dev, format, camera_buffer, model_transform_buffer, gaussian_transform_buffer, gaussians_buffer,
-               indirect_indices_buffer, splats_buffer, preprocessor.frame_consts()),
-      timer(dev) {
+               indirect_indices_buffer, splats_buffer, preprocessor.frame_consts(), q),
+      timer(dev, q) {
   renderer.set_timer(&timer);
 }
 

@@ -37,7 +37,11 @@ class Device {
   Device(const Device&) = delete;
   Device& operator=(const Device&) = delete;
 
-  hipStream_t stream() const { return stream_; }
+  // Queue q's stream; queue 0 always exists (all at the default priority).
+  hipStream_t stream(int q = 0) const { return streams_.at(size_t(q)); }
+  // Adds a queue (a new stream) and returns its index.
+  int add_queue();
+  int num_queues() const { return int(streams_.size()); }
   int ordinal() const { return ordinal_; }
   int num_cus() const { return num_cus_; }
   uint64_t buffer_limit() const { return buffer_limit_; }
@@ -50,15 +54,21 @@ class Device {
   // overflowed; throws on kernel-reported errors.
   void poll();
   void wait();
+  // Reads and clears the kernels' error bits; the caller has waited.
+  uint32_t take_errors();
   void note_render(Renderer* r);
   void forget_renderer(Renderer* r);
+  // Bytes held by DeviceBuffers on this device.
+  uint64_t allocated() const { return allocated_; }
+  void track(int64_t delta) { allocated_ = uint64_t(int64_t(allocated_) + delta); }
 
  private:
   int ordinal_ = 0;
   int num_cus_ = 0;
   uint64_t buffer_limit_ = 0;
-  hipStream_t stream_ = nullptr;
+  std::vector<hipStream_t> streams_;
   uint32_t* err_ = nullptr;
+  uint64_t allocated_ = 0;
   std::vector<Renderer*> pending_;
 };
 
@@ -94,7 +104,7 @@ class DeviceBuffer {
 // may change as soon as update() returns.
 class UniformBuffer {
  public:
-  UniformBuffer(Device& dev, size_t bytes);
+  UniformBuffer(Device& dev, size_t bytes, int queue = 0);
   ~UniformBuffer();
   UniformBuffer(const UniformBuffer&) = delete;
   UniformBuffer& operator=(const UniformBuffer&) = delete;
@@ -103,6 +113,7 @@ class UniformBuffer {
 
  private:
   Device& device_;
+  hipStream_t st_;
   DeviceBuffer buf_;
   size_t bytes_;
   void* staging_ = nullptr;
@@ -129,7 +140,7 @@ CameraPod camera_pod_new(const Camera& c, uint32_t w, uint32_t h);  // CameraPod
 
 class CameraBuffer {
  public:
-  explicit CameraBuffer(Device& dev);
+  explicit CameraBuffer(Device& dev, int queue = 0);
   void update(const Camera& c, uint32_t w, uint32_t h);
   void update_with_pod(const CameraPod& pod);
   const CameraPod* dev() const { return static_cast<const CameraPod*>(buf_.dev()); }
@@ -142,7 +153,7 @@ class CameraBuffer {
 
 class ModelTransformBuffer {
  public:
-  explicit ModelTransformBuffer(Device& dev);
+  explicit ModelTransformBuffer(Device& dev, int queue = 0);
   void update(const float pos[3], const float rot[4], const float scale[3]);
   const ModelTransformPod* dev() const { return static_cast<const ModelTransformPod*>(buf_.dev()); }
 
@@ -152,7 +163,7 @@ class ModelTransformBuffer {
 
 class GaussianTransformBuffer {
  public:
-  explicit GaussianTransformBuffer(Device& dev);
+  explicit GaussianTransformBuffer(Device& dev, int queue = 0);
   void update(float size, int32_t mode, uint32_t sh_deg, bool no_sh0, float max_std_dev);
   const GaussianTransformPod* dev() const { return static_cast<const GaussianTransformPod*>(buf_.dev()); }
 
@@ -256,7 +267,7 @@ class Texture {
 // Stage timing with HIP events on the device stream.
 class StageTimer {
  public:
-  explicit StageTimer(Device& dev);
+  explicit StageTimer(Device& dev, int queue = 0);
   ~StageTimer();
   void set_enabled(bool on) { enabled_ = on; }
   bool enabled() const { return enabled_; }
@@ -265,6 +276,7 @@ class StageTimer {
 
  private:
   Device* dev_;
+  hipStream_t st_;
   bool enabled_ = false;
   hipEvent_t ev_[GS3D_STAGE_COUNT][2];
   bool used_[GS3D_STAGE_COUNT];
@@ -277,12 +289,13 @@ class Preprocessor {
   Preprocessor(Device& dev, const CameraBuffer& camera, const ModelTransformBuffer& model,
                const GaussianTransformBuffer& gt, const GaussiansBuffer& gaussians, IndirectArgsBuffer& indirect_args,
                RadixSortIndirectArgsBuffer& rs_args, IndirectIndicesBuffer& indices, GaussiansDepthBuffer& depth,
-               SplatsBuffer& splats);
+               SplatsBuffer& splats, int queue = 0);
   void preprocess(uint32_t gaussian_count);
   const FrameConsts* frame_consts() const { return consts_.as<FrameConsts>(); }
 
  private:
   Device& dev_;
+  hipStream_t st_;
   const CameraBuffer& camera_;
   const ModelTransformBuffer& model_;
   const GaussianTransformBuffer& gt_;
@@ -314,13 +327,14 @@ class SortScratchBuffers {
 // index) sort in place, count from the RadixSortIndirectArgsBuffer.
 class RadixSorter {
  public:
-  RadixSorter(Device& dev, GaussiansDepthBuffer& depth, IndirectIndicesBuffer& indices);
+  RadixSorter(Device& dev, GaussiansDepthBuffer& depth, IndirectIndicesBuffer& indices, int queue = 0);
   void sort(const RadixSortIndirectArgsBuffer& rs_args);
   // Sort an explicit device count (count = *count_dev, at most len).
   void sort_count(const uint32_t* count_dev, uint32_t count_mul);
 
  private:
   Device& dev_;
+  hipStream_t st_;
   GaussiansDepthBuffer& depth_;
   IndirectIndicesBuffer& indices_;
   DeviceBuffer keys_b_, vals_b_;
@@ -332,10 +346,16 @@ class Renderer {
  public:
   Renderer(Device& dev, TextureFormat format, const CameraBuffer& camera, const ModelTransformBuffer& model,
            const GaussianTransformBuffer& gt, const GaussiansBuffer& gaussians, const IndirectIndicesBuffer& indices,
-           SplatsBuffer& splats, const FrameConsts* shared_consts);
+           SplatsBuffer& splats, const FrameConsts* shared_consts, int queue = 0);
   ~Renderer();
   // Renderer::render; `standalone` recomputes the splats (vertex stage).
-  void render(Texture& target, const IndirectArgsBuffer& indirect_args, bool standalone);
+  // clear: LoadOp::Clear(BLACK) (src/renderer.rs:175); otherwise the splats
+  // are blended over what the target holds (LoadOp::Load, as the draws of
+  // several models share one pass, src/multi_model.rs:505-527).
+  void render(Texture& target, const IndirectArgsBuffer& indirect_args, bool standalone, bool clear = true);
+  void set_seg_len(uint32_t s) { seg_len_ = s; }
+  uint32_t tile_w() const { return tile_w_; }
+  uint32_t tile_h() const { return tile_h_; }
   void set_blend(BlendMode b) { blend_ = b; }
   void set_tile_size(uint32_t w, uint32_t h);
   StageTimer& timer() { return timer_; }
@@ -344,12 +364,16 @@ class Renderer {
   // lists (capacity is grown, the frame must be rendered again).
   bool grow_if_overflowed();
   void rerender();
+  // (tile, splat) pairs of the last completed frame, and the list capacity
+  uint32_t last_pair_total() const { return *static_cast<volatile uint32_t*>(pair_total_host_); }
+  uint32_t capacity() const { return capacity_; }
 
  private:
   void ensure_capacity(uint32_t pairs);
   StageTimer& active_timer() { return ext_timer_ ? *ext_timer_ : timer_; }
 
   Device& dev_;
+  hipStream_t st_;
   TextureFormat format_;
   const CameraBuffer& camera_;
   const ModelTransformBuffer& model_;
@@ -369,18 +393,24 @@ class Renderer {
   // compositing work list (fast blend)
   uint32_t seg_len_ = 512;
   DeviceBuffer item_tile_, item_seg_, seg_base_, plan_, partial_;
+  uint32_t* pair_total_host_ = nullptr;  // pinned, mapped
+  uint32_t* pair_total_dev_ = nullptr;   // its device alias
   StageTimer timer_;
   StageTimer* ext_timer_ = nullptr;
   // last frame, for re-rendering after an overflow
   Texture* last_target_ = nullptr;
   const IndirectArgsBuffer* last_args_ = nullptr;
   bool last_standalone_ = false;
+  bool last_clear_ = true;
 };
 
 // Viewer (src/lib.rs:65-276).
 class Viewer {
  public:
   Viewer(Device& dev, TextureFormat format, const gs3d_gaussian* g, uint32_t n, PodKind pod);
+  // A viewer over a resident model shared with other viewers, recording on
+  // the device's queue `queue`.
+  Viewer(Device& dev, TextureFormat format, std::shared_ptr<GaussiansBuffer> model, int queue = 0);
   void update_camera(const Camera& c, uint32_t w, uint32_t h) { camera_buffer.update(c, w, h); }
   void update_camera_with_pod(const CameraPod& pod) { camera_buffer.update_with_pod(pod); }
   void update_model_transform(const float pos[3], const float rot[4], const float scale[3]) {
@@ -392,10 +422,12 @@ class Viewer {
   void render(Texture& target);
 
   Device& device;
+  int queue;
+  std::shared_ptr<GaussiansBuffer> model;
   CameraBuffer camera_buffer;
   ModelTransformBuffer model_transform_buffer;
   GaussianTransformBuffer gaussian_transform_buffer;
-  GaussiansBuffer gaussians_buffer;
+  GaussiansBuffer& gaussians_buffer;
   IndirectArgsBuffer indirect_args_buffer;
   RadixSortIndirectArgsBuffer radix_sort_indirect_args_buffer;
   IndirectIndicesBuffer indirect_indices_buffer;
