@@ -185,6 +185,10 @@ gs3d_status gs3d_viewer_set_tile_size(gs3d_viewer* v, uint32_t tile_w, uint32_t 
 gs3d_status gs3d_viewer_set_profiling(gs3d_viewer* v, int enabled);
 /* Milliseconds of each gs3d_stage in the last profiled frame; waits. */
 gs3d_status gs3d_viewer_stage_times(gs3d_viewer* v, float ms[GS3D_STAGE_COUNT]);
+/* Renders one frame (preprocess, sort, exact-blend compositing) counting the
+ * fragments that pass the discard test of render.wesl:143-166 (the
+ * reference's per-fragment work, for roofline accounting); waits. */
+gs3d_status gs3d_viewer_count_fragments(gs3d_viewer* v, gs3d_texture* target, uint64_t* out);
 
 /* Buffer downloads (the Viewer's pub buffers, src/lib.rs:66-73); each waits
  * for the queue first. */

@@ -178,6 +178,7 @@ def _declare(L):
         "gs3d_viewer_set_tile_size": ([vp, u32, u32], i32),
         "gs3d_viewer_set_profiling": ([vp, i32], i32),
         "gs3d_viewer_stage_times": ([vp, vp], i32),
+        "gs3d_viewer_count_fragments": ([vp, vp, vp], i32),
         "gs3d_viewer_gaussian_count": ([vp, vp], i32),
         "gs3d_viewer_read_indirect_args": ([vp, vp], i32),
         "gs3d_viewer_read_radix_sort_indirect_args": ([vp, vp], i32),
@@ -232,7 +233,8 @@ EXPORTED_SYMBOLS = (
     "gs3d_viewer_update_camera_with_pod", "gs3d_viewer_update_model_transform",
     "gs3d_viewer_update_gaussian_transform", "gs3d_viewer_render", "gs3d_viewer_preprocess", "gs3d_viewer_sort",
     "gs3d_viewer_draw", "gs3d_viewer_draw_standalone", "gs3d_viewer_set_blend_mode", "gs3d_viewer_set_tile_size",
-    "gs3d_viewer_set_profiling", "gs3d_viewer_stage_times", "gs3d_viewer_gaussian_count",
+    "gs3d_viewer_set_profiling", "gs3d_viewer_stage_times", "gs3d_viewer_count_fragments",
+    "gs3d_viewer_gaussian_count",
     "gs3d_viewer_read_indirect_args", "gs3d_viewer_read_radix_sort_indirect_args",
     "gs3d_viewer_read_indirect_indices", "gs3d_viewer_read_gaussians_depth", "gs3d_depth_buffer_len",
     "gs3d_radix_sorter_new", "gs3d_radix_sorter_destroy", "gs3d_radix_sorter_write", "gs3d_radix_sorter_sort",
@@ -463,6 +465,12 @@ class Viewer:
         out = np.zeros(5, np.float32)
         _check(lib().gs3d_viewer_stage_times(self._h, _ptr(out)))
         return out
+
+    def count_fragments(self, target: Texture) -> int:
+        """Fragments of one frame that pass the discard test (waits)."""
+        n = C.c_uint64()
+        _check(lib().gs3d_viewer_count_fragments(self._h, target.handle, C.byref(n)))
+        return n.value
 
     # -- buffer downloads
     def indirect_args(self) -> np.ndarray:

@@ -273,6 +273,15 @@ gs3d_status gs3d_viewer_stage_times(gs3d_viewer* v, float ms[GS3D_STAGE_COUNT]) 
   });
 }
 
+gs3d_status gs3d_viewer_count_fragments(gs3d_viewer* v, gs3d_texture* target, uint64_t* out) {
+  return guard([&] {
+    need(v != nullptr && target != nullptr && out != nullptr, "null argument");
+    v->v.preprocessor.preprocess(v->v.gaussians_buffer.len());
+    v->v.radix_sorter.sort(v->v.radix_sort_indirect_args_buffer);
+    *out = v->v.renderer.count_fragments(target->tex, v->v.indirect_args_buffer);
+  });
+}
+
 gs3d_status gs3d_viewer_gaussian_count(gs3d_viewer* v, uint32_t* out) {
   return guard([&] {
     need(v != nullptr && out != nullptr, "null argument");

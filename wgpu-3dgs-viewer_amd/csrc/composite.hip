@@ -349,6 +349,7 @@ __global__ __launch_bounds__(kCompMaxThreads) void k_composite_exact(CompositeAr
   const bool q8 = a.format == kFormatRgba8Unorm;
   const uint32_t nt = blockDim.x, tid = threadIdx.x, w = tid / kWave;
 
+  uint32_t frags = 0;
   float dr = 0.0f, dg = 0.0f, db = 0.0f, da = 1.0f;
   if (a.load && t.inside) {
     const size_t o = size_t(t.py) * a.grid.width + t.px;
@@ -378,6 +379,7 @@ __global__ __launch_bounds__(kCompMaxThreads) void k_composite_exact(CompositeAr
       float4 col;
       const float al = fragment(s, q, t, mode, sd2l, ell2l, col);
       if (al < 0.0f) continue;
+      ++frags;
       const float om = 1.0f - al;
       dr = col.z * al + dr * om;
       dg = col.w * al + dg * om;
@@ -397,6 +399,7 @@ __global__ __launch_bounds__(kCompMaxThreads) void k_composite_exact(CompositeAr
       reinterpret_cast<uint32_t*>(a.out)[o] = unorm8(dr) | (unorm8(dg) << 8) | (unorm8(db) << 16) | (unorm8(da) << 24);
     else
       reinterpret_cast<float4*>(a.out)[o] = make_float4(dr, dg, db, da);
+    if (a.frag_count && frags) atomicAdd(a.frag_count, frags);
   }
 }
 

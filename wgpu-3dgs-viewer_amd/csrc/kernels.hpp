@@ -98,6 +98,7 @@ struct CompositeArgs {
   int format;
   int blend;
   int load;  // blend over the target's contents instead of clearing to black
+  uint32_t* frag_count;  // exact mode only: counts the non-discarded fragments (or null)
   void* out;
   // fast mode work list (see composite.hip)
   uint32_t capacity;   // pair capacity: bounds the work list

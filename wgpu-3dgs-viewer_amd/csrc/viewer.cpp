@@ -586,6 +586,29 @@ bool Renderer::grow_if_overflowed() {
   return true;
 }
 
+uint64_t Renderer::count_fragments(Texture& target, const IndirectArgsBuffer& indirect_args) {
+  if (frag_count_.size() == 0) frag_count_ = DeviceBuffer(dev_, 64);
+  const uint32_t zero = 0;
+  hip_check(hipMemcpyAsync(frag_count_.get(), &zero, 4, hipMemcpyHostToDevice, st_), "hipMemcpyAsync");
+  const BlendMode old = blend_;
+  blend_ = kBlendExactUnorm8;
+  counting_ = true;
+  try {
+    render(target, indirect_args, false);
+  } catch (...) {
+    blend_ = old;
+    counting_ = false;
+    throw;
+  }
+  blend_ = old;
+  counting_ = false;
+  dev_.poll();
+  uint32_t n = 0;
+  hip_check(hipMemcpyAsync(&n, frag_count_.get(), 4, hipMemcpyDeviceToHost, st_), "hipMemcpyAsync");
+  hip_check(hipStreamSynchronize(st_), "hipStreamSynchronize");
+  return n;
+}
+
 void Renderer::rerender() {
   if (last_target_ && last_args_) render(*last_target_, *last_args_, last_standalone_, last_clear_);
 }
@@ -677,6 +700,7 @@ void Renderer::render(Texture& target, const IndirectArgsBuffer& indirect_args, 
   ca.format = format_;
   ca.blend = blend_;
   ca.load = clear ? 0 : 1;
+  ca.frag_count = counting_ ? frag_count_.as<uint32_t>() : nullptr;
   ca.out = target.dev();
   ca.capacity = capacity_;
   ca.seg_len = seg_len_;

@@ -354,6 +354,9 @@ class Renderer {
   // several models share one pass, src/multi_model.rs:505-527).
   void render(Texture& target, const IndirectArgsBuffer& indirect_args, bool standalone, bool clear = true);
   void set_seg_len(uint32_t s) { seg_len_ = s; }
+  // Renders once in exact mode counting the fragments that are not
+  // discarded (the reference's per-fragment work); waits.
+  uint64_t count_fragments(Texture& target, const IndirectArgsBuffer& indirect_args);
   uint32_t tile_w() const { return tile_w_; }
   uint32_t tile_h() const { return tile_h_; }
   void set_blend(BlendMode b) { blend_ = b; }
@@ -395,6 +398,8 @@ class Renderer {
   DeviceBuffer item_tile_, item_seg_, seg_base_, plan_, partial_;
   uint32_t* pair_total_host_ = nullptr;  // pinned, mapped
   uint32_t* pair_total_dev_ = nullptr;   // its device alias
+  DeviceBuffer frag_count_;
+  bool counting_ = false;
   StageTimer timer_;
   StageTimer* ext_timer_ = nullptr;
   // last frame, for re-rendering after an overflow
