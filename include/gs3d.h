@@ -111,7 +111,9 @@ typedef enum gs3d_stage {
     GS3D_STAGE_BIN = 2,
     GS3D_STAGE_TILE_SORT = 3,
     GS3D_STAGE_COMPOSITE = 4,
-    GS3D_STAGE_COUNT = 5
+    /* the main compositing kernel alone, an interval inside COMPOSITE */
+    GS3D_STAGE_COMPOSITE_MAIN = 5,
+    GS3D_STAGE_COUNT = 6
 } gs3d_stage;
 
 typedef struct gs3d_device gs3d_device;
@@ -337,6 +339,18 @@ gs3d_status gs3d_gaussian_cov3d(const gs3d_gaussian* g, float out[6]);
 /* The f16 pod's rotation/scale (8 halves per Gaussian: rot xyzw, scale xyz,
  * 0) and SH (48 halves: 45 coefficients, 3 zeros) as uploaded. */
 gs3d_status gs3d_pack_half(const gs3d_gaussian* g, uint32_t count, uint16_t* rot_scale, uint16_t* sh);
+
+/* PLY ingestion, replacing wgpu_3dgs_core::Gaussians::read_from_file with
+ * GaussiansSource::Ply (examples/simple.rs:157-160, examples/multi_model.rs:
+ * 131-134): the standard 3DGS vertex layout (x y z, f_dc_0..2, f_rest_*,
+ * opacity, scale_0..2, rot_0..3; other properties ignored), binary
+ * little-endian or ASCII. With out == NULL, *count receives the number of
+ * Gaussians; otherwise capacity must be at least that and *count receives the
+ * number written. rot_0 is w; scale and opacity are stored as log and logit;
+ * rgb = 0.5 + SH_C0 * f_dc; f_rest is channel-major. */
+gs3d_status gs3d_ply_read_file(const char* path, gs3d_gaussian* out, uint64_t capacity, uint64_t* count);
+gs3d_status gs3d_ply_read_memory(const void* data, uint64_t bytes, gs3d_gaussian* out, uint64_t capacity,
+                                 uint64_t* count);
 
 /* Synthetic scenes. config 0: means U[-1,1]^3; 1: means U[-4,4]^3 (both
  * log-scale N(-4,0.5)); 2: 64 anisotropic clusters + 20% on the plane

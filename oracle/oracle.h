@@ -29,7 +29,7 @@
  * path) and double ("_f64", the FP64 host oracle used to state the error of
  * the FP16 storage form).
  *
- * Parity status per function (how each is pinned is in tests/test_oracle_*.py):
+ * Parity status per function (how each is pinned is in tests/test_oracle.py):
  *   orc_camera_*            pinned by closed forms (orthonormal view, eye maps
  *                           to origin, known projection entries)
  *   orc_cov3d_*             pinned against numpy R S S^T R^T in float64

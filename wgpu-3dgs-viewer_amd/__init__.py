@@ -195,6 +195,8 @@ def _declare(L):
         "gs3d_gaussian_cov3d": ([vp, vp], i32),
         "gs3d_pack_half": ([vp, u32, vp, vp], i32),
         "gs3d_synth_gaussians": ([i32, u64, u32, u32, vp], i32),
+        "gs3d_ply_read_file": ([C.c_char_p, vp, u64, vp], i32),
+        "gs3d_ply_read_memory": ([vp, u64, vp, u64, vp], i32),
         "gs3d_model_new": ([vp, vp, u32, i32, pp], i32),
         "gs3d_model_destroy": ([vp], None),
         "gs3d_multi_model_viewer_new": ([vp, i32, pp], i32),
@@ -239,7 +241,7 @@ EXPORTED_SYMBOLS = (
     "gs3d_viewer_read_indirect_indices", "gs3d_viewer_read_gaussians_depth", "gs3d_depth_buffer_len",
     "gs3d_radix_sorter_new", "gs3d_radix_sorter_destroy", "gs3d_radix_sorter_write", "gs3d_radix_sorter_sort",
     "gs3d_radix_sorter_read", "gs3d_camera_pod_new", "gs3d_camera_pod_look_at", "gs3d_gaussian_cov3d",
-    "gs3d_pack_half", "gs3d_synth_gaussians", "gs3d_model_new", "gs3d_model_destroy",
+    "gs3d_pack_half", "gs3d_synth_gaussians", "gs3d_ply_read_file", "gs3d_ply_read_memory", "gs3d_model_new", "gs3d_model_destroy",
     "gs3d_multi_model_viewer_new", "gs3d_multi_model_viewer_destroy", "gs3d_multi_model_viewer_insert_model",
     "gs3d_multi_model_viewer_remove_model", "gs3d_multi_model_viewer_update_camera",
     "gs3d_multi_model_viewer_update_camera_with_pod", "gs3d_multi_model_viewer_update_model_transform",
@@ -290,6 +292,25 @@ def synth_gaussians(config: int, n: int, seed: int, first: int = 0) -> np.ndarra
     out = np.zeros(n, GAUSSIAN_DTYPE)
     if n:
         _check(lib().gs3d_synth_gaussians(int(config), int(seed), int(first), int(n), _ptr(out)))
+    return out
+
+
+def read_ply(source) -> np.ndarray:
+    """Gaussians::read_from_file(path, GaussiansSource::Ply) (examples/simple.rs:157-160):
+    a standard 3DGS PLY file (path) or its bytes, as gs3d_gaussian records."""
+    n = C.c_uint64()
+    if isinstance(source, (bytes, bytearray, memoryview, np.ndarray)):
+        buf = np.frombuffer(bytes(source), np.uint8)
+        _check(lib().gs3d_ply_read_memory(_ptr(buf), buf.nbytes, None, 0, C.byref(n)))
+        out = np.zeros(n.value, GAUSSIAN_DTYPE)
+        _check(lib().gs3d_ply_read_memory(_ptr(buf), buf.nbytes, _ptr(out) if n.value else None, n.value,
+                                          C.byref(n)))
+        return out
+    path = os.fsencode(source)
+    _check(lib().gs3d_ply_read_file(path, None, 0, C.byref(n)))
+    out = np.zeros(n.value, GAUSSIAN_DTYPE)
+    if n.value:
+        _check(lib().gs3d_ply_read_file(path, _ptr(out), n.value, C.byref(n)))
     return out
 
 
@@ -462,7 +483,7 @@ class Viewer:
         _check(lib().gs3d_viewer_set_profiling(self._h, int(bool(on))))
 
     def stage_times(self) -> np.ndarray:
-        out = np.zeros(5, np.float32)
+        out = np.zeros(6, np.float32)
         _check(lib().gs3d_viewer_stage_times(self._h, _ptr(out)))
         return out
 

@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "libgs3d.so")
 ARCH = os.environ.get("GS3D_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["preprocess.hip", "radix_sort.hip", "tiles.hip", "composite.hip", "viewer.cpp", "batch.cpp",
-           "capi.cpp", "synth.cpp"]
+           "capi.cpp", "synth.cpp", "ply.cpp"]
 
 # -ffp-contract=off: the splat arithmetic follows a fixed operation order
 # (csrc/math.hpp) so that depth keys and visibility match the f32 oracle bit

@@ -432,7 +432,8 @@ uint32_t composite_max_heavy_items(uint32_t capacity, uint32_t seg_len) {
   return 2u * (capacity / seg_len) + 2u;
 }
 
-void launch_composite(hipStream_t st, const CompositeArgs& a, int num_cus) {
+void launch_composite(hipStream_t st, const CompositeArgs& a, int num_cus, hipEvent_t main_begin,
+                      hipEvent_t main_end) {
   const uint32_t tiles = a.grid.tiles_x * a.grid.tiles_y;
   const uint32_t threads = a.grid.tile_w * a.grid.tile_h;
   if (tiles == 0) return;
@@ -442,10 +443,14 @@ void launch_composite(hipStream_t st, const CompositeArgs& a, int num_cus) {
     uint32_t grid = uint32_t(num_cus) * 8u * (kCompMaxThreads / threads);
     const uint32_t cap = composite_max_items(tiles, a.capacity, a.seg_len);
     if (grid > cap) grid = cap;
+    if (main_begin) (void)hipEventRecord(main_begin, st);
     hipLaunchKernelGGL(k_composite_fast, dim3(grid), dim3(threads), 0, st, a);
+    if (main_end) (void)hipEventRecord(main_end, st);
     hipLaunchKernelGGL(k_merge, dim3(tiles), dim3(threads), 0, st, a);
   } else {
+    if (main_begin) (void)hipEventRecord(main_begin, st);
     hipLaunchKernelGGL(k_composite_exact, dim3(tiles), dim3(threads), 0, st, a);
+    if (main_end) (void)hipEventRecord(main_end, st);
   }
 }
 

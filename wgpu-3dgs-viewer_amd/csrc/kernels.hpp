@@ -120,7 +120,10 @@ void launch_ranges(hipStream_t st, const uint32_t* pair_count, uint32_t capacity
                    uint32_t* ranges, int num_cus, uint32_t* host_total);
 uint32_t composite_max_items(uint32_t num_tiles, uint32_t capacity, uint32_t seg_len);
 uint32_t composite_max_heavy_items(uint32_t capacity, uint32_t seg_len);
-void launch_composite(hipStream_t st, const CompositeArgs& a, int num_cus);
+// main_begin/main_end, when not null, are recorded around the main
+// compositing kernel (for per-kernel timing).
+void launch_composite(hipStream_t st, const CompositeArgs& a, int num_cus, hipEvent_t main_begin = nullptr,
+                      hipEvent_t main_end = nullptr);
 void launch_clear(hipStream_t st, void* out, uint32_t pixels, int format);
 
 }  // namespace gs3d

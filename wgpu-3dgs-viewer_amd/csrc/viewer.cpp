@@ -415,6 +415,12 @@ void StageTimer::mark(int stage, bool end) {
   if (end) used_[stage] = true;
 }
 
+hipEvent_t StageTimer::event(int stage, bool end) {
+  if (!enabled_) return nullptr;
+  if (end) used_[stage] = true;
+  return ev_[stage][end ? 1 : 0];
+}
+
 void StageTimer::read(float ms[GS3D_STAGE_COUNT]) {
   dev_->wait();
   for (int s = 0; s < GS3D_STAGE_COUNT; ++s) {
@@ -588,8 +594,7 @@ bool Renderer::grow_if_overflowed() {
 
 uint64_t Renderer::count_fragments(Texture& target, const IndirectArgsBuffer& indirect_args) {
   if (frag_count_.size() == 0) frag_count_ = DeviceBuffer(dev_, 64);
-  const uint32_t zero = 0;
-  hip_check(hipMemcpyAsync(frag_count_.get(), &zero, 4, hipMemcpyHostToDevice, st_), "hipMemcpyAsync");
+  hip_check(hipMemsetAsync(frag_count_.get(), 0, 4, st_), "hipMemsetAsync");
   const BlendMode old = blend_;
   blend_ = kBlendExactUnorm8;
   counting_ = true;
@@ -709,7 +714,8 @@ void Renderer::render(Texture& target, const IndirectArgsBuffer& indirect_args, 
   ca.seg_base = seg_base_.as<uint32_t>();
   ca.plan = plan_.as<uint32_t>();
   ca.partial = partial_.as<float4>();
-  launch_composite(st, ca, dev_.num_cus());
+  launch_composite(st, ca, dev_.num_cus(), tm.event(GS3D_STAGE_COMPOSITE_MAIN, false),
+                   tm.event(GS3D_STAGE_COMPOSITE_MAIN, true));
   tm.mark(GS3D_STAGE_COMPOSITE, true);
   hip_check(hipGetLastError(), "launch render");
 

@@ -272,6 +272,9 @@ class StageTimer {
   void set_enabled(bool on) { enabled_ = on; }
   bool enabled() const { return enabled_; }
   void mark(int stage, bool end);
+  // The event pair of a stage for a launcher to record itself (null when
+  // timing is off); marks the stage as recorded.
+  hipEvent_t event(int stage, bool end);
   void read(float ms[GS3D_STAGE_COUNT]);
 
  private:
@@ -449,6 +452,11 @@ class Viewer {
 uint16_t float_to_half(float f);  // round to nearest even
 void gaussian_cov3d(const gs3d_gaussian& g, float out[6]);
 uint32_t pack_color(const uint8_t c[4]);
+
+// PLY ingestion (ply.cpp). With out == nullptr only the vertex count is
+// returned; otherwise capacity must hold every vertex.
+uint64_t ply_read(const uint8_t* data, size_t size, gs3d_gaussian* out, uint64_t capacity);
+uint64_t ply_read_file(const char* path, gs3d_gaussian* out, uint64_t capacity);
 
 // Synthetic scenes (synth.cpp).
 void synth_gaussians(int config, uint64_t seed, uint32_t first, uint32_t count, gs3d_gaussian* out);
