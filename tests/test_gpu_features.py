@@ -274,3 +274,25 @@ def test_server_results_same_alone_and_in_the_mix(gs, device):
         alone = srv.render_alone(reqs[i])
         assert np.array_equal(alone, images[i]), f"request {i} differs alone and in the mix"
     srv.close()
+
+
+# ---- PLY ingestion on the device -------------------------------------------------
+
+def test_ply_model_renders_like_oracle(gs, orc, device):
+    """The reference's fixture (coverage/model.ply) read by gs3d_ply_read_file
+    and rendered: visible set, keys and image against the oracle."""
+    import os
+    _case("model.ply")
+    g = gs.read_ply(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "model.ply"))
+    w, h = 640, 480
+    pod = gs.CameraPod.look_at((4.0, 4.0, -14.0), (4.0, 4.0, 4.0), (w, h))
+    v, img = _render(gs, device, g, pod, w, h)
+    frame = orc.frame_from_pod(pod.to_bytes())
+    pos, color, sh, cov = oracle_inputs(orc, g)
+    oidx, okey, oimg = orc.full_frame(frame, pos, color, sh, cov, w, h, orc.TARGET_RGBA8_F32)
+    vis = int(v.indirect_args()[1])
+    assert vis == len(oidx) == 9
+    assert np.array_equal(v.indirect_indices(vis), oidx)
+    assert np.array_equal(v.gaussians_depth(vis).view(np.uint32), okey.view(np.uint32))
+    print(f"model.ply: max diff {max_abs_diff(img, oimg)}/255 PSNR {psnr(img, oimg):.1f} dB", flush=True)
+    assert max_abs_diff(img, oimg) <= 2 and psnr(img, oimg) >= 45.0

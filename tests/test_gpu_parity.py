@@ -183,3 +183,43 @@ def test_config1_full_frame(gs, orc, device):
     d, p = max_abs_diff(img, oimg), psnr(img, oimg)
     print(f"config1: V={v} max diff {d}/255 PSNR {p:.2f} dB", flush=True)
     assert d <= 2 and p >= 45.0
+
+
+def test_fp16_storage_error_vs_fp64_oracle(gs, orc, device):
+    """Error of the f16 storage form (f16 SH / rotation / scale, f32
+    arithmetic) against the FP64 host oracle on the unrounded inputs, beside
+    the error of the f32 storage form. Both images are f32 (Rgba32Float
+    target); the FP64 image uses the f32 oracle's visible set and order."""
+    print("[case] fp16 storage vs fp64 oracle", flush=True)
+    n, w, h = 300_000, 960, 540
+    g = gs.synth_gaussians(2, n, 2)
+    pod = camera_pod(gs, 2, w, h, azimuth=0.7)
+    frame = orc.frame_from_pod(pod.to_bytes())
+    pos, color, sh, cov = oracle_inputs(orc, g)
+    oidx, _, _ = orc.full_frame(frame, pos, color, sh, cov, w, h, orc.TARGET_RGBA8_F32, threads=8)
+    cov64 = orc.cov3d_f64(g["scale"], g["rot"])
+    sp = orc.splats(frame, oidx, pos, color, sh, cov64, f64=True)
+    ref = orc.render(frame, w, h, sp, orc.TARGET_RGBA32F, threads=8, f64=True)
+    errs = {}
+    for kind in (gs.GaussianPod.HALF, gs.GaussianPod.SINGLE):
+        v = gs.Viewer(device, gs.TextureFormat.RGBA32_FLOAT, g, kind)
+        v.update_camera_with_pod(pod)
+        t = gs.Texture(device, w, h, gs.TextureFormat.RGBA32_FLOAT)
+        v.render(t)
+        device.poll()
+        img = t.read().astype(np.float64)
+        d = np.abs(img - ref)
+        errs[kind] = (float(d[..., :3].max()), float(d[..., :3].mean()), float(np.percentile(d[..., :3], 99.9)),
+                      float(d[..., 3].max()))
+        v.close()
+    h16, h32 = errs[gs.GaussianPod.HALF], errs[gs.GaussianPod.SINGLE]
+    print(f"f16 storage vs fp64: rgb max {h16[0]:.3e} mean {h16[1]:.3e} p99.9 {h16[2]:.3e} alpha max {h16[3]:.3e}",
+          flush=True)
+    print(f"f32 storage vs fp64: rgb max {h32[0]:.3e} mean {h32[1]:.3e} p99.9 {h32[2]:.3e} alpha max {h32[3]:.3e}",
+          flush=True)
+    # stated bounds: colour within 16/255 everywhere and a mean error below
+    # half a unorm8 step; the target's alpha stays 1 (cleared to 1, over
+    # operator)
+    assert h16[0] <= 16.0 / 255 and h16[1] <= 0.5 / 255
+    assert h32[0] <= 4.0 / 255 and h32[1] <= 0.25 / 255
+    assert h16[3] <= 1e-5 and h32[3] <= 1e-5

@@ -25,7 +25,9 @@ from typing import Optional, Sequence
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libgs3d.so")
+# GS3D_LIBRARY names another build of the library (for example one built
+# with different tuning defines by build.py --out).
+LIB_PATH = os.environ.get("GS3D_LIBRARY") or os.path.join(_HERE, "libgs3d.so")
 
 # gs3d_gaussian (wgpu_3dgs_core::Gaussian): 224 bytes
 GAUSSIAN_DTYPE = np.dtype([("rot", "<f4", (4,)), ("pos", "<f4", (3,)), ("color", "u1", (4,)),

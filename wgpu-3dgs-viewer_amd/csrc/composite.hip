@@ -6,19 +6,21 @@
 // 16x16 or 32x8 pixels); each wave owns an 8x8 block of the tile. The tile's
 // splats (its run of the tile-sorted pairs, in draw order) are staged through
 // LDS in batches of one splat per lane. While staging, every splat's pixel
-// bound is tested against the tile's 8x8 blocks and each wave receives, by
-// ballot compaction, the ordered list of the staged splats that can touch its
-// block: a wave then evaluates only those (most splats of dense scenes cover
-// a few pixels and touch one or two blocks).
+// bound is tested against the 4x4 quadrants of each wave's 8x8 block (fast)
+// or against the 8x8 blocks (exact), and each quadrant (block) receives, by
+// ballot compaction, the ordered list of the staged splats that can touch it.
+// Most splats of dense scenes cover a few pixels: a wave steps only through
+// the longest of its four quadrant lists, each 16-lane quadrant blending its
+// own list.
 //
 // fast  : front to back in f32 with transmittance. The over operator is
 //         associative, so a tile whose list is longer than one segment is
 //         split into segments composited in parallel, each to a partial
 //         (colour, transmittance) per pixel, and merged in order afterwards:
 //         the few very dense tiles of a clustered scene no longer set the
-//         kernel's length. Multi-segment tiles are scheduled first. A lane
-//         stops when its transmittance falls below 1e-4, a workgroup when
-//         all its lanes have.
+//         kernel's length. Multi-segment tiles are scheduled first. A wave
+//         stops when the transmittance of all its lanes is below 1e-4, a
+//         workgroup when all its waves have.
 // exact : back to front, one workgroup per tile; the destination is rounded
 //         to the target format after every splat, as the blender does for an
 //         Rgba8Unorm target (rounding is not associative: no segments).
@@ -32,7 +34,6 @@ namespace gs3d {
 
 constexpr int kCompMaxThreads = 256;
 constexpr int kCompMaxWaves = kCompMaxThreads / kWave;
-constexpr int kPlanThreads = 1024;
 constexpr float kLog2e = 1.4426950408889634f;
 
 // Lanes stop once what is left behind contributes at most 1e-4 of a colour.
@@ -50,13 +51,31 @@ __device__ __forceinline__ float h_hi(uint32_t w) { return __half2float(__ushort
 
 // One batch of staged splats plus, per wave, the ordered slots that touch
 // the wave's 8x8 block.
+// Slot kDummy holds a splat of opacity 0 (blending it changes nothing): lanes
+// whose list is shorter than the wave's longest blend it. List rows carry
+// kListPad spare entries so that a step may read past a list's end.
+constexpr uint32_t kDummy = kCompMaxThreads;
+constexpr uint32_t kListPad = 4;
+
+// The fast path lists splats per 4x4 quadrant of each wave's 8x8 block (16
+// lists for a 256-pixel tile); the exact path per 8x8 block (lists 0..3).
+constexpr int kQuads = 4 * kCompMaxWaves;
+
 struct Stage {
-  float4 geo[kCompMaxThreads];  // splat: cx, cy, ca*log2e, 2*cb*log2e   point: cx, cy, hx, hy
-  float4 col[kCompMaxThreads];  // cc*log2e, opacity, r, g
-  float blue[kCompMaxThreads];
-  uint16_t list[kCompMaxWaves][kCompMaxThreads];
-  uint32_t wcount[kCompMaxWaves][kCompMaxWaves];  // [staging wave][target wave]
+  float4 geo[kCompMaxThreads + 1];  // splat: cx, cy, ca*log2e, 2*cb*log2e   point: cx, cy, hx, hy
+  float4 col[kCompMaxThreads + 1];  // cc*log2e, opacity, r, g
+  float blue[kCompMaxThreads + 1];
+  uint16_t list[kQuads][kCompMaxThreads + kListPad];
+  uint32_t wcount[kCompMaxWaves][kQuads];  // [staging wave][target list]
 };
+
+__device__ __forceinline__ void init_dummy(Stage& s) {
+  if (threadIdx.x == 0) {
+    s.geo[kDummy] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    s.col[kDummy] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    s.blue[kDummy] = 0.0f;
+  }
+}
 
 struct Tiling {
   uint32_t px, py;
@@ -135,6 +154,62 @@ __device__ __forceinline__ void stage_batch(Stage& s, const Tiling& t, const Spl
   __syncthreads();
 }
 
+// Stage splat k (or nothing) into slot threadIdx.x and build one ordered list
+// per 4x4 quadrant of every wave's 8x8 block: quadrant (qx, qy) of wave w is
+// list 4w + qx + 2qy. Contains __syncthreads; every lane must call it.
+__device__ __forceinline__ void stage_batch_quads(Stage& s, const Tiling& t, const Splat2D* splats, bool valid,
+                                                  uint32_t k, bool clamp_src, int mode) {
+  const uint32_t tid = threadIdx.x, lane = tid % kWave, sw = tid / kWave;
+  uint32_t mask = 0u;
+  if (valid) {
+    const Splat2D r = splats[k];
+    float cr = h_lo(r.rg), cg = h_hi(r.rg), cb = h_lo(r.ba);
+    const float op = h_hi(r.ba);
+    if (clamp_src) {
+      cr = clamp01(cr);
+      cg = clamp01(cg);
+      cb = clamp01(cb);
+    }
+    if (mode == kModePoint) {
+      s.geo[tid] = make_float4(r.cx, r.cy, r.ca, r.cb);
+      s.col[tid] = make_float4(0.0f, op, cr, cg);
+    } else {
+      s.geo[tid] = make_float4(r.cx, r.cy, r.ca * kLog2e, 2.0f * r.cb * kLog2e);
+      s.col[tid] = make_float4(r.cc * kLog2e, op, cr, cg);
+    }
+    s.blue[tid] = cb;
+    // quadrant pixel centres span [o + 0.5, o + 3.5]: half a pixel of margin
+    const float hx = h_lo(r.hxy), hy = h_hi(r.hxy);
+    const float x0 = r.cx - hx - t.ox, x1 = r.cx + hx - t.ox;
+    const float y0 = r.cy - hy - t.oy, y1 = r.cy + hy - t.oy;
+    for (uint32_t w = 0; w < t.waves; ++w) {
+      const float bx = float((w % t.blocks_x) * 8), by = float((w / t.blocks_x) * 8);
+#pragma unroll
+      for (uint32_t qd = 0; qd < 4; ++qd) {
+        const float qx = bx + float((qd & 1u) * 4), qy = by + float((qd >> 1) * 4);
+        const bool hit = !(x1 < qx) && !(x0 > qx + 4.0f) && !(y1 < qy) && !(y0 > qy + 4.0f);
+        mask |= hit ? (1u << (4 * w + qd)) : 0u;
+      }
+    }
+  }
+  uint64_t bal[kQuads];
+#pragma unroll
+  for (int q = 0; q < kQuads; ++q) {
+    bal[q] = __ballot((mask >> q) & 1u);
+    if (lane == 0) s.wcount[sw][q] = uint32_t(__popcll(bal[q]));
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < kQuads; ++q) {
+    if ((mask >> q) & 1u) {
+      uint32_t off = 0;
+      for (uint32_t p = 0; p < sw; ++p) off += s.wcount[p][q];
+      s.list[q][off + mask_rank(bal[q])] = uint16_t(tid);
+    }
+  }
+  __syncthreads();
+}
+
 __device__ __forceinline__ uint32_t wave_list_len(const Stage& s, const Tiling& t, uint32_t w) {
   uint32_t n = 0;
   for (uint32_t q = 0; q < t.waves; ++q) n += s.wcount[q][w];
@@ -156,6 +231,85 @@ __device__ __forceinline__ float fragment(const Stage& s, uint32_t q, const Tili
   if (!(r2 <= sd2l)) return -1.0f;
   if (mode == kModeEllipse) return col.y + (1.0f - col.y) * ((r2 > ell2l) ? 1.0f : 0.0f);
   return col.y * __builtin_amdgcn_exp2f(-r2);
+}
+
+// Fragment alpha for the fast path: 0 for a discarded fragment (blending
+// alpha 0 leaves colour and transmittance unchanged), so the loop has no
+// branches.
+template <int MODE>
+__device__ __forceinline__ float fragment_alpha(float4 g, float4 c, const Tiling& t, float sd2l, float ell2l) {
+#pragma clang fp contract(fast)
+  const float dx = t.fpx - g.x;
+  const float dy = t.fpy - g.y;
+  if (MODE == kModePoint) return (dx >= -g.z && dx < g.z && dy >= -g.w && dy < g.w) ? c.y : 0.0f;
+  const float r2 = dx * (g.z * dx + g.w * dy) + c.x * dy * dy;
+  float al;
+  if (MODE == kModeEllipse)
+    al = c.y + (1.0f - c.y) * ((r2 > ell2l) ? 1.0f : 0.0f);
+  else
+    al = c.y * __builtin_amdgcn_exp2f(-r2);
+  return (r2 <= sd2l) ? al : 0.0f;
+}
+
+// Splats blended per step of the fast loop (1, 2 or 4): more overlaps the
+// LDS reads of a step, fewer keeps registers for occupancy.
+#ifndef GS3D_COMPOSITE_UNROLL
+#define GS3D_COMPOSITE_UNROLL 2
+#endif
+constexpr int kUnroll = GS3D_COMPOSITE_UNROLL;
+static_assert(kUnroll == 1 || kUnroll == 2 || kUnroll == 4, "unroll must be 1, 2 or 4");
+static_assert(kListPad % kUnroll == 0, "a step reads at most kListPad entries past a list");
+
+// Front-to-back blend of the lane's quadrant list, kUnroll entries per step;
+// a wave steps through its longest quadrant list and the lanes of shorter
+// lists blend the dummy slot. Lanes past the transmittance threshold keep
+// blending (their colour only gets closer to the exact sum) until the whole
+// wave is done.
+template <int MODE>
+__device__ __forceinline__ void blend_quads(const Stage& s, const Tiling& t, uint32_t w, float sd2l, float ell2l,
+                                            float& T, float& cr, float& cg, float& cb, bool& done) {
+#pragma clang fp contract(fast)
+  const uint32_t lane = threadIdx.x % kWave;
+  const uint32_t myq = 4u * w + (((lane >> 2) & 1u) | ((lane >> 5) << 1));
+  uint32_t lens[4] = {0u, 0u, 0u, 0u};
+  for (uint32_t p = 0; p < t.waves; ++p) {
+#pragma unroll
+    for (uint32_t qd = 0; qd < 4; ++qd) lens[qd] += s.wcount[p][4 * w + qd];
+  }
+  const uint32_t mylen = lens[myq & 3u];
+  const uint32_t wmax = max(max(lens[0], lens[1]), max(lens[2], lens[3]));
+  for (uint32_t j = 0; j < wmax; j += kUnroll) {
+    if ((j & 31u) == 0u && __all(done)) break;
+    uint32_t q[kUnroll];
+    if (kUnroll == 1) {
+      q[0] = j < mylen ? uint32_t(s.list[myq][j]) : kDummy;
+    } else {
+#pragma unroll
+      for (int k = 0; k < kUnroll; k += 2) {
+        const uint32_t qq = *reinterpret_cast<const uint32_t*>(&s.list[myq][j + k]);
+        q[k] = (j + k < mylen) ? (qq & 0xFFFFu) : kDummy;
+        q[k + 1 < kUnroll ? k + 1 : 0] = (j + k + 1 < mylen) ? (qq >> 16) : kDummy;
+      }
+    }
+    float4 g[kUnroll], c[kUnroll];
+    float bl[kUnroll];
+#pragma unroll
+    for (int k = 0; k < kUnroll; ++k) {
+      g[k] = s.geo[q[k]];
+      c[k] = s.col[q[k]];
+      bl[k] = s.blue[q[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < kUnroll; ++k) {
+      const float al = fragment_alpha<MODE>(g[k], c[k], t, sd2l, ell2l);
+      const float wgt = T * al;
+      cr += wgt * c[k].z;
+      cg += wgt * c[k].w;
+      cb += wgt * bl[k];
+      T *= 1.0f - al;
+    }
+    done = T < kMinTransmittance;
+  }
 }
 
 // Final pixel: the front-to-back colour plus what is left of the background
@@ -192,69 +346,125 @@ __device__ __forceinline__ uint32_t segments_of(uint32_t len, uint32_t seg) {
   return len <= seg ? 1u : (len + seg - 1u) / seg;
 }
 
-// Work list: one item per (tile, segment); items of multi-segment tiles come
-// first. One workgroup scans all tiles (14,400 at 2560x1440 in 16x16).
-__global__ __launch_bounds__(kPlanThreads) void k_plan(CompositeArgs a) {
-  __shared__ uint32_t s_sum[kPlanThreads / kWave];
-  __shared__ uint32_t s_heavy_total;
-  const uint32_t nt = a.grid.tiles_x * a.grid.tiles_y;
-  const uint32_t per = (nt + kPlanThreads - 1) / kPlanThreads;
-  const uint32_t t0 = threadIdx.x * per;
-  const uint32_t t1 = (t0 + per < nt) ? t0 + per : nt;
+// Work list: one item per (tile, segment); the segments of multi-segment
+// tiles come first (heavy-first), then one item per other tile. Built in two
+// launches over blocks of kPlanTiles tiles: per-block counts, then each block
+// scans the block counts before it and writes its items in tile order.
+constexpr uint32_t kPlanTiles = 256;
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t x, uint32_t* s_wave, uint32_t& total) {
   const uint32_t lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
-  const uint32_t max_items = nt + a.capacity / a.seg_len + 1u;  // composite_max_items
-  for (int pass = 0; pass < 2; ++pass) {
-    // pass 0: multi-segment tiles (their segments); pass 1: the other tiles
-    uint32_t cnt = 0;
-    for (uint32_t t = t0; t < t1; ++t) {
-      const uint32_t ns = segments_of(a.ranges[2 * t + 1] - a.ranges[2 * t], a.seg_len);
-      cnt += (pass == 0) ? (ns > 1 ? ns : 0u) : (ns > 1 ? 0u : 1u);
-    }
-    uint32_t x = cnt;
+  uint32_t inc = x;
 #pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-      const uint32_t y = __shfl_up(x, off, kWave);
-      if (lane >= uint32_t(off)) x += y;
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint32_t y = __shfl_up(inc, off, kWave);
+    if (lane >= uint32_t(off)) inc += y;
+  }
+  if (lane == kWave - 1) s_wave[w] = inc;
+  __syncthreads();
+  uint32_t pre = 0;
+  total = 0;
+  for (uint32_t q = 0; q < kPlanTiles / kWave; ++q) {
+    if (q < w) pre += s_wave[q];
+    total += s_wave[q];
+  }
+  __syncthreads();
+  return pre + inc - x;
+}
+
+__global__ __launch_bounds__(kPlanTiles) void k_plan_count(CompositeArgs a) {
+  __shared__ uint32_t s_wave[2][kPlanTiles / kWave];
+  const uint32_t nt = a.grid.tiles_x * a.grid.tiles_y;
+  const uint32_t t = blockIdx.x * kPlanTiles + threadIdx.x;
+  uint32_t heavy = 0, light = 0;
+  if (t < nt) {
+    const uint32_t ns = segments_of(a.ranges[2 * t + 1] - a.ranges[2 * t], a.seg_len);
+    heavy = ns > 1 ? ns : 0u;
+    light = ns > 1 ? 0u : 1u;
+  }
+  uint32_t th, tl;
+  block_exclusive_scan(heavy, s_wave[0], th);
+  block_exclusive_scan(light, s_wave[1], tl);
+  if (threadIdx.x == 0) {
+    a.plan_counts[2 * blockIdx.x] = th;
+    a.plan_counts[2 * blockIdx.x + 1] = tl;
+  }
+}
+
+__global__ __launch_bounds__(kPlanTiles) void k_plan(CompositeArgs a) {
+  __shared__ uint32_t s_wave[2][kPlanTiles / kWave];
+  __shared__ uint32_t s_red[4][kPlanTiles / kWave];
+  const uint32_t nt = a.grid.tiles_x * a.grid.tiles_y;
+  const uint32_t nb = (nt + kPlanTiles - 1) / kPlanTiles;
+  const uint32_t max_items = nt + a.capacity / a.seg_len + 1u;  // composite_max_items
+  const uint32_t lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  // heavy / light items of the blocks before this one, and of all blocks
+  uint32_t hb = 0, lb = 0, ha = 0, la = 0;
+  for (uint32_t b = threadIdx.x; b < nb; b += kPlanTiles) {
+    const uint32_t h = a.plan_counts[2 * b], l = a.plan_counts[2 * b + 1];
+    ha += h;
+    la += l;
+    if (b < blockIdx.x) {
+      hb += h;
+      lb += l;
     }
-    if (lane == kWave - 1) s_sum[w] = x;
-    __syncthreads();
-    uint32_t pre = 0;
-    for (uint32_t q = 0; q < w; ++q) pre += s_sum[q];
-    uint32_t total = 0;
-    for (uint32_t q = 0; q < kPlanThreads / kWave; ++q) total += s_sum[q];
-    __syncthreads();
-    uint32_t item = (pass == 0 ? 0u : s_heavy_total) + pre + x - cnt;
-    for (uint32_t t = t0; t < t1; ++t) {
-      const uint32_t ns = segments_of(a.ranges[2 * t + 1] - a.ranges[2 * t], a.seg_len);
-      if (pass == 0 && ns > 1) {
-        a.seg_base[t] = item;
-        for (uint32_t sg = 0; sg < ns; ++sg, ++item) {
-          if (item < max_items) {
-            a.item_tile[item] = t;
-            a.item_seg[item] = sg;
-          }
-        }
-      } else if (pass == 1 && ns <= 1) {
+  }
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    hb += __shfl_xor(hb, off, kWave);
+    lb += __shfl_xor(lb, off, kWave);
+    ha += __shfl_xor(ha, off, kWave);
+    la += __shfl_xor(la, off, kWave);
+  }
+  if (lane == 0) {
+    s_red[0][w] = hb;
+    s_red[1][w] = lb;
+    s_red[2][w] = ha;
+    s_red[3][w] = la;
+  }
+  __syncthreads();
+  hb = lb = ha = la = 0;
+  for (uint32_t q = 0; q < kPlanTiles / kWave; ++q) {
+    hb += s_red[0][q];
+    lb += s_red[1][q];
+    ha += s_red[2][q];
+    la += s_red[3][q];
+  }
+  const uint32_t t = blockIdx.x * kPlanTiles + threadIdx.x;
+  uint32_t ns = 0, heavy = 0, light = 0;
+  if (t < nt) {
+    ns = segments_of(a.ranges[2 * t + 1] - a.ranges[2 * t], a.seg_len);
+    heavy = ns > 1 ? ns : 0u;
+    light = ns > 1 ? 0u : 1u;
+  }
+  uint32_t th, tl;
+  const uint32_t hx = block_exclusive_scan(heavy, s_wave[0], th);
+  const uint32_t lx = block_exclusive_scan(light, s_wave[1], tl);
+  if (t < nt) {
+    if (ns > 1) {
+      uint32_t item = hb + hx;
+      a.seg_base[t] = item;
+      for (uint32_t sg = 0; sg < ns; ++sg, ++item) {
         if (item < max_items) {
           a.item_tile[item] = t;
-          a.item_seg[item] = 0u;
+          a.item_seg[item] = sg;
         }
-        ++item;
+      }
+    } else {
+      const uint32_t item = ha + lb + lx;
+      if (item < max_items) {
+        a.item_tile[item] = t;
+        a.item_seg[item] = 0u;
       }
     }
-    if (threadIdx.x == 0) {
-      if (pass == 0) {
-        s_heavy_total = total;
-        a.plan[1] = total;  // multi-segment items
-      } else {
-        // all items; never more than the list holds (the bound is exact for
-        // lists built from at most `capacity` pairs)
-        const uint32_t all = s_heavy_total + total;
-        a.plan[0] = all < max_items ? all : max_items;
-        a.plan[2] = 0u;  // work ticket
-      }
-    }
-    __syncthreads();
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // all items, never more than the list holds (the bound is exact for lists
+    // built from at most `capacity` pairs); multi-segment items; work ticket
+    const uint32_t all = ha + la;
+    a.plan[0] = all < max_items ? all : max_items;
+    a.plan[1] = ha;
+    a.plan[2] = 0u;
   }
 }
 
@@ -270,6 +480,7 @@ __global__ __launch_bounds__(kCompMaxThreads) void k_composite_fast(CompositeArg
   const bool q8 = a.format == kFormatRgba8Unorm;
   const uint32_t nt = blockDim.x, tid = threadIdx.x, w = tid / kWave;
   const uint32_t items = a.plan[0], heavy = a.plan[1];
+  init_dummy(s);
 
   for (;;) {
     const uint32_t item = take_ticket(a.plan + 2, &s_item);
@@ -287,22 +498,14 @@ __global__ __launch_bounds__(kCompMaxThreads) void k_composite_fast(CompositeArg
       const uint32_t cnt = (bend - back) < nt ? (bend - back) : nt;
       __syncthreads();
       const bool valid = tid < cnt;
-      stage_batch(s, t, a.splats, valid, valid ? a.pair_vals[bend - 1 - tid] : 0u, q8, mode);  // nearest first
-      const uint32_t len = wave_list_len(s, t, w);
-      for (uint32_t j = 0; j < len; ++j) {
-        if ((j & 31u) == 0u && __all(done)) break;
-        if (done) continue;
-        const uint32_t q = s.list[w][j];
-        float4 col;
-        const float al = fragment(s, q, t, mode, sd2l, ell2l, col);
-        if (al < 0.0f) continue;
-        const float wgt = T * al;
-        cr += wgt * col.z;
-        cg += wgt * col.w;
-        cb += wgt * s.blue[q];
-        T *= 1.0f - al;
-        done = T < kMinTransmittance;
-      }
+      // nearest first
+      stage_batch_quads(s, t, a.splats, valid, valid ? a.pair_vals[bend - 1 - tid] : 0u, q8, mode);
+      if (mode == kModeSplat)
+        blend_quads<kModeSplat>(s, t, w, sd2l, ell2l, T, cr, cg, cb, done);
+      else if (mode == kModeEllipse)
+        blend_quads<kModeEllipse>(s, t, w, sd2l, ell2l, T, cr, cg, cb, done);
+      else
+        blend_quads<kModePoint>(s, t, w, sd2l, ell2l, T, cr, cg, cb, done);
       bend -= cnt;
       if (__syncthreads_and(done ? 1 : 0)) break;
     }
@@ -427,6 +630,8 @@ uint32_t composite_max_items(uint32_t num_tiles, uint32_t capacity, uint32_t seg
   return num_tiles + capacity / seg_len + 1;
 }
 
+uint32_t composite_plan_blocks(uint32_t num_tiles) { return (num_tiles + kPlanTiles - 1) / kPlanTiles; }
+
 uint32_t composite_max_heavy_items(uint32_t capacity, uint32_t seg_len) {
   // multi-segment tiles have len > seg_len, so ceil(len/seg_len) < 2 len/seg_len
   return 2u * (capacity / seg_len) + 2u;
@@ -438,7 +643,9 @@ void launch_composite(hipStream_t st, const CompositeArgs& a, int num_cus, hipEv
   const uint32_t threads = a.grid.tile_w * a.grid.tile_h;
   if (tiles == 0) return;
   if (a.blend == kBlendFast) {
-    hipLaunchKernelGGL(k_plan, dim3(1), dim3(kPlanThreads), 0, st, a);
+    const uint32_t nb = composite_plan_blocks(tiles);
+    hipLaunchKernelGGL(k_plan_count, dim3(nb), dim3(kPlanTiles), 0, st, a);
+    hipLaunchKernelGGL(k_plan, dim3(nb), dim3(kPlanTiles), 0, st, a);
     // persistent workgroups take items by ticket; 8 per CU of 256 lanes
     uint32_t grid = uint32_t(num_cus) * 8u * (kCompMaxThreads / threads);
     const uint32_t cap = composite_max_items(tiles, a.capacity, a.seg_len);

@@ -88,31 +88,56 @@ __device__ __forceinline__ void cov3d_of(const CovWords& w, float c[6]) {
   }
 }
 
+// SH colour of one instance (utils.wesl:82-136) for the splat record:
+// the 15 basis values of the view direction are formed once, then the
+// coefficients are streamed 16 bytes at a time into the three channel sums,
+// so no more than one chunk of coefficients is live in registers. The sums
+// run in coefficient order, not the shader's association (the record is
+// compared within a tolerance, not bit for bit).
 template <int POD>
-__device__ __forceinline__ void load_sh(const GaussianStreams& g, uint32_t i, float sh[48]) {
+__device__ __forceinline__ void sh_color(const GaussianStreams& g, uint32_t i, float x, float y, float z, int sh_deg,
+                                         float rgb[3]) {
+#pragma clang fp contract(fast)
+  if (sh_deg < 1) return;
+  const float xx = x * x, yy = y * y, zz = z * z;
+  const float xy = x * y, yz = y * z, xz = x * z;
+  const float basis[15] = {
+      -ShC::c1 * y, ShC::c1 * z, -ShC::c1 * x,
+      ShC::c2_0 * xy, ShC::c2_1 * yz, ShC::c2_2 * ((2.0f * zz - xx) - yy), ShC::c2_3 * xz, ShC::c2_4 * (xx - yy),
+      (ShC::c3_0 * y) * (3.0f * xx - yy), (ShC::c3_1 * xy) * z, (ShC::c3_2 * y) * ((4.0f * zz - xx) - yy),
+      (ShC::c3_3 * z) * ((2.0f * zz - 3.0f * xx) - 3.0f * yy), (ShC::c3_4 * x) * ((4.0f * zz - xx) - yy),
+      (ShC::c3_5 * z) * (xx - yy), (ShC::c3_6 * x) * (xx - 3.0f * yy)};
+  // coefficient j (0..44) is band-k coefficient k = j / 3 of channel j % 3;
+  // bands start at k = 0 (degree 1), 3 (degree 2) and 8 (degree 3)
+  const int kmax = sh_deg >= 3 ? 15 : (sh_deg == 2 ? 8 : 3);
+  auto add = [&](int j, float v) {
+    if (j < 45 && j / 3 < kmax) rgb[j % 3] += basis[j / 3] * v;
+  };
   if constexpr (POD == kPodSingle) {
     const float4* p = g.sh32 + size_t(i) * 12;
 #pragma unroll
     for (int q = 0; q < 12; ++q) {
+      if (4 * q >= 3 * kmax) break;
       const float4 v = p[q];
-      sh[4 * q + 0] = v.x;
-      sh[4 * q + 1] = v.y;
-      sh[4 * q + 2] = v.z;
-      sh[4 * q + 3] = v.w;
+      add(4 * q + 0, v.x);
+      add(4 * q + 1, v.y);
+      add(4 * q + 2, v.z);
+      add(4 * q + 3, v.w);
     }
   } else {
     const uint4* p = g.sh16 + size_t(i) * 6;
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
+      if (8 * q >= 3 * kmax) break;
       const uint4 v = p[q];
-      sh[8 * q + 0] = half_lo(v.x);
-      sh[8 * q + 1] = half_hi(v.x);
-      sh[8 * q + 2] = half_lo(v.y);
-      sh[8 * q + 3] = half_hi(v.y);
-      sh[8 * q + 4] = half_lo(v.z);
-      sh[8 * q + 5] = half_hi(v.z);
-      sh[8 * q + 6] = half_lo(v.w);
-      sh[8 * q + 7] = half_hi(v.w);
+      add(8 * q + 0, half_lo(v.x));
+      add(8 * q + 1, half_hi(v.x));
+      add(8 * q + 2, half_lo(v.y));
+      add(8 * q + 3, half_hi(v.y));
+      add(8 * q + 4, half_lo(v.z));
+      add(8 * q + 5, half_hi(v.z));
+      add(8 * q + 6, half_lo(v.w));
+      add(8 * q + 7, half_hi(v.w));
     }
   }
 }
@@ -142,14 +167,13 @@ __device__ __forceinline__ void compute_splat(const FrameConsts& fc, const Gauss
   m3v(fc.inv_sr, wvd, mvd);
   const float il = __builtin_amdgcn_rsqf(mvd[0] * mvd[0] + mvd[1] * mvd[1] + mvd[2] * mvd[2]);
   const float x = -mvd[0] * il, y = -mvd[1] * il, z = -mvd[2] * il;
-  float sh[48];
-  if (fc.sh_deg >= 1) load_sh<POD>(g, i, sh);
   float rgb[3];
 #pragma unroll
-  for (int ch = 0; ch < 3; ++ch) {
-    const float base = fc.no_sh0 ? 0.5f : float((color_bits >> (8 * ch)) & 0xFFu) * (1.0f / 255.0f);
-    rgb[ch] = view_color_channel(base, x, y, z, fc.sh_deg, [&](int k) { return sh[k * 3 + ch]; });
-  }
+  for (int ch = 0; ch < 3; ++ch)
+    rgb[ch] = fc.no_sh0 ? 0.5f : float((color_bits >> (8 * ch)) & 0xFFu) * (1.0f / 255.0f);
+  sh_color<POD>(g, i, x, y, z, fc.sh_deg, rgb);
+#pragma unroll
+  for (int ch = 0; ch < 3; ++ch) rgb[ch] = fmaxf(rgb[ch], 0.0f);
   float alpha = float(color_bits >> 24) * (1.0f / 255.0f);
 
   const float iw = __builtin_amdgcn_rcpf(proj[3]);

@@ -107,6 +107,7 @@ struct CompositeArgs {
   uint32_t* item_seg;  // [composite_max_items]
   uint32_t* seg_base;  // [tiles] first item of a multi-segment tile
   uint32_t* plan;      // {items, multi-segment items, work ticket}
+  uint32_t* plan_counts;  // [composite_plan_blocks][2] per-block item counts
   float4* partial;     // [composite_max_heavy_items][tile pixels]
 };
 
@@ -120,6 +121,7 @@ void launch_ranges(hipStream_t st, const uint32_t* pair_count, uint32_t capacity
                    uint32_t* ranges, int num_cus, uint32_t* host_total);
 uint32_t composite_max_items(uint32_t num_tiles, uint32_t capacity, uint32_t seg_len);
 uint32_t composite_max_heavy_items(uint32_t capacity, uint32_t seg_len);
+uint32_t composite_plan_blocks(uint32_t num_tiles);
 // main_begin/main_end, when not null, are recorded around the main
 // compositing kernel (for per-kernel timing).
 void launch_composite(hipStream_t st, const CompositeArgs& a, int num_cus, hipEvent_t main_begin = nullptr,

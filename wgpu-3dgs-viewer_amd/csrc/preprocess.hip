@@ -21,7 +21,14 @@ namespace gs3d {
 
 constexpr int kPreThreads = 256;
 constexpr int kPreWaves = kPreThreads / kWave;
-constexpr int kPreItems = 4;
+// Gaussians per thread; waves per SIMD the register budget is sized for
+#ifndef GS3D_PRE_ITEMS
+#define GS3D_PRE_ITEMS 4
+#endif
+#ifndef GS3D_PRE_WAVES_PER_EU
+#define GS3D_PRE_WAVES_PER_EU 1
+#endif
+constexpr int kPreItems = GS3D_PRE_ITEMS;
 constexpr uint32_t kPreTile = kPreThreads * kPreItems;
 
 __global__ __launch_bounds__(256) void k_frame_pre(const CameraPod* __restrict__ cam,
@@ -58,7 +65,8 @@ __global__ __launch_bounds__(64) void k_frame_consts(const CameraPod* __restrict
 }
 
 template <int POD>
-__global__ __launch_bounds__(kPreThreads) void k_preprocess(PreprocessArgs a) {
+__global__ __launch_bounds__(kPreThreads) __attribute__((amdgpu_waves_per_eu(GS3D_PRE_WAVES_PER_EU))) void k_preprocess(
+    PreprocessArgs a) {
   __shared__ uint32_t s_cnt[kPreItems][kPreWaves];
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_base;

@@ -4,8 +4,10 @@
 // (src/radix_sorter.rs:98-940, src/shader/radix_sort.wgsl): same result (a
 // stable ascending sort), different machinery:
 //   * one upsweep reads the keys once and builds every pass's 256-bin
-//     histogram (LDS atomics, then one global atomic per bin per workgroup);
-//   * each pass is a persistent kernel: a workgroup takes 4096-key tiles by
+//     histogram (wave-aggregated LDS atomics, then one global atomic per bin
+//     per workgroup);
+//   * each pass is a persistent kernel: a workgroup takes tiles of
+//     256 x GS3D_SORT_ITEMS keys by
 //     ticket, ranks them with wave64 ballot matching (8 ballots per key give
 //     the lanes sharing its digit), publishes per-digit counts, looks back
 //     over its predecessors for the global offset, reorders the tile in LDS
@@ -22,8 +24,12 @@ namespace gs3d {
 
 constexpr int kRsThreads = 256;
 constexpr int kRsWaves = kRsThreads / kWave;
-constexpr int kRsItems = 16;
-constexpr uint32_t kRsTile = kRsThreads * kRsItems;  // 4096 keys
+// keys per thread of a onesweep tile
+#ifndef GS3D_SORT_ITEMS
+#define GS3D_SORT_ITEMS 16
+#endif
+constexpr int kRsItems = GS3D_SORT_ITEMS;
+constexpr uint32_t kRsTile = kRsThreads * kRsItems;
 
 __device__ __forceinline__ uint32_t sort_n(const SortCount& c) {
   const uint64_t v = uint64_t(*c.count) * uint64_t(c.count_mul);
@@ -67,15 +73,37 @@ __global__ __launch_bounds__(256) void k_rs_zero(SortCount cnt, SortScratch sc, 
     for (uint32_t j = pad_begin + gid; j < pad_end; j += stride) pad_keys[j] = 0xFFFFFFFFu;
 }
 
+// Lanes of the wave whose 8-bit digit equals this lane's (ballot matching,
+// one ballot per digit bit). Inactive lanes take no part.
+__device__ __forceinline__ uint64_t match_digit(uint32_t d) {
+  uint64_t m = __ballot(1);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t bb = __ballot(bit);
+    m &= bit ? bb : ~bb;
+  }
+  return m;
+}
+
+// Histograms of every pass's digit from one read of the keys. Each wave
+// counts into its own LDS histograms with one atomic per distinct digit of a
+// 64-key step (the lanes sharing a digit add together: depth keys share their
+// high digits, and per-key atomics on one bin would serialize).
 __global__ __launch_bounds__(256) void k_rs_upsweep(SortCount cnt, const uint32_t* __restrict__ keys,
                                                     uint32_t* __restrict__ hist, int passes) {
-  __shared__ uint32_t s_h[4][256];
+  __shared__ uint32_t s_h[kRsWaves][4][256];
   const uint32_t n = sort_n(cnt);
-  for (int p = 0; p < 4; ++p) s_h[p][threadIdx.x] = 0u;
+  const uint32_t w = threadIdx.x / kWave;
+  for (int q = 0; q < kRsWaves; ++q)
+    for (int p = 0; p < 4; ++p) s_h[q][p][threadIdx.x] = 0u;
   __syncthreads();
   const uint32_t stride = gridDim.x * blockDim.x * 4u;
-  for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) * 4u; i < n; i += stride) {
-    uint32_t k4[4];
+  // the loop bound is uniform per wave (it depends on the wave's first key)
+  const uint32_t wave0 = (blockIdx.x * blockDim.x + w * kWave) * 4u;
+  for (uint32_t i0 = wave0; i0 < n; i0 += stride) {
+    const uint32_t i = i0 + (threadIdx.x % kWave) * 4u;
+    uint32_t k4[4] = {0u, 0u, 0u, 0u};
     if (i + 3u < n) {
       const uint4 v = *reinterpret_cast<const uint4*>(keys + i);
       k4[0] = v.x;
@@ -85,13 +113,21 @@ __global__ __launch_bounds__(256) void k_rs_upsweep(SortCount cnt, const uint32_
     } else {
       for (int q = 0; q < 4; ++q) k4[q] = (i + q < n) ? keys[i + q] : 0u;
     }
-    const int m = (i + 3u < n) ? 4 : int(n - i);
-    for (int q = 0; q < m; ++q)
-      for (int p = 0; p < passes; ++p) atomicAdd(&s_h[p][(k4[q] >> (8 * p)) & 0xFFu], 1u);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (i + q < n) {
+        for (int p = 0; p < passes; ++p) {
+          const uint32_t d = (k4[q] >> (8 * p)) & 0xFFu;
+          const uint64_t m = match_digit(d);
+          if (mask_rank(m) == 0u) atomicAdd(&s_h[w][p][d], uint32_t(__popcll(m)));
+        }
+      }
+    }
   }
   __syncthreads();
   for (int p = 0; p < passes; ++p) {
-    const uint32_t c = s_h[p][threadIdx.x];
+    uint32_t c = 0;
+    for (int q = 0; q < kRsWaves; ++q) c += s_h[q][p][threadIdx.x];
     if (c) atomicAdd(&hist[p * 256 + threadIdx.x], c);
   }
 }

@@ -695,6 +695,7 @@ void Renderer::render(Texture& target, const IndirectArgsBuffer& indirect_args, 
   grow(item_seg_, items * 4);
   grow(seg_base_, size_t(num_tiles) * 4);
   grow(plan_, 16);
+  grow(plan_counts_, size_t(composite_plan_blocks(num_tiles)) * 8);
   grow(partial_, heavy * tile_px * sizeof(float4));
   CompositeArgs ca{};
   ca.fc = fc;
@@ -713,6 +714,7 @@ void Renderer::render(Texture& target, const IndirectArgsBuffer& indirect_args, 
   ca.item_seg = item_seg_.as<uint32_t>();
   ca.seg_base = seg_base_.as<uint32_t>();
   ca.plan = plan_.as<uint32_t>();
+  ca.plan_counts = plan_counts_.as<uint32_t>();
   ca.partial = partial_.as<float4>();
   launch_composite(st, ca, dev_.num_cus(), tm.event(GS3D_STAGE_COMPOSITE_MAIN, false),
                    tm.event(GS3D_STAGE_COMPOSITE_MAIN, true));

@@ -398,7 +398,7 @@ class Renderer {
   uint32_t ranges_words_ = 0;
   // compositing work list (fast blend)
   uint32_t seg_len_ = 512;
-  DeviceBuffer item_tile_, item_seg_, seg_base_, plan_, partial_;
+  DeviceBuffer item_tile_, item_seg_, seg_base_, plan_, plan_counts_, partial_;
   uint32_t* pair_total_host_ = nullptr;  // pinned, mapped
   uint32_t* pair_total_dev_ = nullptr;   // its device alias
   DeviceBuffer frag_count_;
