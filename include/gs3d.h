@@ -85,8 +85,23 @@ typedef enum gs3d_gaussian_pod {
 /* wgpu::TextureFormat of the render target (src/renderer.rs:120-125). */
 typedef enum gs3d_texture_format {
     GS3D_FORMAT_RGBA8_UNORM = 0,
-    GS3D_FORMAT_RGBA32_FLOAT = 1
+    GS3D_FORMAT_RGBA32_FLOAT = 1,
+    /* a depth attachment (wgpu::TextureFormat::Depth32Float), one f32 per
+     * pixel; usable only as the depth texture of a draw */
+    GS3D_FORMAT_DEPTH32_FLOAT = 2
 } gs3d_texture_format;
+
+/* wgpu::CompareFunction (same numbering) of a depth test. */
+typedef enum gs3d_compare {
+    GS3D_COMPARE_NEVER = 1,
+    GS3D_COMPARE_LESS = 2,
+    GS3D_COMPARE_EQUAL = 3,
+    GS3D_COMPARE_LESS_EQUAL = 4,
+    GS3D_COMPARE_GREATER = 5,
+    GS3D_COMPARE_NOT_EQUAL = 6,
+    GS3D_COMPARE_GREATER_EQUAL = 7,
+    GS3D_COMPARE_ALWAYS = 8
+} gs3d_compare;
 
 /* GaussianDisplayMode (render.wesl:12-14). */
 typedef enum gs3d_display_mode {
@@ -148,6 +163,9 @@ void gs3d_texture_destroy(gs3d_texture* tex);
 /* Copies the texture to host memory (row-major, top row first, 4 channels);
  * waits for the queue first. */
 gs3d_status gs3d_texture_read(gs3d_texture* tex, void* dst, size_t bytes);
+/* queue.write_texture: replaces a texture's contents (bytes = its size, f32
+ * per pixel for a depth texture); waits for the device first. */
+gs3d_status gs3d_texture_write(gs3d_texture* tex, const void* src, size_t bytes);
 
 /* ---- Viewer (src/lib.rs:65-276) ------------------------------------------ */
 
@@ -166,8 +184,28 @@ gs3d_status gs3d_viewer_update_model_transform(gs3d_viewer* v, const float pos[3
 /* Viewer::update_gaussian_transform (src/lib.rs:237-254). */
 gs3d_status gs3d_viewer_update_gaussian_transform(gs3d_viewer* v, float size, gs3d_display_mode display_mode,
                                                   uint32_t sh_deg, int no_sh0, float max_std_dev);
+/* Selection early-out of the preprocessor (src/shader/preprocess.wesl:69-78,
+ * feature "viewer-selection"; buffers src/lib.rs:74-77): the SelectionBuffer
+ * contents, bit i of word i / 32 for Gaussian i, exactly ceil(N / 32) words.
+ * Gaussian i is skipped when its bit equals the invert flag. Until the first
+ * call the bitset is all zero. */
+gs3d_status gs3d_viewer_update_selection(gs3d_viewer* v, const uint32_t* words, uint32_t word_count);
+/* PreprocessorInvertSelectionBuffer::update (src/selection/buffer.rs:167-171);
+ * the initial value is true (selected Gaussians are hidden). */
+gs3d_status gs3d_viewer_update_invert_selection(gs3d_viewer* v, int invert);
 /* Viewer::render (src/lib.rs:266-275): preprocess -> sort -> render. */
 gs3d_status gs3d_viewer_render(gs3d_viewer* v, gs3d_texture* target);
+/* ViewerCreateOptions::depth_stencil (src/lib.rs:279-290, passed to
+ * Renderer::new, src/renderer.rs:120-137 and 300-305): enabled != 0 gives the
+ * renderer a depth test with `compare` and `depth_write`; its frames are then
+ * drawn with gs3d_viewer_render_with_depth only. A fragment's depth is the
+ * clip z / w of its splat's centre. */
+gs3d_status gs3d_viewer_set_depth_stencil(gs3d_viewer* v, int enabled, gs3d_compare compare, int depth_write);
+/* Viewer::render drawn through Renderer::render_with_pass into a caller's
+ * pass that has a depth attachment (src/renderer.rs:186-195): `depth` is a
+ * GS3D_FORMAT_DEPTH32_FLOAT texture of the target's size; load != 0 keeps the
+ * target's colour (LoadOp::Load), otherwise it is cleared to black. */
+gs3d_status gs3d_viewer_render_with_depth(gs3d_viewer* v, gs3d_texture* target, gs3d_texture* depth, int load);
 /* Preprocessor::preprocess (src/preprocessor.rs:283-290). */
 gs3d_status gs3d_viewer_preprocess(gs3d_viewer* v);
 /* RadixSorter::sort (src/radix_sorter.rs:56-66). */
@@ -251,6 +289,13 @@ gs3d_status gs3d_multi_model_viewer_update_camera_with_pod(gs3d_multi_model_view
 gs3d_status gs3d_multi_model_viewer_update_model_transform(gs3d_multi_model_viewer* v, const char* key,
                                                            const float pos[3], const float rot[4],
                                                            const float scale[3]);
+/* The selection buffers of one model (MultiModelViewerGaussianBuffers,
+ * src/multi_model.rs:81-84), as gs3d_viewer_update_selection and
+ * gs3d_viewer_update_invert_selection; ModelNotFound for an unknown key. */
+gs3d_status gs3d_multi_model_viewer_update_selection(gs3d_multi_model_viewer* v, const char* key,
+                                                     const uint32_t* words, uint32_t word_count);
+gs3d_status gs3d_multi_model_viewer_update_invert_selection(gs3d_multi_model_viewer* v, const char* key,
+                                                            int invert);
 /* MultiModelViewer::update_gaussian_transform (src/multi_model.rs:446-464). */
 gs3d_status gs3d_multi_model_viewer_update_gaussian_transform(gs3d_multi_model_viewer* v, float size,
                                                               gs3d_display_mode display_mode, uint32_t sh_deg,

@@ -43,11 +43,11 @@ class Frame(C.Structure):
 SPLAT_F32 = np.dtype(
     [("cx", "<f4"), ("cy", "<f4"), ("ux", "<f4"), ("uy", "<f4"), ("vx", "<f4"), ("vy", "<f4"),
      ("r", "<f4"), ("g", "<f4"), ("b", "<f4"), ("a", "<f4"), ("hx", "<f4"), ("hy", "<f4"),
-     ("drawable", "<i4"), ("pad", "<i4")])
+     ("drawable", "<i4"), ("pad", "<i4"), ("z", "<f4")])
 SPLAT_F64 = np.dtype(
     [("cx", "<f8"), ("cy", "<f8"), ("ux", "<f8"), ("uy", "<f8"), ("vx", "<f8"), ("vy", "<f8"),
      ("r", "<f8"), ("g", "<f8"), ("b", "<f8"), ("a", "<f8"), ("hx", "<f8"), ("hy", "<f8"),
-     ("drawable", "<i4"), ("pad", "<i4")])
+     ("drawable", "<i4"), ("pad", "<i4"), ("z", "<f8")])
 
 _lib = None
 
@@ -90,6 +90,8 @@ def _declare(L):
     L.orc_splats_f32.argtypes = [F, C.c_uint32, u32p, fp, u8p, fp, fp, C.c_void_p]
     L.orc_splats_f64.argtypes = [F, C.c_uint32, u32p, dp, u8p, dp, dp, C.c_void_p]
     L.orc_render_f32.argtypes = [F, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]
+    L.orc_render_depth_f32.argtypes = [F, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_int32, C.c_void_p,
+                                       C.c_int32, fp, C.c_int32, C.c_int32]
     L.orc_render_f64.argtypes = [F, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, dp, C.c_int32]
     L.orc_stable_sort.argtypes = [C.c_uint32, u32p, u32p]
     L.orc_frame_f32.argtypes = [F, C.c_uint32, fp, u8p, fp, fp, C.c_uint32, C.c_uint32, C.c_int32,
@@ -259,9 +261,20 @@ def splats(frame: Frame, idx, pos, color, sh, cov3d, f64=False):
     return out[:v]
 
 
-def render(frame: Frame, w, h, sp, target=TARGET_RGBA8_STEP, threads=1, f64=False):
-    """Rasterise and blend splat records in draw order; returns (h, w, 4)."""
+def render(frame: Frame, w, h, sp, target=TARGET_RGBA8_STEP, threads=1, f64=False, depth=None, compare=8,
+           depth_write=False):
+    """Rasterise and blend splat records in draw order; returns (h, w, 4).
+    depth: an (h, w) f32 depth attachment tested with the wgpu compare
+    function `compare` (1 never .. 8 always) and updated in place when
+    depth_write (f32 targets only)."""
     v = sp.shape[0]
+    if depth is not None:
+        assert not f64 and depth.dtype == np.float32 and depth.shape == (h, w) and depth.flags.c_contiguous
+        sp = np.ascontiguousarray(sp, dtype=SPLAT_F32)
+        out = np.zeros((h, w, 4), np.float32 if target == TARGET_RGBA32F else np.uint8)
+        lib().orc_render_depth_f32(C.byref(frame), w, h, v, sp.ctypes.data, target, out.ctypes.data, threads,
+                                   _p(depth, C.c_float), int(compare), int(bool(depth_write)))
+        return out
     if f64:
         sp = np.ascontiguousarray(sp, dtype=SPLAT_F64)
         out = np.zeros((h, w, 4), np.float64)

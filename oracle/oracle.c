@@ -255,8 +255,21 @@ static int splat_rect(const orc_splat_f32* s, uint32_t w, int32_t y0, int32_t y1
     return 1;
 }
 
-void orc_render_f32(const orc_frame* f, uint32_t w, uint32_t h, uint32_t v, const orc_splat_f32* s,
-                    int32_t target, void* out, int32_t threads) {
+static int depth_pass(int32_t cmp, float z, float d) {
+    switch (cmp) {
+        case 1: return 0;
+        case 2: return z < d;
+        case 3: return z == d;
+        case 4: return z <= d;
+        case 5: return z > d;
+        case 6: return z != d;
+        case 7: return z >= d;
+        default: return 1;
+    }
+}
+
+static void render_f32(const orc_frame* f, uint32_t w, uint32_t h, uint32_t v, const orc_splat_f32* s,
+                       int32_t target, void* out, int32_t threads, float* depth, int32_t cmp, int32_t write) {
     const int32_t band = 16;
     const int32_t nb = (int32_t)((h + band - 1) / band);
     if (threads <= 0) threads = 1;
@@ -287,6 +300,11 @@ void orc_render_f32(const orc_frame* f, uint32_t w, uint32_t h, uint32_t v, cons
                 for (int32_t x = x0; x < x1; ++x) {
                     float al = frag_alpha_f32(f, sp, (float)x + 0.5f, (float)y + 0.5f);
                     if (al < 0.0f) continue;
+                    if (depth) {
+                        float* dz = depth + (size_t)y * w + (size_t)x;
+                        if (!depth_pass(cmp, sp->z, *dz)) continue;
+                        if (write) *dz = sp->z;
+                    }
                     if (target != ORC_TARGET_RGBA32F) al = clamp01(al);
                     const float om = 1.0f - al;
                     const size_t p = (size_t)(y - y0) * w + (size_t)x;
@@ -315,6 +333,16 @@ void orc_render_f32(const orc_frame* f, uint32_t w, uint32_t h, uint32_t v, cons
         free(acc);
         free(acc8);
     }
+}
+
+void orc_render_f32(const orc_frame* f, uint32_t w, uint32_t h, uint32_t v, const orc_splat_f32* s,
+                    int32_t target, void* out, int32_t threads) {
+    render_f32(f, w, h, v, s, target, out, threads, NULL, 8, 0);
+}
+
+void orc_render_depth_f32(const orc_frame* f, uint32_t w, uint32_t h, uint32_t v, const orc_splat_f32* s,
+                          int32_t target, void* out, int32_t threads, float* depth, int32_t cmp, int32_t write) {
+    render_f32(f, w, h, v, s, target, out, threads, depth, cmp, write);
 }
 
 void orc_render_f64(const orc_frame* f, uint32_t w, uint32_t h, uint32_t v, const orc_splat_f64* s,

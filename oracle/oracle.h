@@ -92,6 +92,7 @@ typedef struct orc_splat_f32 {
     float hx, hy;      /* point mode: half extents in pixels */
     int32_t drawable;
     int32_t pad;
+    float z;           /* fragment depth: clip z / w of the centre */
 } orc_splat_f32;
 
 typedef struct orc_splat_f64 {
@@ -101,6 +102,7 @@ typedef struct orc_splat_f64 {
     double hx, hy;
     int32_t drawable;
     int32_t pad;
+    double z;
 } orc_splat_f64;
 
 /* ---- camera (src/camera.rs:85-92; glam look_to_rh / perspective_rh) ---- */
@@ -146,6 +148,12 @@ void orc_splats_f64(const orc_frame* f, uint32_t v, const uint32_t* idx, const d
  * (RGBA8 targets) or w*h*4 f32 (RGBA32F). threads <= 0 means 1. */
 void orc_render_f32(const orc_frame* f, uint32_t w, uint32_t h, uint32_t v,
                     const orc_splat_f32* s, int32_t target, void* out, int32_t threads);
+/* orc_render_f32 with a depth attachment (a pipeline given a depth_stencil,
+ * src/renderer.rs:300-305): `depth` (w*h f32, row-major) is tested with the
+ * wgpu::CompareFunction `cmp` (1 never .. 8 always) for every fragment that
+ * survives the discard, in draw order, and written when `write`. */
+void orc_render_depth_f32(const orc_frame* f, uint32_t w, uint32_t h, uint32_t v, const orc_splat_f32* s,
+                          int32_t target, void* out, int32_t threads, float* depth, int32_t cmp, int32_t write);
 void orc_render_f64(const orc_frame* f, uint32_t w, uint32_t h, uint32_t v,
                     const orc_splat_f64* s, double* out, int32_t threads);
 

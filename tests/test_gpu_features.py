@@ -296,3 +296,129 @@ def test_ply_model_renders_like_oracle(gs, orc, device):
     assert np.array_equal(v.gaussians_depth(vis).view(np.uint32), okey.view(np.uint32))
     print(f"model.ply: max diff {max_abs_diff(img, oimg)}/255 PSNR {psnr(img, oimg):.1f} dB", flush=True)
     assert max_abs_diff(img, oimg) <= 2 and psnr(img, oimg) >= 45.0
+
+
+# ---- selection early-out (preprocess.wesl:69-78) ---------------------------------
+
+def test_selection_early_out(gs, orc, device):
+    """Visible set with a selection bitset: the oracle's visible set with the
+    Gaussians whose bit equals the invert flag removed, for both flags, plus
+    the all-zero default and the bitset's length check."""
+    _case("selection early-out")
+    g, pod, w, h = _small_scene(gs, n=5000)
+    v = gs.Viewer(device, gs.TextureFormat.RGBA8_UNORM, g)
+    v.update_camera_with_pod(pod)
+    t = gs.Texture(device, w, h, gs.TextureFormat.RGBA8_UNORM)
+    frame = orc.frame_from_pod(pod.to_bytes())
+    pos, color, sh, cov = oracle_inputs(orc, g)
+    oidx, okey, _ = orc.full_frame(frame, pos, color, sh, cov, w, h, orc.TARGET_RGBA8_F32)
+    rng = np.random.default_rng(5)
+    bits = rng.integers(0, 2, size=g.shape[0]).astype(bool)
+    words = np.zeros((g.shape[0] + 31) // 32, np.uint32)
+    for i in np.flatnonzero(bits):
+        words[i // 32] |= np.uint32(1 << (i % 32))
+    # default: bitset zero, invert true -> nothing hidden
+    v.render(t)
+    device.poll()
+    assert np.array_equal(v.indirect_indices(int(v.indirect_args()[1])), oidx)
+    v.update_selection(words)
+    for invert in (True, False):
+        v.update_invert_selection(invert)
+        v.render(t)
+        device.poll()
+        vis = int(v.indirect_args()[1])
+        keep = oidx[bits[oidx] != invert]
+        got = v.indirect_indices(vis)
+        assert vis == len(keep) and np.array_equal(got, keep), f"invert={invert}"
+        # keys of the kept Gaussians are the oracle's
+        okeep = okey[bits[oidx] != invert]
+        assert np.array_equal(v.gaussians_depth(vis).view(np.uint32), okeep.view(np.uint32))
+    with pytest.raises(gs.Gs3dError, match="selection needs"):
+        v.update_selection(words[:-1])
+    v.close()
+
+
+# ---- depth attachment (ViewerCreateOptions::depth_stencil) ----------------------
+
+def _depth_plane(sp, w, h):
+    """A depth attachment: the left half at a depth in the widest gap between
+    the middle half of the splats' fragment depths, the right half at 1."""
+    z = np.sort(sp["z"][sp["drawable"] != 0].astype(np.float64))
+    lo, hi = len(z) // 4, 3 * len(z) // 4
+    gaps = np.diff(z[lo:hi])
+    k = int(np.argmax(gaps))
+    zt = np.float32(0.5 * (z[lo + k] + z[lo + k + 1]))
+    assert gaps[k] > 1e-6
+    d = np.ones((h, w), np.float32)
+    d[:, : w // 2] = zt
+    return d
+
+
+@pytest.mark.parametrize("blend,write", [("fast", False), ("fast", True), ("exact", False), ("exact", True)])
+def test_depth_attachment_matches_oracle(gs, orc, device, blend, write):
+    _case(f"depth attachment blend={blend} write={write}")
+    g, pod, w, h = _small_scene(gs)
+    v = gs.Viewer(device, gs.TextureFormat.RGBA8_UNORM, g)
+    v.update_camera_with_pod(pod)
+    v.set_blend_mode(gs.BlendMode.FAST if blend == "fast" else gs.BlendMode.EXACT)
+    v.set_depth_stencil(True, gs.CompareFunction.LESS, write)
+    frame = orc.frame_from_pod(pod.to_bytes())
+    pos, color, sh, cov = oracle_inputs(orc, g)
+    oidx, _, _ = orc.full_frame(frame, pos, color, sh, cov, w, h, orc.TARGET_RGBA8_F32)
+    sp = orc.splats(frame, oidx, pos, color, sh, cov)
+    d0 = _depth_plane(sp, w, h)
+    depth = gs.Texture(device, w, h, gs.TextureFormat.DEPTH32_FLOAT)
+    depth.write(d0)
+    t = gs.Texture(device, w, h, gs.TextureFormat.RGBA8_UNORM)
+    v.render_with_depth(t, depth)
+    img = t.read()
+    target = orc.TARGET_RGBA8_F32 if blend == "fast" else orc.TARGET_RGBA8_STEP
+    # the exact blend tests in draw order against the attachment as updated;
+    # the fast blend against the attachment as it was before the draw (the
+    # same for Less except where two splats' depth order and draw order
+    # disagree by rounding)
+    od = d0.copy()
+    oimg = orc.render(frame, w, h, sp, target, depth=od, compare=2, depth_write=write and blend == "exact")
+    od_seq = d0.copy()
+    orc.render(frame, w, h, sp, target, depth=od_seq, compare=2, depth_write=write)
+    plain = orc.render(frame, w, h, sp, target)
+    print(f"depth {blend} write={write}: max diff {max_abs_diff(img, oimg)}/255, "
+          f"{np.mean(np.any(oimg != plain, axis=2)) * 100:.1f}% of pixels changed by the test", flush=True)
+    assert np.any(oimg != plain)  # the plane hides something
+    assert max_abs_diff(img, oimg) <= 2
+    gd = depth.read()
+    if write:
+        # the nearest kept, passing fragment's depth (clip z / w, ulps apart)
+        close = np.isclose(gd, od_seq, rtol=2e-6, atol=0)
+        print(f"depth written: {close.mean() * 100:.3f}% of pixels as the oracle's", flush=True)
+        assert close.mean() >= 0.999
+        assert np.any(gd != d0)
+    else:
+        assert np.array_equal(gd, d0)
+    # a depth-tested renderer needs the attachment, and one of the target's size
+    with pytest.raises(gs.Gs3dError, match="depth attachment"):
+        v.render(t)
+    small = gs.Texture(device, w // 2, h, gs.TextureFormat.DEPTH32_FLOAT)
+    with pytest.raises(gs.Gs3dError, match="Depth32Float of the target's size"):
+        v.render_with_depth(t, small)
+    v.set_depth_stencil(False)
+    v.render(t)
+    device.poll()
+    v.close()
+
+
+def test_handles_outlive_their_device_handle(gs):
+    """Destroying a device handle before the viewers and textures made from
+    it only drops a reference: the rest are still usable and destroyable."""
+    _case("device destroyed first")
+    dev = gs.Device(0)
+    g, pod, w, h = _small_scene(gs, n=512)
+    v = gs.Viewer(dev, gs.TextureFormat.RGBA8_UNORM, g)
+    v.update_camera_with_pod(pod)
+    t = gs.Texture(dev, w, h, gs.TextureFormat.RGBA8_UNORM)
+    dev.close()
+    v.render(t)
+    img = t.read()
+    assert img.shape == (h, w, 4)
+    v.close()
+    t.close()

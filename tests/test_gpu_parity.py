@@ -217,9 +217,9 @@ def test_fp16_storage_error_vs_fp64_oracle(gs, orc, device):
           flush=True)
     print(f"f32 storage vs fp64: rgb max {h32[0]:.3e} mean {h32[1]:.3e} p99.9 {h32[2]:.3e} alpha max {h32[3]:.3e}",
           flush=True)
-    # stated bounds: colour within 16/255 everywhere and a mean error below
-    # half a unorm8 step; the target's alpha stays 1 (cleared to 1, over
-    # operator)
-    assert h16[0] <= 16.0 / 255 and h16[1] <= 0.5 / 255
-    assert h32[0] <= 4.0 / 255 and h32[1] <= 0.25 / 255
+    # stated bounds: colour within 2/255 everywhere and a mean error below
+    # 0.05/255 (measured: f16 max 0.92/255, mean 0.006/255; f32 max 0.15/255);
+    # the target's alpha stays 1 (cleared to 1, over operator)
+    assert h16[0] <= 2.0 / 255 and h16[1] <= 0.05 / 255
+    assert h32[0] <= 1.0 / 255 and h32[1] <= 0.05 / 255
     assert h16[3] <= 1e-5 and h32[3] <= 1e-5

@@ -356,3 +356,67 @@ def test_reference_predicate_behind_camera(orc, gs):
     idx, img = _reference_single_red(orc, gs, pos=(0.0, 0.0, -1.0))  # tests/e2e/viewer.rs:157-201
     assert idx.size == 0
     assert img[..., :3].sum() == 0
+
+
+# ---- depth attachment (pipeline depth_stencil, src/renderer.rs:300-305) ----------
+
+def _two_splat_scene(orc):
+    """A far red splat and a near half-transparent blue one on the axis,
+    drawn far first; their fragment depths are the centres' clip z / w."""
+    view, proj, frame = _camera(orc, eye=(0, 0, 0), target=(0, 0, -1), w=32, h=32)
+    pos = np.array([[0, 0, -2.0], [0, 0, -1.0]], np.float32)
+    cov = np.array([[1e-2, 0, 0, 1e-2, 0, 1e-2]] * 2, np.float32)  # 3 sd: ~4 and ~8 px
+    col = np.array([[255, 0, 0, 255], [0, 0, 255, 128]], np.uint8)
+    sp = orc.splats(frame, np.array([0, 1], np.uint32), pos, col, np.zeros((2, 45), np.float32), cov)
+    return frame, proj, sp
+
+
+def test_fragment_depth_closed_form(orc):
+    # perspective_rh (depth 0..1): z_ndc = far / (far - near) * (1 - near / d)
+    frame, proj, sp = _two_splat_scene(orc)
+    n, f = 0.1, 1e4
+    for k, d in enumerate((2.0, 1.0)):
+        assert sp["z"][k] == pytest.approx(f / (f - n) * (1.0 - n / d), rel=1e-6)
+    assert sp["z"][1] < sp["z"][0]
+
+
+def test_depth_test_compare_functions(orc):
+    frame, proj, sp = _two_splat_scene(orc)
+    plain = orc.render(frame, 32, 32, sp, orc.TARGET_RGBA32F)
+    ones = lambda: np.ones((32, 32), np.float32)  # noqa: E731
+    zeros = lambda: np.zeros((32, 32), np.float32)  # noqa: E731
+    # every drawn fragment lies in front of the far plane: Less against 1.0,
+    # and Always, draw everything; Never and Less against 0.0 draw nothing
+    assert np.array_equal(orc.render(frame, 32, 32, sp, orc.TARGET_RGBA32F, depth=ones(), compare=2), plain)
+    assert np.array_equal(orc.render(frame, 32, 32, sp, orc.TARGET_RGBA32F, depth=zeros(), compare=8), plain)
+    black = orc.render(frame, 32, 32, sp[:0], orc.TARGET_RGBA32F)
+    assert np.array_equal(orc.render(frame, 32, 32, sp, orc.TARGET_RGBA32F, depth=ones(), compare=1), black)
+    assert np.array_equal(orc.render(frame, 32, 32, sp, orc.TARGET_RGBA32F, depth=zeros(), compare=2), black)
+    # a plane between the two: only the near splat passes Less, only the far
+    # one passes Greater
+    mid = np.full((32, 32), 0.5 * (sp["z"][0] + sp["z"][1]), np.float32)
+    near_only = orc.render(frame, 32, 32, sp[1:], orc.TARGET_RGBA32F)
+    far_only = orc.render(frame, 32, 32, sp[:1], orc.TARGET_RGBA32F)
+    assert np.array_equal(orc.render(frame, 32, 32, sp, orc.TARGET_RGBA32F, depth=mid.copy(), compare=2), near_only)
+    assert np.array_equal(orc.render(frame, 32, 32, sp, orc.TARGET_RGBA32F, depth=mid.copy(), compare=5), far_only)
+    assert far_only[16, 16, 0] > 0 and near_only[16, 16, 2] > 0
+
+
+def test_depth_write_follows_draw_order(orc):
+    frame, proj, sp = _two_splat_scene(orc)
+    # drawn near first with depth writes and Less, the far splat then fails
+    # wherever the near one kept a fragment
+    rev = sp[::-1].copy()
+    d = np.ones((32, 32), np.float32)
+    img = orc.render(frame, 32, 32, rev, orc.TARGET_RGBA32F, depth=d, compare=2, depth_write=True)
+    near_only = orc.render(frame, 32, 32, sp[1:], orc.TARGET_RGBA32F)
+    near_mask = near_only[..., 2] > 0
+    assert np.array_equal(img[near_mask], near_only[near_mask])
+    assert np.all(d[near_mask] == sp["z"][1])
+    # without writes the far splat blends over the near one there
+    img2 = orc.render(frame, 32, 32, rev, orc.TARGET_RGBA32F, depth=np.ones((32, 32), np.float32), compare=2)
+    assert img2[16, 16, 0] > 0 and img[16, 16, 0] == 0
+    # back to front (the renderer's order) a write changes nothing for Less
+    d3 = np.ones((32, 32), np.float32)
+    img3 = orc.render(frame, 32, 32, sp, orc.TARGET_RGBA32F, depth=d3, compare=2, depth_write=True)
+    assert np.array_equal(img3, orc.render(frame, 32, 32, sp, orc.TARGET_RGBA32F))

@@ -57,6 +57,19 @@ class ModelNotFound(Gs3dError):
 class TextureFormat(enum.IntEnum):
     RGBA8_UNORM = 0
     RGBA32_FLOAT = 1
+    DEPTH32_FLOAT = 2  # depth attachment
+
+
+class CompareFunction(enum.IntEnum):
+    """wgpu::CompareFunction."""
+    NEVER = 1
+    LESS = 2
+    EQUAL = 3
+    LESS_EQUAL = 4
+    GREATER = 5
+    NOT_EQUAL = 6
+    GREATER_EQUAL = 7
+    ALWAYS = 8
 
 
 class GaussianPod(enum.IntEnum):
@@ -180,6 +193,13 @@ def _declare(L):
         "gs3d_viewer_set_tile_size": ([vp, u32, u32], i32),
         "gs3d_viewer_set_profiling": ([vp, i32], i32),
         "gs3d_viewer_stage_times": ([vp, vp], i32),
+        "gs3d_viewer_update_selection": ([vp, vp, u32], i32),
+        "gs3d_viewer_set_depth_stencil": ([vp, i32, i32, i32], i32),
+        "gs3d_viewer_render_with_depth": ([vp, vp, vp, i32], i32),
+        "gs3d_texture_write": ([vp, vp, C.c_size_t], i32),
+        "gs3d_viewer_update_invert_selection": ([vp, i32], i32),
+        "gs3d_multi_model_viewer_update_selection": ([vp, C.c_char_p, vp, u32], i32),
+        "gs3d_multi_model_viewer_update_invert_selection": ([vp, C.c_char_p, i32], i32),
         "gs3d_viewer_count_fragments": ([vp, vp, vp], i32),
         "gs3d_viewer_gaussian_count": ([vp, vp], i32),
         "gs3d_viewer_read_indirect_args": ([vp, vp], i32),
@@ -238,6 +258,9 @@ EXPORTED_SYMBOLS = (
     "gs3d_viewer_update_gaussian_transform", "gs3d_viewer_render", "gs3d_viewer_preprocess", "gs3d_viewer_sort",
     "gs3d_viewer_draw", "gs3d_viewer_draw_standalone", "gs3d_viewer_set_blend_mode", "gs3d_viewer_set_tile_size",
     "gs3d_viewer_set_profiling", "gs3d_viewer_stage_times", "gs3d_viewer_count_fragments",
+    "gs3d_viewer_update_selection", "gs3d_viewer_update_invert_selection",
+    "gs3d_viewer_set_depth_stencil", "gs3d_viewer_render_with_depth", "gs3d_texture_write",
+    "gs3d_multi_model_viewer_update_selection", "gs3d_multi_model_viewer_update_invert_selection",
     "gs3d_viewer_gaussian_count",
     "gs3d_viewer_read_indirect_args", "gs3d_viewer_read_radix_sort_indirect_args",
     "gs3d_viewer_read_indirect_indices", "gs3d_viewer_read_gaussians_depth", "gs3d_depth_buffer_len",
@@ -407,11 +430,20 @@ class Texture:
         return self._h
 
     def read(self) -> np.ndarray:
-        """(height, width, 4) uint8 or float32, top row first."""
-        dt = np.uint8 if self.format == TextureFormat.RGBA8_UNORM else np.float32
-        out = np.zeros((self.height, self.width, 4), dt)
+        """(height, width, 4) uint8 or float32 ((height, width) float32 for a
+        depth texture), top row first."""
+        if self.format == TextureFormat.DEPTH32_FLOAT:
+            out = np.zeros((self.height, self.width), np.float32)
+        else:
+            dt = np.uint8 if self.format == TextureFormat.RGBA8_UNORM else np.float32
+            out = np.zeros((self.height, self.width, 4), dt)
         _check(lib().gs3d_texture_read(self._h, _ptr(out), out.nbytes))
         return out
+
+    def write(self, data: np.ndarray) -> None:
+        """queue.write_texture: the whole texture, in read()'s layout."""
+        a = np.ascontiguousarray(data)
+        _check(lib().gs3d_texture_write(self._h, _ptr(a), a.nbytes))
 
     def close(self) -> None:
         if self._h:
@@ -488,6 +520,24 @@ class Viewer:
         out = np.zeros(6, np.float32)
         _check(lib().gs3d_viewer_stage_times(self._h, _ptr(out)))
         return out
+
+    def set_depth_stencil(self, enabled: bool, compare: "CompareFunction" = None, depth_write: bool = False):
+        """ViewerCreateOptions::depth_stencil: a depth test for the renderer."""
+        cmp = CompareFunction.LESS if compare is None else compare
+        _check(lib().gs3d_viewer_set_depth_stencil(self._h, int(bool(enabled)), int(cmp), int(bool(depth_write))))
+
+    def render_with_depth(self, target: "Texture", depth: "Texture", load: bool = False) -> None:
+        """Render into target with a Depth32Float attachment (render_with_pass)."""
+        _check(lib().gs3d_viewer_render_with_depth(self._h, target.handle, depth.handle, int(bool(load))))
+
+    def update_selection(self, words) -> None:
+        """SelectionBuffer contents: one bit per Gaussian, ceil(N / 32) u32 words."""
+        w = np.ascontiguousarray(words, np.uint32)
+        _check(lib().gs3d_viewer_update_selection(self._h, _ptr(w) if w.size else None, w.size))
+
+    def update_invert_selection(self, invert: bool) -> None:
+        """PreprocessorInvertSelectionBuffer::update (initially true)."""
+        _check(lib().gs3d_viewer_update_invert_selection(self._h, int(bool(invert))))
 
     def count_fragments(self, target: Texture) -> int:
         """Fragments of one frame that pass the discard test (waits)."""
@@ -642,6 +692,14 @@ class MultiModelViewer:
 
     def set_blend_mode(self, mode: BlendMode):
         _check(lib().gs3d_multi_model_viewer_set_blend_mode(self._h, int(mode)))
+
+    def update_selection(self, key: str, words) -> None:
+        w = np.ascontiguousarray(words, np.uint32)
+        _check(lib().gs3d_multi_model_viewer_update_selection(self._h, key.encode(), _ptr(w) if w.size else None,
+                                                              w.size))
+
+    def update_invert_selection(self, key: str, invert: bool) -> None:
+        _check(lib().gs3d_multi_model_viewer_update_invert_selection(self._h, key.encode(), int(bool(invert))))
 
     def render(self, target: Texture, keys: Sequence[str]):
         arr = (C.c_char_p * max(len(keys), 1))(*[k.encode() for k in keys])

@@ -43,6 +43,12 @@ void MultiModelViewer::update_camera_with_pod(const CameraPod& pod) {
   for (auto& kv : models_) kv.second->update_camera_with_pod(pod);
 }
 
+Viewer& MultiModelViewer::model(const std::string& key) {
+  auto it = models_.find(key);
+  if (it == models_.end()) throw Error(GS3D_ERROR_MODEL_NOT_FOUND, "model with the given key does not exist");
+  return *it->second;
+}
+
 void MultiModelViewer::update_model_transform(const std::string& key, const float pos[3], const float rot[4],
                                               const float scale[3]) {
   auto it = models_.find(key);
@@ -80,8 +86,10 @@ void MultiModelViewer::render(Texture& target, const std::vector<std::string>& k
   }
   // src/multi_model.rs:491-503: every model is preprocessed and sorted ...
   for (Viewer* v : vs) {
+    v->splats_buffer.set_rect_grid(v->renderer.grid_for(target));
+    v->preprocessor.set_sort_hist(v->radix_sorter.hist(), v->radix_sorter.len());
     v->preprocessor.preprocess(v->gaussians_buffer.len());
-    v->radix_sorter.sort(v->radix_sort_indirect_args_buffer);
+    v->radix_sorter.sort(v->radix_sort_indirect_args_buffer, true);
   }
   // ... then drawn in the caller's order into one pass (:505-527)
   for (size_t i = 0; i < vs.size(); ++i) vs[i]->renderer.render(target, vs[i]->indirect_args_buffer, false, i == 0);
@@ -222,8 +230,14 @@ void BatchRenderer::autotune(const CameraPod* pods, int reps) {
     int best_k = 1;
     for (int k = 0; k < 3; ++k) {
       v.renderer.set_tile_size(sizes[k][0], sizes[k][1]);
-      v.render(*targets_[i]);  // warm: sizes the buffers for this tile size
+      // warm: sizes the buffers (and the tile lists) for this tile size, so
+      // that the timed draws below are complete frames
+      v.render(*targets_[i]);
       dev_.wait();
+      for (int g = 0; g < 4 && v.renderer.grow_if_overflowed(); ++g) {
+        v.render(*targets_[i]);
+        dev_.wait();
+      }
       hipEvent_t a, b;
       hip_check(hipEventCreate(&a), "hipEventCreate");
       hip_check(hipEventCreate(&b), "hipEventCreate");

@@ -43,6 +43,8 @@ class MultiModelViewer {
   void render(Texture& target, const std::vector<std::string>& keys);
   void set_blend(BlendMode b);
   size_t size() const { return models_.size(); }
+  // The viewer of one model; ModelNotFound for an unknown key.
+  Viewer& model(const std::string& key);
 
  private:
   Device& dev_;

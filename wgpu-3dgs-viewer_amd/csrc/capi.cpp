@@ -13,51 +13,64 @@
 #include "math.hpp"
 #include "viewer.hpp"
 
+// Every handle made from a device shares ownership of it (declared first,
+// released last): destroying the device handle while others are alive only
+// drops its reference.
+using DevicePtr = std::shared_ptr<gs3d::Device>;
+
+struct gs3d_device {
+  DevicePtr dev;
+  explicit gs3d_device(int ordinal) : dev(std::make_shared<gs3d::Device>(ordinal)) {}
+};
+
 struct gs3d_model {
+  DevicePtr keep;
   std::shared_ptr<gs3d::GaussiansBuffer> buf;
 };
 
 struct gs3d_multi_model_viewer {
+  DevicePtr keep;
   gs3d::MultiModelViewer v;
-  gs3d_multi_model_viewer(gs3d::Device& d, gs3d::TextureFormat f) : v(d, f) {}
+  gs3d_multi_model_viewer(DevicePtr d, gs3d::TextureFormat f) : keep(std::move(d)), v(*keep, f) {}
 };
 
 struct gs3d_batch {
+  DevicePtr keep;
   gs3d::BatchRenderer b;
-  gs3d_batch(gs3d::Device& d, std::vector<std::shared_ptr<gs3d::GaussiansBuffer>> m, int q) : b(d, std::move(m), q) {}
+  gs3d_batch(DevicePtr d, std::vector<std::shared_ptr<gs3d::GaussiansBuffer>> m, int q)
+      : keep(std::move(d)), b(*keep, std::move(m), q) {}
 };
 
 struct gs3d_server {
+  DevicePtr keep;
   gs3d::Server s;
-  gs3d_server(gs3d::Device& d, std::vector<std::shared_ptr<gs3d::GaussiansBuffer>> m, int q, uint64_t budget)
-      : s(d, std::move(m), q, budget) {}
-};
-
-struct gs3d_device {
-  gs3d::Device dev;
-  explicit gs3d_device(int ordinal) : dev(ordinal) {}
+  gs3d_server(DevicePtr d, std::vector<std::shared_ptr<gs3d::GaussiansBuffer>> m, int q, uint64_t budget)
+      : keep(std::move(d)), s(*keep, std::move(m), q, budget) {}
 };
 
 struct gs3d_texture {
+  DevicePtr keep;
   gs3d::Texture tex;
-  gs3d_texture(gs3d::Device& d, uint32_t w, uint32_t h, gs3d::TextureFormat f) : tex(d, w, h, f) {}
+  gs3d_texture(DevicePtr d, uint32_t w, uint32_t h, gs3d::TextureFormat f) : keep(std::move(d)), tex(*keep, w, h, f) {}
 };
 
 struct gs3d_viewer {
+  DevicePtr keep;
   gs3d::Viewer v;
-  gs3d_viewer(gs3d::Device& d, gs3d::TextureFormat f, const gs3d_gaussian* g, uint32_t n, gs3d::PodKind p)
-      : v(d, f, g, n, p) {}
+  gs3d_viewer(DevicePtr d, gs3d::TextureFormat f, const gs3d_gaussian* g, uint32_t n, gs3d::PodKind p)
+      : keep(std::move(d)), v(*keep, f, g, n, p) {}
 };
 
 struct gs3d_radix_sorter {
+  DevicePtr keep;
   gs3d::Device& dev;
   gs3d::IndirectIndicesBuffer vals;
   gs3d::GaussiansDepthBuffer keys;
   gs3d::DeviceBuffer count;
   gs3d::RadixSorter sorter;
   uint32_t n = 0;
-  gs3d_radix_sorter(gs3d::Device& d, uint32_t cap)
-      : dev(d), vals(d, cap), keys(d, cap), count(d, 64), sorter(d, keys, vals) {}
+  gs3d_radix_sorter(DevicePtr d, uint32_t cap)
+      : keep(std::move(d)), dev(*keep), vals(dev, cap), keys(dev, cap), count(dev, 64), sorter(dev, keys, vals) {}
 };
 
 namespace {
@@ -86,9 +99,16 @@ void need(bool ok, const char* what) {
   if (!ok) throw gs3d::Error(GS3D_ERROR_INVALID_ARGUMENT, what);
 }
 
+// a colour format (render targets of viewers)
 gs3d::TextureFormat to_format(gs3d_texture_format f) {
-  need(f == GS3D_FORMAT_RGBA8_UNORM || f == GS3D_FORMAT_RGBA32_FLOAT, "unknown texture format");
+  need(f == GS3D_FORMAT_RGBA8_UNORM || f == GS3D_FORMAT_RGBA32_FLOAT, "unknown colour texture format");
   return f == GS3D_FORMAT_RGBA8_UNORM ? gs3d::kFormatRgba8Unorm : gs3d::kFormatRgba32Float;
+}
+
+// any texture format (textures may also be depth attachments)
+gs3d::TextureFormat to_texture_format(gs3d_texture_format f) {
+  if (f == GS3D_FORMAT_DEPTH32_FLOAT) return gs3d::kFormatDepth32Float;
+  return to_format(f);
 }
 
 }  // namespace
@@ -110,21 +130,21 @@ void gs3d_device_destroy(gs3d_device* dev) { delete dev; }
 gs3d_status gs3d_device_poll(gs3d_device* dev) {
   return guard([&] {
     need(dev != nullptr, "device is null");
-    dev->dev.poll();
+    dev->dev->poll();
   });
 }
 
 gs3d_status gs3d_device_set_buffer_limit(gs3d_device* dev, uint64_t bytes) {
   return guard([&] {
     need(dev != nullptr, "device is null");
-    dev->dev.set_buffer_limit(bytes);
+    dev->dev->set_buffer_limit(bytes);
   });
 }
 
 gs3d_status gs3d_device_compute_units(gs3d_device* dev, int* out) {
   return guard([&] {
     need(dev != nullptr && out != nullptr, "null argument");
-    *out = dev->dev.num_cus();
+    *out = dev->dev->num_cus();
   });
 }
 
@@ -133,7 +153,7 @@ gs3d_status gs3d_texture_create(gs3d_device* dev, uint32_t width, uint32_t heigh
   return guard([&] {
     need(dev != nullptr && out != nullptr, "null argument");
     *out = nullptr;
-    *out = new gs3d_texture(dev->dev, width, height, to_format(format));
+    *out = new gs3d_texture(dev->dev, width, height, to_texture_format(format));
   });
 }
 
@@ -143,6 +163,13 @@ gs3d_status gs3d_texture_read(gs3d_texture* tex, void* dst, size_t bytes) {
   return guard([&] {
     need(tex != nullptr && dst != nullptr, "null argument");
     tex->tex.read(dst, bytes);
+  });
+}
+
+gs3d_status gs3d_texture_write(gs3d_texture* tex, const void* src, size_t bytes) {
+  return guard([&] {
+    need(tex != nullptr && src != nullptr, "null argument");
+    tex->tex.write(src, bytes);
   });
 }
 
@@ -209,6 +236,20 @@ gs3d_status gs3d_viewer_update_gaussian_transform(gs3d_viewer* v, float size, gs
   });
 }
 
+gs3d_status gs3d_viewer_update_selection(gs3d_viewer* v, const uint32_t* words, uint32_t word_count) {
+  return guard([&] {
+    need(v != nullptr && (words != nullptr || word_count == 0), "null argument");
+    v->v.update_selection(words, word_count);
+  });
+}
+
+gs3d_status gs3d_viewer_update_invert_selection(gs3d_viewer* v, int invert) {
+  return guard([&] {
+    need(v != nullptr, "viewer is null");
+    v->v.update_invert_selection(invert != 0);
+  });
+}
+
 gs3d_status gs3d_viewer_render(gs3d_viewer* v, gs3d_texture* target) {
   return guard([&] {
     need(v != nullptr && target != nullptr, "null argument");
@@ -227,6 +268,20 @@ gs3d_status gs3d_viewer_sort(gs3d_viewer* v) {
   return guard([&] {
     need(v != nullptr, "viewer is null");
     v->v.radix_sorter.sort(v->v.radix_sort_indirect_args_buffer);
+  });
+}
+
+gs3d_status gs3d_viewer_set_depth_stencil(gs3d_viewer* v, int enabled, gs3d_compare compare, int depth_write) {
+  return guard([&] {
+    need(v != nullptr, "viewer is null");
+    v->v.set_depth_stencil(enabled != 0, gs3d::CompareFn(int(compare)), depth_write != 0);
+  });
+}
+
+gs3d_status gs3d_viewer_render_with_depth(gs3d_viewer* v, gs3d_texture* target, gs3d_texture* depth, int load) {
+  return guard([&] {
+    need(v != nullptr && target != nullptr && depth != nullptr, "null argument");
+    v->v.render_with_depth(target->tex, depth->tex, load != 0);
   });
 }
 
@@ -460,7 +515,8 @@ gs3d_status gs3d_model_new(gs3d_device* dev, const gs3d_gaussian* gaussians, uin
     need(pod == GS3D_POD_SINGLE || pod == GS3D_POD_HALF, "unknown gaussian pod");
     *out = nullptr;
     auto m = std::make_unique<gs3d_model>();
-    m->buf = std::make_shared<gs3d::GaussiansBuffer>(dev->dev, gaussians, count,
+    m->keep = dev->dev;
+    m->buf = std::make_shared<gs3d::GaussiansBuffer>(*dev->dev, gaussians, count,
                                                      pod == GS3D_POD_SINGLE ? gs3d::kPodSingle : gs3d::kPodHalf);
     *out = m.release();
   });
@@ -547,6 +603,22 @@ gs3d_status gs3d_multi_model_viewer_update_model_transform(gs3d_multi_model_view
   return guard([&] {
     need(v != nullptr && key != nullptr && pos && rot && scale, "null argument");
     v->v.update_model_transform(key, pos, rot, scale);
+  });
+}
+
+gs3d_status gs3d_multi_model_viewer_update_selection(gs3d_multi_model_viewer* v, const char* key,
+                                                     const uint32_t* words, uint32_t word_count) {
+  return guard([&] {
+    need(v != nullptr && key != nullptr && (words != nullptr || word_count == 0), "null argument");
+    v->v.model(key).update_selection(words, word_count);
+  });
+}
+
+gs3d_status gs3d_multi_model_viewer_update_invert_selection(gs3d_multi_model_viewer* v, const char* key,
+                                                            int invert) {
+  return guard([&] {
+    need(v != nullptr && key != nullptr, "null argument");
+    v->v.model(key).update_invert_selection(invert != 0);
   });
 }
 

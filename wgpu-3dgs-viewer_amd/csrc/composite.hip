@@ -51,29 +51,42 @@ __device__ __forceinline__ float h_hi(uint32_t w) { return __half2float(__ushort
 
 // One batch of staged splats plus, per wave, the ordered slots that touch
 // the wave's 8x8 block.
-// Slot kDummy holds a splat of opacity 0 (blending it changes nothing): lanes
-// whose list is shorter than the wave's longest blend it. List rows carry
-// kListPad spare entries so that a step may read past a list's end.
-constexpr uint32_t kDummy = kCompMaxThreads;
+// Slot NT (one past the staged splats) holds a splat of opacity 0 (blending
+// it changes nothing): lanes whose list is shorter than the wave's longest
+// blend it. List rows carry kListPad spare entries so that a step may read
+// past a list's end.
 constexpr uint32_t kListPad = 4;
 
-// The fast path lists splats per 4x4 quadrant of each wave's 8x8 block (16
-// lists for a 256-pixel tile); the exact path per 8x8 block (lists 0..3).
-constexpr int kQuads = 4 * kCompMaxWaves;
-
-struct Stage {
-  float4 geo[kCompMaxThreads + 1];  // splat: cx, cy, ca*log2e, 2*cb*log2e   point: cx, cy, hx, hy
-  float4 col[kCompMaxThreads + 1];  // cc*log2e, opacity, r, g
-  float blue[kCompMaxThreads + 1];
-  uint16_t list[kQuads][kCompMaxThreads + kListPad];
-  uint32_t wcount[kCompMaxWaves][kQuads];  // [staging wave][target list]
+// LDS of a workgroup of NT lanes (one per pixel of its tile). The fast path
+// lists splats per 4x4 quadrant of each wave's 8x8 block (4 NT / 64 lists);
+// the exact path per 8x8 block (lists 0 .. NT / 64 - 1). Sized per tile
+// shape, so that 8x8 tiles (one wave) take a quarter of a 256-lane tile's
+// LDS and as many workgroups fit a CU.
+template <int NT>
+struct StageT {
+  static constexpr int kWaves = NT / kWave;
+  static constexpr int kQuads = 4 * kWaves;
+  static constexpr uint32_t kDummy = NT;
+  float4 geo[NT + 1];  // splat: cx, cy, ca*log2e, 2*cb*log2e   point: cx, cy, hx, hy
+  float4 col[NT + 1];  // cc*log2e, opacity (fast splat: log2 opacity), r, g
+  float blue[NT + 1];
+  float z[NT + 1];  // fragment depth, for depth-tested draws
+  uint16_t list[kQuads][NT + kListPad];
+  uint32_t wcount[kWaves][kWaves];  // exact path: [staging wave][target wave]
+  uint64_t qbal[kWaves][kQuads];    // fast path: [staging wave][quadrant] ballots
 };
+using Stage = StageT<kCompMaxThreads>;
 
-__device__ __forceinline__ void init_dummy(Stage& s) {
+// The fast path stages log2(opacity) for splat mode (its alpha is then one
+// exp2 of log2(opacity) - |qo|^2 log2(e)): the dummy's opacity 0 is -inf.
+template <int NT>
+__device__ __forceinline__ void init_dummy(StageT<NT>& s, int mode) {
+  constexpr uint32_t kDummy = StageT<NT>::kDummy;
   if (threadIdx.x == 0) {
     s.geo[kDummy] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    s.col[kDummy] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    s.col[kDummy] = make_float4(0.0f, mode == kModeSplat ? -__builtin_inff() : 0.0f, 0.0f, 0.0f);
     s.blue[kDummy] = 0.0f;
+    s.z[kDummy] = 0.0f;
   }
 }
 
@@ -103,8 +116,8 @@ __device__ __forceinline__ Tiling setup_tile(const TileGrid& g, uint32_t tile) {
 
 // Stage splat k (or nothing when !valid) into slot threadIdx.x and build the
 // per-wave lists. Contains __syncthreads; every lane must call it.
-__device__ __forceinline__ void stage_batch(Stage& s, const Tiling& t, const Splat2D* splats, bool valid, uint32_t k,
-                                            bool clamp_src, int mode) {
+__device__ __forceinline__ void stage_batch(Stage& s, const Tiling& t, const Splat2D* splats, const float* zsrc,
+                                            bool valid, uint32_t k, bool clamp_src, int mode) {
   const uint32_t tid = threadIdx.x, lane = tid % kWave, sw = tid / kWave;
   uint32_t mask = 0u;
   if (valid) {
@@ -124,6 +137,7 @@ __device__ __forceinline__ void stage_batch(Stage& s, const Tiling& t, const Spl
       s.col[tid] = make_float4(r.cc * kLog2e, op, cr, cg);
     }
     s.blue[tid] = cb;
+    if (zsrc) s.z[tid] = zsrc[k];
     // which 8x8 blocks of the tile the bound [c - h, c + h] can touch (pixel
     // centres of block (i, j) span [o + 8i + 0.5, o + 8i + 7.5]; half a pixel
     // of margin)
@@ -157,8 +171,10 @@ __device__ __forceinline__ void stage_batch(Stage& s, const Tiling& t, const Spl
 // Stage splat k (or nothing) into slot threadIdx.x and build one ordered list
 // per 4x4 quadrant of every wave's 8x8 block: quadrant (qx, qy) of wave w is
 // list 4w + qx + 2qy. Contains __syncthreads; every lane must call it.
-__device__ __forceinline__ void stage_batch_quads(Stage& s, const Tiling& t, const Splat2D* splats, bool valid,
-                                                  uint32_t k, bool clamp_src, int mode) {
+template <int NT>
+__device__ __forceinline__ void stage_batch_quads(StageT<NT>& s, const Tiling& t, const Splat2D* splats,
+                                                  const float* zsrc, bool valid, uint32_t k, bool clamp_src,
+                                                  int mode) {
   const uint32_t tid = threadIdx.x, lane = tid % kWave, sw = tid / kWave;
   uint32_t mask = 0u;
   if (valid) {
@@ -175,37 +191,42 @@ __device__ __forceinline__ void stage_batch_quads(Stage& s, const Tiling& t, con
       s.col[tid] = make_float4(0.0f, op, cr, cg);
     } else {
       s.geo[tid] = make_float4(r.cx, r.cy, r.ca * kLog2e, 2.0f * r.cb * kLog2e);
-      s.col[tid] = make_float4(r.cc * kLog2e, op, cr, cg);
+      // log2(0) = -inf: exp2 gives alpha 0
+      s.col[tid] = make_float4(r.cc * kLog2e, mode == kModeSplat ? __log2f(op) : op, cr, cg);
     }
     s.blue[tid] = cb;
-    // quadrant pixel centres span [o + 0.5, o + 3.5]: half a pixel of margin
+    if (zsrc) s.z[tid] = zsrc[k];
+    // Quadrant column c of the tile holds pixel centres [4c + 0.5, 4c + 3.5]
+    // (tile-relative); the bound [x0, x1] may touch it when x1 >= 4c and
+    // x0 <= 4c + 4 (half a pixel of margin). Same for rows. Quadrant (c, r)
+    // is list 4w + (c & 1) + 2 (r & 1) of wave w = c / 2 + blocks_x (r / 2).
     const float hx = h_lo(r.hxy), hy = h_hi(r.hxy);
     const float x0 = r.cx - hx - t.ox, x1 = r.cx + hx - t.ox;
     const float y0 = r.cy - hy - t.oy, y1 = r.cy + hy - t.oy;
-    for (uint32_t w = 0; w < t.waves; ++w) {
-      const float bx = float((w % t.blocks_x) * 8), by = float((w / t.blocks_x) * 8);
-#pragma unroll
-      for (uint32_t qd = 0; qd < 4; ++qd) {
-        const float qx = bx + float((qd & 1u) * 4), qy = by + float((qd >> 1) * 4);
-        const bool hit = !(x1 < qx) && !(x0 > qx + 4.0f) && !(y1 < qy) && !(y0 > qy + 4.0f);
-        mask |= hit ? (1u << (4 * w + qd)) : 0u;
-      }
-    }
+    const int ncols = int(t.blocks_x) * 2, nrows = int(t.waves / t.blocks_x) * 2;
+    const int c0 = max(0, int(ceilf((x0 - 4.0f) * 0.25f))), c1 = min(ncols - 1, int(floorf(x1 * 0.25f)));
+    const int r0 = max(0, int(ceilf((y0 - 4.0f) * 0.25f))), r1 = min(nrows - 1, int(floorf(y1 * 0.25f)));
+    uint32_t colmask = 0u;
+    for (int c = c0; c <= c1; ++c) colmask |= 1u << (4 * (c >> 1) + (c & 1));
+    for (int rr = r0; rr <= r1; ++rr) mask |= colmask << (4 * int(t.blocks_x) * (rr >> 1) + 2 * (rr & 1));
   }
+  constexpr int kQuads = StageT<NT>::kQuads;
   uint64_t bal[kQuads];
 #pragma unroll
-  for (int q = 0; q < kQuads; ++q) {
-    bal[q] = __ballot((mask >> q) & 1u);
-    if (lane == 0) s.wcount[sw][q] = uint32_t(__popcll(bal[q]));
+  for (int q = 0; q < kQuads; ++q) bal[q] = __ballot((mask >> q) & 1u);
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < kQuads; ++q) s.qbal[sw][q] = bal[q];
   }
   __syncthreads();
-#pragma unroll
-  for (int q = 0; q < kQuads; ++q) {
-    if ((mask >> q) & 1u) {
-      uint32_t off = 0;
-      for (uint32_t p = 0; p < sw; ++p) off += s.wcount[p][q];
-      s.list[q][off + mask_rank(bal[q])] = uint16_t(tid);
-    }
+  // each lane appends its slot to the lists of its quadrants (usually one or
+  // two): offset = the earlier staging waves' counts + rank in this wave
+  for (uint32_t m = mask; m != 0u; m &= m - 1u) {
+    const uint32_t q = uint32_t(__builtin_ctz(m));
+    uint32_t off = 0;
+    for (uint32_t p = 0; p < sw; ++p) off += uint32_t(__popcll(s.qbal[p][q]));
+    // quadrant lists hold the slot's byte offset in geo / col (16 per slot)
+    s.list[q][off + mask_rank(s.qbal[sw][q])] = uint16_t(tid * 16u);
   }
   __syncthreads();
 }
@@ -233,23 +254,51 @@ __device__ __forceinline__ float fragment(const Stage& s, uint32_t q, const Tili
   return col.y * __builtin_amdgcn_exp2f(-r2);
 }
 
+// Depth test of a fragment at depth z against the attachment's value d.
+__device__ __forceinline__ bool depth_pass(int cmp, float z, float d) {
+  switch (cmp) {
+    case kCmpNever: return false;
+    case kCmpLess: return z < d;
+    case kCmpEqual: return z == d;
+    case kCmpLessEqual: return z <= d;
+    case kCmpGreater: return z > d;
+    case kCmpNotEqual: return z != d;
+    case kCmpGreaterEqual: return z >= d;
+    default: return true;
+  }
+}
+
 // Fragment alpha for the fast path: 0 for a discarded fragment (blending
 // alpha 0 leaves colour and transmittance unchanged), so the loop has no
-// branches.
+// branches. `kept` tells whether the fragment survives the discard test.
 template <int MODE>
-__device__ __forceinline__ float fragment_alpha(float4 g, float4 c, const Tiling& t, float sd2l, float ell2l) {
+__device__ __forceinline__ float fragment_alpha(float4 g, float4 c, const Tiling& t, float sd2l, float ell2l,
+                                                bool& kept) {
 #pragma clang fp contract(fast)
   const float dx = t.fpx - g.x;
   const float dy = t.fpy - g.y;
-  if (MODE == kModePoint) return (dx >= -g.z && dx < g.z && dy >= -g.w && dy < g.w) ? c.y : 0.0f;
+  if (MODE == kModePoint) {
+    kept = dx >= -g.z && dx < g.z && dy >= -g.w && dy < g.w;
+    return kept ? c.y : 0.0f;
+  }
   const float r2 = dx * (g.z * dx + g.w * dy) + c.x * dy * dy;
   float al;
   if (MODE == kModeEllipse)
     al = c.y + (1.0f - c.y) * ((r2 > ell2l) ? 1.0f : 0.0f);
   else
-    al = c.y * __builtin_amdgcn_exp2f(-r2);
-  return (r2 <= sd2l) ? al : 0.0f;
+    al = __builtin_amdgcn_exp2f(c.y - r2);  // c.y = log2(opacity)
+  kept = r2 <= sd2l;
+  return kept ? al : 0.0f;
 }
+
+// Depth state of one lane of a depth-tested draw: the attachment's value
+// before the draw and the depth of the nearest kept, passing fragment.
+struct DepthLane {
+  int cmp;
+  float ref;
+  float first;
+  bool found;
+};
 
 // Splats blended per step of the fast loop (1, 2 or 4): more overlaps the
 // LDS reads of a step, fewer keeps registers for occupancy.
@@ -265,48 +314,68 @@ static_assert(kListPad % kUnroll == 0, "a step reads at most kListPad entries pa
 // lists blend the dummy slot. Lanes past the transmittance threshold keep
 // blending (their colour only gets closer to the exact sum) until the whole
 // wave is done.
-template <int MODE>
-__device__ __forceinline__ void blend_quads(const Stage& s, const Tiling& t, uint32_t w, float sd2l, float ell2l,
-                                            float& T, float& cr, float& cg, float& cb, bool& done) {
+template <int MODE, bool DEPTH, int NT>
+__device__ __forceinline__ void blend_quads(const StageT<NT>& s, const Tiling& t, uint32_t w, float sd2l, float ell2l,
+                                            float& T, float& cr, float& cg, float& cb, bool& done, DepthLane& dl) {
 #pragma clang fp contract(fast)
   const uint32_t lane = threadIdx.x % kWave;
   const uint32_t myq = 4u * w + (((lane >> 2) & 1u) | ((lane >> 5) << 1));
   uint32_t lens[4] = {0u, 0u, 0u, 0u};
   for (uint32_t p = 0; p < t.waves; ++p) {
 #pragma unroll
-    for (uint32_t qd = 0; qd < 4; ++qd) lens[qd] += s.wcount[p][4 * w + qd];
+    for (uint32_t qd = 0; qd < 4; ++qd) lens[qd] += uint32_t(__popcll(s.qbal[p][4 * w + qd]));
   }
   const uint32_t mylen = lens[myq & 3u];
-  const uint32_t wmax = max(max(lens[0], lens[1]), max(lens[2], lens[3]));
+  // the same for the whole wave: keep the loop bound in a scalar register
+  const uint32_t wmax =
+      __builtin_amdgcn_readfirstlane(max(max(lens[0], lens[1]), max(lens[2], lens[3])));
+  constexpr uint32_t kDummyOff = StageT<NT>::kDummy * 16u;
+  const char* geo = reinterpret_cast<const char*>(s.geo);
+  const char* col = reinterpret_cast<const char*>(s.col);
+  const char* blue = reinterpret_cast<const char*>(s.blue);
+  const char* zs = reinterpret_cast<const char*>(s.z);
   for (uint32_t j = 0; j < wmax; j += kUnroll) {
     if ((j & 31u) == 0u && __all(done)) break;
-    uint32_t q[kUnroll];
+    uint32_t q[kUnroll];  // byte offsets of the slots in geo / col
     if (kUnroll == 1) {
-      q[0] = j < mylen ? uint32_t(s.list[myq][j]) : kDummy;
+      q[0] = j < mylen ? uint32_t(s.list[myq][j]) : kDummyOff;
     } else {
 #pragma unroll
       for (int k = 0; k < kUnroll; k += 2) {
         const uint32_t qq = *reinterpret_cast<const uint32_t*>(&s.list[myq][j + k]);
-        q[k] = (j + k < mylen) ? (qq & 0xFFFFu) : kDummy;
-        q[k + 1 < kUnroll ? k + 1 : 0] = (j + k + 1 < mylen) ? (qq >> 16) : kDummy;
+        q[k] = (j + k < mylen) ? (qq & 0xFFFFu) : kDummyOff;
+        q[k + 1 < kUnroll ? k + 1 : 0] = (j + k + 1 < mylen) ? (qq >> 16) : kDummyOff;
       }
     }
     float4 g[kUnroll], c[kUnroll];
-    float bl[kUnroll];
+    float bl[kUnroll], zz[kUnroll];
 #pragma unroll
     for (int k = 0; k < kUnroll; ++k) {
-      g[k] = s.geo[q[k]];
-      c[k] = s.col[q[k]];
-      bl[k] = s.blue[q[k]];
+      g[k] = *reinterpret_cast<const float4*>(geo + q[k]);
+      c[k] = *reinterpret_cast<const float4*>(col + q[k]);
+      bl[k] = *reinterpret_cast<const float*>(blue + (q[k] >> 2));
+      if (DEPTH) zz[k] = *reinterpret_cast<const float*>(zs + (q[k] >> 2));
     }
 #pragma unroll
     for (int k = 0; k < kUnroll; ++k) {
-      const float al = fragment_alpha<MODE>(g[k], c[k], t, sd2l, ell2l);
+      bool kept;
+      float al = fragment_alpha<MODE>(g[k], c[k], t, sd2l, ell2l, kept);
+      if (DEPTH) {
+        // the test is against the attachment as it was before the draw: for
+        // Less / LessEqual and back-to-front order, what depth writes by
+        // farther splats would change only matters at equal depths
+        const bool pass = depth_pass(dl.cmp, zz[k], dl.ref) && q[k] != kDummyOff;
+        al = pass ? al : 0.0f;
+        if (kept && pass && !dl.found) {
+          dl.first = zz[k];
+          dl.found = true;
+        }
+      }
       const float wgt = T * al;
       cr += wgt * c[k].z;
       cg += wgt * c[k].w;
       cb += wgt * bl[k];
-      T *= 1.0f - al;
+      T = __builtin_fmaf(-T, al, T);
     }
     done = T < kMinTransmittance;
   }
@@ -373,6 +442,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t x, uint32_t* s
 }
 
 __global__ __launch_bounds__(kPlanTiles) void k_plan_count(CompositeArgs a) {
+  if (*a.pair_count > a.capacity) return;  // truncated lists: the frame is drawn again
   __shared__ uint32_t s_wave[2][kPlanTiles / kWave];
   const uint32_t nt = a.grid.tiles_x * a.grid.tiles_y;
   const uint32_t t = blockIdx.x * kPlanTiles + threadIdx.x;
@@ -392,6 +462,7 @@ __global__ __launch_bounds__(kPlanTiles) void k_plan_count(CompositeArgs a) {
 }
 
 __global__ __launch_bounds__(kPlanTiles) void k_plan(CompositeArgs a) {
+  if (*a.pair_count > a.capacity) return;  // truncated lists: the frame is drawn again
   __shared__ uint32_t s_wave[2][kPlanTiles / kWave];
   __shared__ uint32_t s_red[4][kPlanTiles / kWave];
   const uint32_t nt = a.grid.tiles_x * a.grid.tiles_y;
@@ -468,9 +539,11 @@ __global__ __launch_bounds__(kPlanTiles) void k_plan(CompositeArgs a) {
   }
 }
 
-__global__ __launch_bounds__(kCompMaxThreads) void k_composite_fast(CompositeArgs a) {
+template <bool DEPTH, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_composite_fast(CompositeArgs a) {
 #pragma clang fp contract(fast)
-  __shared__ Stage s;
+  if (*a.pair_count > a.capacity) return;  // truncated lists: the frame is drawn again
+  __shared__ StageT<NT> s;
   __shared__ uint32_t s_item;
   const FrameConsts& fc = *a.fc;
   const int mode = fc.display_mode;
@@ -480,7 +553,7 @@ __global__ __launch_bounds__(kCompMaxThreads) void k_composite_fast(CompositeArg
   const bool q8 = a.format == kFormatRgba8Unorm;
   const uint32_t nt = blockDim.x, tid = threadIdx.x, w = tid / kWave;
   const uint32_t items = a.plan[0], heavy = a.plan[1];
-  init_dummy(s);
+  init_dummy(s, mode);
 
   for (;;) {
     const uint32_t item = take_ticket(a.plan + 2, &s_item);
@@ -491,6 +564,9 @@ __global__ __launch_bounds__(kCompMaxThreads) void k_composite_fast(CompositeArg
     // segment counted from the front (nearest end) of the tile's list
     const uint32_t front = tend - seg * a.seg_len;
     const uint32_t back = (item < heavy && front - tstart > a.seg_len) ? front - a.seg_len : tstart;
+    const size_t o = size_t(t.py) * a.grid.width + t.px;
+    DepthLane dl{a.depth_cmp, 0.0f, 0.0f, false};
+    if (DEPTH && t.inside) dl.ref = a.depth[o];
 
     float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
     bool done = !t.inside;
@@ -499,21 +575,24 @@ __global__ __launch_bounds__(kCompMaxThreads) void k_composite_fast(CompositeArg
       __syncthreads();
       const bool valid = tid < cnt;
       // nearest first
-      stage_batch_quads(s, t, a.splats, valid, valid ? a.pair_vals[bend - 1 - tid] : 0u, q8, mode);
+      stage_batch_quads(s, t, a.splats, DEPTH ? a.splat_depth : nullptr, valid,
+                        valid ? a.pair_vals[bend - 1 - tid] : 0u, q8, mode);
       if (mode == kModeSplat)
-        blend_quads<kModeSplat>(s, t, w, sd2l, ell2l, T, cr, cg, cb, done);
+        blend_quads<kModeSplat, DEPTH, NT>(s, t, w, sd2l, ell2l, T, cr, cg, cb, done, dl);
       else if (mode == kModeEllipse)
-        blend_quads<kModeEllipse>(s, t, w, sd2l, ell2l, T, cr, cg, cb, done);
+        blend_quads<kModeEllipse, DEPTH, NT>(s, t, w, sd2l, ell2l, T, cr, cg, cb, done, dl);
       else
-        blend_quads<kModePoint>(s, t, w, sd2l, ell2l, T, cr, cg, cb, done);
+        blend_quads<kModePoint, DEPTH, NT>(s, t, w, sd2l, ell2l, T, cr, cg, cb, done, dl);
       bend -= cnt;
       if (__syncthreads_and(done ? 1 : 0)) break;
     }
     if (t.inside) {
-      if (item < heavy)
+      if (item < heavy) {
         a.partial[size_t(item) * nt + tid] = make_float4(cr, cg, cb, T);
-      else
+      } else {
         store_pixel(a, t, cr, cg, cb, T);
+        if (DEPTH && a.depth_write && dl.found) a.depth[o] = dl.first;
+      }
     }
   }
 }
@@ -521,6 +600,7 @@ __global__ __launch_bounds__(kCompMaxThreads) void k_composite_fast(CompositeArg
 // Front-to-back merge of the segments of multi-segment tiles.
 __global__ __launch_bounds__(kCompMaxThreads) void k_merge(CompositeArgs a) {
 #pragma clang fp contract(fast)
+  if (*a.pair_count > a.capacity) return;  // truncated lists: the frame is drawn again
   const uint32_t tile = blockIdx.x;
   const uint32_t len = a.ranges[2 * tile + 1] - a.ranges[2 * tile];
   const uint32_t ns = segments_of(len, a.seg_len);
@@ -540,6 +620,7 @@ __global__ __launch_bounds__(kCompMaxThreads) void k_merge(CompositeArgs a) {
 }
 
 __global__ __launch_bounds__(kCompMaxThreads) void k_composite_exact(CompositeArgs a) {
+  if (*a.pair_count > a.capacity) return;  // truncated lists: the frame is drawn again
   __shared__ Stage s;
   const uint32_t tile = blockIdx.x;
   const Tiling t = setup_tile(a.grid, tile);
@@ -554,6 +635,8 @@ __global__ __launch_bounds__(kCompMaxThreads) void k_composite_exact(CompositeAr
 
   uint32_t frags = 0;
   float dr = 0.0f, dg = 0.0f, db = 0.0f, da = 1.0f;
+  const bool dtest = a.depth != nullptr;
+  float zcur = (dtest && t.inside) ? a.depth[size_t(t.py) * a.grid.width + t.px] : 0.0f;
   if (a.load && t.inside) {
     const size_t o = size_t(t.py) * a.grid.width + t.px;
     if (q8) {
@@ -574,7 +657,9 @@ __global__ __launch_bounds__(kCompMaxThreads) void k_composite_exact(CompositeAr
     const uint32_t cnt = (end - b0) < nt ? (end - b0) : nt;
     __syncthreads();
     const bool valid = tid < cnt;
-    stage_batch(s, t, a.splats, valid, valid ? a.pair_vals[b0 + tid] : 0u, q8, mode);  // farthest first
+    // farthest first
+    stage_batch(s, t, a.splats, dtest ? a.splat_depth : nullptr, valid, valid ? a.pair_vals[b0 + tid] : 0u, q8,
+                mode);
     if (!t.inside) continue;
     const uint32_t len = wave_list_len(s, t, w);
     for (uint32_t j = 0; j < len; ++j) {
@@ -582,6 +667,12 @@ __global__ __launch_bounds__(kCompMaxThreads) void k_composite_exact(CompositeAr
       float4 col;
       const float al = fragment(s, q, t, mode, sd2l, ell2l, col);
       if (al < 0.0f) continue;
+      if (dtest) {
+        // depth test against the attachment as updated by earlier draws
+        const float z = s.z[q];
+        if (!depth_pass(a.depth_cmp, z, zcur)) continue;
+        if (a.depth_write) zcur = z;
+      }
       ++frags;
       const float om = 1.0f - al;
       dr = col.z * al + dr * om;
@@ -602,6 +693,7 @@ __global__ __launch_bounds__(kCompMaxThreads) void k_composite_exact(CompositeAr
       reinterpret_cast<uint32_t*>(a.out)[o] = unorm8(dr) | (unorm8(dg) << 8) | (unorm8(db) << 16) | (unorm8(da) << 24);
     else
       reinterpret_cast<float4*>(a.out)[o] = make_float4(dr, dg, db, da);
+    if (dtest && a.depth_write) a.depth[o] = zcur;
     if (a.frag_count && frags) atomicAdd(a.frag_count, frags);
   }
 }
@@ -651,7 +743,18 @@ void launch_composite(hipStream_t st, const CompositeArgs& a, int num_cus, hipEv
     const uint32_t cap = composite_max_items(tiles, a.capacity, a.seg_len);
     if (grid > cap) grid = cap;
     if (main_begin) (void)hipEventRecord(main_begin, st);
-    hipLaunchKernelGGL(k_composite_fast, dim3(grid), dim3(threads), 0, st, a);
+    // 8x8 tiles (one wave per workgroup) use a workgroup of 64 lanes
+    if (threads == kWave) {
+      if (a.depth)
+        hipLaunchKernelGGL((k_composite_fast<true, kWave>), dim3(grid), dim3(threads), 0, st, a);
+      else
+        hipLaunchKernelGGL((k_composite_fast<false, kWave>), dim3(grid), dim3(threads), 0, st, a);
+    } else {
+      if (a.depth)
+        hipLaunchKernelGGL((k_composite_fast<true, kCompMaxThreads>), dim3(grid), dim3(threads), 0, st, a);
+      else
+        hipLaunchKernelGGL((k_composite_fast<false, kCompMaxThreads>), dim3(grid), dim3(threads), 0, st, a);
+    }
     if (main_end) (void)hipEventRecord(main_end, st);
     hipLaunchKernelGGL(k_merge, dim3(tiles), dim3(threads), 0, st, a);
   } else {

@@ -88,12 +88,32 @@ __device__ __forceinline__ void cov3d_of(const CovWords& w, float c[6]) {
   }
 }
 
+// Inclusive tile rectangle a splat can touch, packed (tx0 | ty0 << 16,
+// tx1 | ty1 << 16); (0xFFFFFFFF, 0) when it touches none. A pixel x is
+// covered only if |x + 0.5 - cx| <= hx, so [cx - hx - 1, cx + hx + 1] is a
+// conservative pixel range; hxy holds the f16 half extents rounded up (0:
+// draws no pixel).
+__device__ __forceinline__ uint2 tile_rect(float cx, float cy, uint32_t hxy, const TileGrid& g) {
+  const uint2 none = make_uint2(0xFFFFFFFFu, 0u);
+  if (hxy == 0u) return none;
+  const float hx = __half2float(__ushort_as_half((unsigned short)(hxy & 0xFFFFu)));
+  const float hy = __half2float(__ushort_as_half((unsigned short)(hxy >> 16)));
+  const float fx0 = floorf(cx - hx - 1.0f), fx1 = floorf(cx + hx + 1.0f);
+  const float fy0 = floorf(cy - hy - 1.0f), fy1 = floorf(cy + hy + 1.0f);
+  const float wmax = float(g.width - 1), hmax = float(g.height - 1);
+  if (!(fx1 >= 0.0f) || !(fx0 <= wmax) || !(fy1 >= 0.0f) || !(fy0 <= hmax)) return none;
+  const uint32_t x0 = uint32_t(fmaxf(fx0, 0.0f)), x1 = uint32_t(fminf(fx1, wmax));
+  const uint32_t y0 = uint32_t(fmaxf(fy0, 0.0f)), y1 = uint32_t(fminf(fy1, hmax));
+  return make_uint2((x0 / g.tile_w) | ((y0 / g.tile_h) << 16), (x1 / g.tile_w) | ((y1 / g.tile_h) << 16));
+}
+
 // SH colour of one instance (utils.wesl:82-136) for the splat record:
-// the 15 basis values of the view direction are formed once, then the
-// coefficients are streamed 16 bytes at a time into the three channel sums,
-// so no more than one chunk of coefficients is live in registers. The sums
-// run in coefficient order, not the shader's association (the record is
-// compared within a tolerance, not bit for bit).
+// every 16-byte chunk of coefficients is requested first (all in flight at
+// once), the 15 basis values of the view direction are formed meanwhile,
+// then each chunk is folded into the three channel sums as packed words,
+// without unpacking all 45 coefficients at once. The sums run in
+// coefficient order, not the shader's association (the record is compared
+// within a tolerance, not bit for bit).
 template <int POD>
 __device__ __forceinline__ void sh_color(const GaussianStreams& g, uint32_t i, float x, float y, float z, int sh_deg,
                                          float rgb[3]) {
@@ -115,29 +135,37 @@ __device__ __forceinline__ void sh_color(const GaussianStreams& g, uint32_t i, f
   };
   if constexpr (POD == kPodSingle) {
     const float4* p = g.sh32 + size_t(i) * 12;
+    const int nq = (3 * kmax + 3) / 4;
+    float4 v[12];
+#pragma unroll
+    for (int q = 0; q < 12; ++q)
+      if (q < nq) v[q] = p[q];
 #pragma unroll
     for (int q = 0; q < 12; ++q) {
-      if (4 * q >= 3 * kmax) break;
-      const float4 v = p[q];
-      add(4 * q + 0, v.x);
-      add(4 * q + 1, v.y);
-      add(4 * q + 2, v.z);
-      add(4 * q + 3, v.w);
+      if (q >= nq) break;
+      add(4 * q + 0, v[q].x);
+      add(4 * q + 1, v[q].y);
+      add(4 * q + 2, v[q].z);
+      add(4 * q + 3, v[q].w);
     }
   } else {
     const uint4* p = g.sh16 + size_t(i) * 6;
+    const int nq = (3 * kmax + 7) / 8;
+    uint4 v[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+      if (q < nq) v[q] = p[q];
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
-      if (8 * q >= 3 * kmax) break;
-      const uint4 v = p[q];
-      add(8 * q + 0, half_lo(v.x));
-      add(8 * q + 1, half_hi(v.x));
-      add(8 * q + 2, half_lo(v.y));
-      add(8 * q + 3, half_hi(v.y));
-      add(8 * q + 4, half_lo(v.z));
-      add(8 * q + 5, half_hi(v.z));
-      add(8 * q + 6, half_lo(v.w));
-      add(8 * q + 7, half_hi(v.w));
+      if (q >= nq) break;
+      add(8 * q + 0, half_lo(v[q].x));
+      add(8 * q + 1, half_hi(v[q].x));
+      add(8 * q + 2, half_lo(v[q].y));
+      add(8 * q + 3, half_hi(v[q].y));
+      add(8 * q + 4, half_lo(v[q].z));
+      add(8 * q + 5, half_hi(v[q].z));
+      add(8 * q + 6, half_lo(v[q].w));
+      add(8 * q + 7, half_hi(v[q].w));
     }
   }
 }
@@ -155,7 +183,8 @@ __device__ __forceinline__ void sh_color(const GaussianStreams& g, uint32_t i, f
 template <int POD>
 __device__ __forceinline__ void compute_splat(const FrameConsts& fc, const GaussianStreams& g, uint32_t i,
                                               const float pos[3], uint32_t color_bits, const float world[4],
-                                              const float c[6], Splat2D* out) {
+                                              const float c[6], Splat2D* out, float* depth_out,
+                                              uint2* rect_out = nullptr, const TileGrid* rect_grid = nullptr) {
 #pragma clang fp contract(fast)
   float view[4], proj[4];
   m4v(fc.view, world, view);
@@ -177,6 +206,9 @@ __device__ __forceinline__ void compute_splat(const FrameConsts& fc, const Gauss
   float alpha = float(color_bits >> 24) * (1.0f / 255.0f);
 
   const float iw = __builtin_amdgcn_rcpf(proj[3]);
+  // fragment depth: every corner of the quad keeps the centre's clip z and w
+  // (render.wesl:119-125), so the rasterised depth is z / w (IEEE divide)
+  if (depth_out) *depth_out = proj[2] / proj[3];
   Splat2D s;
   s.cx = (proj[0] * iw + 1.0f) * 0.5f * fc.size[0];
   s.cy = (1.0f - proj[1] * iw) * 0.5f * fc.size[1];
@@ -259,6 +291,7 @@ __device__ __forceinline__ void compute_splat(const FrameConsts& fc, const Gauss
   s.ba = pack_half2(rgb[2], alpha);
   s.hxy = drawable ? pack_half2_up(hx, hy) : 0u;
   *out = s;
+  if (rect_out) *rect_out = tile_rect(s.cx, s.cy, s.hxy, *rect_grid);
 }
 
 }  // namespace gs3d

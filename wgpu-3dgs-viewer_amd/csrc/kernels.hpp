@@ -26,17 +26,36 @@ struct PreprocessArgs {
   uint32_t* ticket;
   uint32_t num_tiles;
   uint32_t* err;
+  // preprocess.wesl:69-78: Gaussian i is skipped when bit i of `selection`
+  // equals `invert_selection` (no selection: null)
+  const uint32_t* selection;
+  uint32_t invert_selection;
+  // per Gaussian: the fragment depth of its splat (clip z / w of the centre,
+  // render.wesl:108-125), for depth-tested draws (null: not written)
+  float* splat_depth;
+  // per Gaussian: the tile rectangle of its splat on rect_grid (tile_rect),
+  // for the binning of the frame (null: not written)
+  uint2* rects;
+  TileGrid rect_grid;
+  // the depth sort's first-pass histogram (digit 0 of the keys it will sort:
+  // the visible keys and the 2.0 pads up to min(dispatch.x * 3840,
+  // sort_max)), zeroed by k_frame_pre (null: the sort counts for itself)
+  uint32_t* sort_hist;
+  uint32_t sort_max;
 };
 
 uint32_t preprocess_num_tiles(uint32_t n);
 void launch_frame_pre(hipStream_t st, const CameraPod* cam, const ModelTransformPod* mt,
                       const GaussianTransformPod* gt, FrameConsts* fc, uint32_t* indirect_args,
-                      uint32_t* dispatch_args, uint32_t* status, uint32_t num_tiles, uint32_t* ticket);
+                      uint32_t* dispatch_args, uint32_t* status, uint32_t num_tiles, uint32_t* ticket,
+                      uint32_t* sort_hist = nullptr);
 void launch_frame_consts(hipStream_t st, const CameraPod* cam, const ModelTransformPod* mt,
                          const GaussianTransformPod* gt, FrameConsts* fc);
 void launch_preprocess(hipStream_t st, int pod, const PreprocessArgs& a);
+// splat_depth: as PreprocessArgs::splat_depth (or null)
 void launch_vertex(hipStream_t st, int pod, const FrameConsts* fc, const GaussianStreams& g,
-                   const uint32_t* indirect_args, const uint32_t* indices, Splat2D* splats, uint32_t max_v);
+                   const uint32_t* indirect_args, const uint32_t* indices, Splat2D* splats, uint32_t max_v,
+                   float* splat_depth);
 
 // ---- radix_sort.hip --------------------------------------------------------
 
@@ -62,22 +81,21 @@ size_t sort_lookback_words(uint32_t count_max, int passes);
 // Ping-pongs between (k0, v0) and (k1, v1); returns 0 when the result is in
 // (k0, v0), 1 when it is in (k1, v1). Keys [pad_begin, pad_end) of
 // pad_keys (when non-null) are set to 0xFFFFFFFF first.
+// Every pass counts the next pass's digits as it goes; the first pass's
+// histogram comes from an upsweep over the keys, or, with hist_ready, from
+// whoever wrote the keys: then sc.hist[0..255] already holds the digit-0
+// counts of the n keys and sc.hist[256..] is zero.
 int launch_radix_sort(hipStream_t st, int num_cus, SortCount cnt, uint32_t* k0, uint32_t* v0, uint32_t* k1,
                       uint32_t* v1, int passes, const SortScratch& sc, uint32_t* pad_keys = nullptr,
-                      uint32_t pad_begin = 0, uint32_t pad_end = 0);
+                      uint32_t pad_begin = 0, uint32_t pad_end = 0, bool hist_ready = false);
 
 // ---- tiles.hip ---------------------------------------------------------------
-
-struct TileGrid {
-  uint32_t width, height;   // target pixels
-  uint32_t tile_w, tile_h;  // tile pixels
-  uint32_t tiles_x, tiles_y;
-};
 
 struct BinArgs {
   const uint32_t* indirect_args;  // instance count at [1]
   const uint32_t* indices;        // sorted Gaussian indices
   const Splat2D* splats;
+  const uint2* rects;  // per-Gaussian tile rectangles on `grid` (null: from the splats)
   TileGrid grid;
   uint32_t* pair_keys;
   uint32_t* pair_vals;
@@ -87,6 +105,9 @@ struct BinArgs {
   uint32_t* ticket;
   uint32_t max_tiles;
   uint32_t* err;
+  // the tile sort's first-pass histogram (digit 0 of the stored pairs' tile
+  // ids), zeroed by k_bin_reset (null: the sort counts for itself)
+  uint32_t* sort_hist;
 };
 
 struct CompositeArgs {
@@ -106,19 +127,29 @@ struct CompositeArgs {
   uint32_t* item_tile; // [composite_max_items]
   uint32_t* item_seg;  // [composite_max_items]
   uint32_t* seg_base;  // [tiles] first item of a multi-segment tile
+  const uint32_t* pair_count;  // the frame's pair total: above capacity, nothing is drawn
+  // depth test (Renderer::new's depth_stencil, src/renderer.rs:120-137):
+  // per-Gaussian fragment depth, the W x H f32 depth attachment (null: no
+  // test), the CompareFn and whether passing fragments write their depth
+  const float* splat_depth;
+  float* depth;
+  int depth_cmp;
+  int depth_write;
   uint32_t* plan;      // {items, multi-segment items, work ticket}
   uint32_t* plan_counts;  // [composite_plan_blocks][2] per-block item counts
   float4* partial;     // [composite_max_heavy_items][tile pixels]
 };
 
 uint32_t bin_max_tiles(uint32_t max_instances);
+// sort_hist: hist_words words zeroed as well (the tile sort's histograms)
 void launch_bin_reset(hipStream_t st, uint32_t* ranges, uint32_t num_tiles, uint32_t* status, uint32_t max_bin_tiles,
-                      uint32_t* ticket, uint32_t* pair_count);
+                      uint32_t* ticket, uint32_t* pair_count, uint32_t* sort_hist = nullptr, uint32_t hist_words = 0);
 void launch_bin(hipStream_t st, const BinArgs& a);
-// host_total: device alias of mapped host memory that receives the pair
-// total (or null)
+// pair_max: device word keeping the largest pair total since the host reset
+// it; host_total: device alias of mapped host memory that receives that
+// maximum (or null)
 void launch_ranges(hipStream_t st, const uint32_t* pair_count, uint32_t capacity, const uint32_t* keys,
-                   uint32_t* ranges, int num_cus, uint32_t* host_total);
+                   uint32_t* ranges, int num_cus, uint32_t* pair_max, uint32_t* host_total);
 uint32_t composite_max_items(uint32_t num_tiles, uint32_t capacity, uint32_t seg_len);
 uint32_t composite_max_heavy_items(uint32_t capacity, uint32_t seg_len);
 uint32_t composite_plan_blocks(uint32_t num_tiles);

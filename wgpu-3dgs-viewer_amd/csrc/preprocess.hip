@@ -36,7 +36,10 @@ __global__ __launch_bounds__(256) void k_frame_pre(const CameraPod* __restrict__
                                                    const GaussianTransformPod* __restrict__ gt,
                                                    FrameConsts* __restrict__ fc, uint32_t* indirect_args,
                                                    uint32_t* dispatch_args, uint32_t* status,
-                                                   uint32_t num_tiles, uint32_t* ticket) {
+                                                   uint32_t num_tiles, uint32_t* ticket, uint32_t* sort_hist) {
+  // the depth sort's histograms, which this frame's projection fills
+  if (sort_hist)
+    for (uint32_t j = threadIdx.x; j < 4u * 256u; j += blockDim.x) sort_hist[j] = 0u;
   if (threadIdx.x == 0) {
     FrameConsts c;
     derive_frame_consts(*cam, *mt, *gt, c);
@@ -71,6 +74,8 @@ __global__ __launch_bounds__(kPreThreads) __attribute__((amdgpu_waves_per_eu(GS3
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_base;
   __shared__ uint32_t s_total;
+  __shared__ uint32_t s_hist[256];  // digit 0 of this tile's depth keys
+  if (a.sort_hist) s_hist[threadIdx.x] = 0u;
 
   const FrameConsts& fc = *a.fc;
   const uint32_t tile = take_ticket(a.ticket, &s_tile);
@@ -87,10 +92,17 @@ __global__ __launch_bounds__(kPreThreads) __attribute__((amdgpu_waves_per_eu(GS3
   // off-screen centres or for the splat of visible ones)
   float4 pcs[kPreItems];
   CovWords cws[kPreItems];
+  bool act[kPreItems];
 #pragma unroll
   for (int k = 0; k < kPreItems; ++k) {
     const uint32_t i = base + uint32_t(k) * kPreThreads + tid;
-    if (i < a.g.n) {
+    act[k] = i < a.g.n;
+    if (act[k] && a.selection) {
+      // selection early-out (preprocess.wesl:69-78)
+      const bool bit = ((a.selection[i >> 5] >> (i & 31u)) & 1u) != 0u;
+      act[k] = bit != (a.invert_selection != 0u);
+    }
+    if (act[k]) {
       pcs[k] = a.g.pc[i];
       cws[k] = load_cov_words<POD>(a.g, i);
     }
@@ -101,7 +113,7 @@ __global__ __launch_bounds__(kPreThreads) __attribute__((amdgpu_waves_per_eu(GS3
     const uint32_t i = base + uint32_t(k) * kPreThreads + tid;
     vis[k] = false;
     key[k] = 0.0f;
-    if (i < a.g.n) {
+    if (act[k]) {
       const float4 pc = pcs[k];
       const float pos[3] = {pc.x, pc.y, pc.z};
       const float p4[4] = {pc.x, pc.y, pc.z, 1.0f};
@@ -131,7 +143,9 @@ __global__ __launch_bounds__(kPreThreads) __attribute__((amdgpu_waves_per_eu(GS3
       }
       if (visible) {
         key[k] = 1.0f - nz;  // preprocess.wesl:105
-        compute_splat<POD>(fc, a.g, i, pos, __float_as_uint(pc.w), world, c, a.splats + i);
+        compute_splat<POD>(fc, a.g, i, pos, __float_as_uint(pc.w), world, c, a.splats + i,
+                           a.splat_depth ? a.splat_depth + i : nullptr, a.rects ? a.rects + i : nullptr,
+                           &a.rect_grid);
       }
       vis[k] = visible;
     }
@@ -178,6 +192,19 @@ __global__ __launch_bounds__(kPreThreads) __attribute__((amdgpu_waves_per_eu(GS3
       const uint32_t p = out_base + s_cnt[k][w] + mask_rank(masks[k]);
       a.indices[p] = i;   // preprocess.wesl:102
       a.depth[p] = key[k];  // preprocess.wesl:105
+      if (a.sort_hist) atomicAdd(&s_hist[__float_as_uint(key[k]) & 0xFFu], 1u);
+    }
+  }
+  if (a.sort_hist) {
+    __syncthreads();
+    if (s_hist[tid]) atomicAdd(&a.sort_hist[tid], s_hist[tid]);
+    if (tile == a.num_tiles - 1 && tid == 0) {
+      // the 2.0 pads in the sorted range (bits 0x40000000: digit 0 is 0)
+      const uint32_t v = s_total;
+      const uint32_t blocks = (v + kHistoBlockKvs - 1u) / kHistoBlockKvs;
+      const uint64_t n64 = uint64_t(blocks) * kHistoBlockKvs;
+      const uint32_t n = n64 < uint64_t(a.sort_max) ? uint32_t(n64) : a.sort_max;
+      if (n > v) atomicAdd(&a.sort_hist[0], n - v);
     }
   }
 
@@ -195,7 +222,8 @@ __global__ __launch_bounds__(kPreThreads) __attribute__((amdgpu_waves_per_eu(GS3
 template <int POD>
 __global__ __launch_bounds__(256) void k_vertex(const FrameConsts* __restrict__ fcp, GaussianStreams g,
                                                 const uint32_t* __restrict__ indirect_args,
-                                                const uint32_t* __restrict__ indices, Splat2D* splats) {
+                                                const uint32_t* __restrict__ indices, Splat2D* splats,
+                                                float* splat_depth) {
   const FrameConsts& fc = *fcp;
   const uint32_t v = indirect_args[1];
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < v; s += gridDim.x * blockDim.x) {
@@ -207,7 +235,8 @@ __global__ __launch_bounds__(256) void k_vertex(const FrameConsts* __restrict__ 
     float world[4], c[6];
     m4v(fc.model, p4, world);
     cov3d_of<POD>(load_cov_words<POD>(g, i), c);
-    compute_splat<POD>(fc, g, i, pos, __float_as_uint(pc.w), world, c, splats + i);
+    compute_splat<POD>(fc, g, i, pos, __float_as_uint(pc.w), world, c, splats + i,
+                       splat_depth ? splat_depth + i : nullptr);
   }
 }
 
@@ -215,9 +244,10 @@ uint32_t preprocess_num_tiles(uint32_t n) { return (n + kPreTile - 1) / kPreTile
 
 void launch_frame_pre(hipStream_t st, const CameraPod* cam, const ModelTransformPod* mt,
                       const GaussianTransformPod* gt, FrameConsts* fc, uint32_t* indirect_args,
-                      uint32_t* dispatch_args, uint32_t* status, uint32_t num_tiles, uint32_t* ticket) {
+                      uint32_t* dispatch_args, uint32_t* status, uint32_t num_tiles, uint32_t* ticket,
+                      uint32_t* sort_hist) {
   hipLaunchKernelGGL(k_frame_pre, dim3(1), dim3(256), 0, st, cam, mt, gt, fc, indirect_args, dispatch_args,
-                     status, num_tiles, ticket);
+                     status, num_tiles, ticket, sort_hist);
 }
 
 void launch_frame_consts(hipStream_t st, const CameraPod* cam, const ModelTransformPod* mt,
@@ -234,14 +264,17 @@ void launch_preprocess(hipStream_t st, int pod, const PreprocessArgs& a) {
 }
 
 void launch_vertex(hipStream_t st, int pod, const FrameConsts* fc, const GaussianStreams& g,
-                   const uint32_t* indirect_args, const uint32_t* indices, Splat2D* splats, uint32_t max_v) {
+                   const uint32_t* indirect_args, const uint32_t* indices, Splat2D* splats, uint32_t max_v,
+                   float* splat_depth) {
   if (max_v == 0) return;
   uint32_t blocks = (max_v + 255) / 256;
   if (blocks > 4096) blocks = 4096;
   if (pod == kPodSingle)
-    hipLaunchKernelGGL(k_vertex<kPodSingle>, dim3(blocks), dim3(256), 0, st, fc, g, indirect_args, indices, splats);
+    hipLaunchKernelGGL(k_vertex<kPodSingle>, dim3(blocks), dim3(256), 0, st, fc, g, indirect_args, indices, splats,
+                       splat_depth);
   else
-    hipLaunchKernelGGL(k_vertex<kPodHalf>, dim3(blocks), dim3(256), 0, st, fc, g, indirect_args, indices, splats);
+    hipLaunchKernelGGL(k_vertex<kPodHalf>, dim3(blocks), dim3(256), 0, st, fc, g, indirect_args, indices, splats,
+                       splat_depth);
 }
 
 }  // namespace gs3d

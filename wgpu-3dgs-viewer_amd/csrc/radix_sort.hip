@@ -3,9 +3,9 @@
 // (Merrill & Garland 2016). Replaces the reference's vendored wgpu_sort
 // (src/radix_sorter.rs:98-940, src/shader/radix_sort.wgsl): same result (a
 // stable ascending sort), different machinery:
-//   * one upsweep reads the keys once and builds every pass's 256-bin
-//     histogram (wave-aggregated LDS atomics, then one global atomic per bin
-//     per workgroup);
+//   * the first pass's 256-bin histogram comes from one upsweep over the
+//     keys (per-wave LDS atomics) or from the kernel that wrote the keys;
+//     every pass counts the next pass's digits of the keys it holds;
 //   * each pass is a persistent kernel: a workgroup takes tiles of
 //     256 x GS3D_SORT_ITEMS keys by
 //     ticket, ranks them with wave64 ballot matching (8 ballots per key give
@@ -55,12 +55,13 @@ __device__ __forceinline__ uint32_t block_inclusive_scan_256(uint32_t x, uint32_
 }
 
 __global__ __launch_bounds__(256) void k_rs_zero(SortCount cnt, SortScratch sc, int passes, uint32_t* pad_keys,
-                                                 uint32_t pad_begin, uint32_t pad_end) {
+                                                 uint32_t pad_begin, uint32_t pad_end, int zero_hist) {
   const uint32_t n = sort_n(cnt);
   const uint32_t tiles = (n + kRsTile - 1) / kRsTile;
   const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t j = gid; j < uint32_t(passes) * 256u; j += stride) sc.hist[j] = 0u;
+  if (zero_hist)
+    for (uint32_t j = gid; j < uint32_t(passes) * 256u; j += stride) sc.hist[j] = 0u;
   if (gid < uint32_t(passes)) sc.tickets[gid] = 0u;
   const uint64_t per_pass = uint64_t(tiles) * 256u;
   for (uint64_t j = gid; j < per_pass * uint64_t(passes); j += stride) {
@@ -73,36 +74,17 @@ __global__ __launch_bounds__(256) void k_rs_zero(SortCount cnt, SortScratch sc, 
     for (uint32_t j = pad_begin + gid; j < pad_end; j += stride) pad_keys[j] = 0xFFFFFFFFu;
 }
 
-// Lanes of the wave whose 8-bit digit equals this lane's (ballot matching,
-// one ballot per digit bit). Inactive lanes take no part.
-__device__ __forceinline__ uint64_t match_digit(uint32_t d) {
-  uint64_t m = __ballot(1);
-#pragma unroll
-  for (int b = 0; b < 8; ++b) {
-    const bool bit = (d >> b) & 1u;
-    const uint64_t bb = __ballot(bit);
-    m &= bit ? bb : ~bb;
-  }
-  return m;
-}
-
-// Histograms of every pass's digit from one read of the keys. Each wave
-// counts into its own LDS histograms with one atomic per distinct digit of a
-// 64-key step (the lanes sharing a digit add together: depth keys share their
-// high digits, and per-key atomics on one bin would serialize).
+// Histogram of the first pass's digit: one read of the keys, each wave
+// counting into its own LDS histogram (no atomic contention between waves).
 __global__ __launch_bounds__(256) void k_rs_upsweep(SortCount cnt, const uint32_t* __restrict__ keys,
-                                                    uint32_t* __restrict__ hist, int passes) {
-  __shared__ uint32_t s_h[kRsWaves][4][256];
+                                                    uint32_t* __restrict__ hist) {
+  __shared__ uint32_t s_h[kRsWaves][256];
   const uint32_t n = sort_n(cnt);
   const uint32_t w = threadIdx.x / kWave;
-  for (int q = 0; q < kRsWaves; ++q)
-    for (int p = 0; p < 4; ++p) s_h[q][p][threadIdx.x] = 0u;
+  for (int q = 0; q < kRsWaves; ++q) s_h[q][threadIdx.x] = 0u;
   __syncthreads();
   const uint32_t stride = gridDim.x * blockDim.x * 4u;
-  // the loop bound is uniform per wave (it depends on the wave's first key)
-  const uint32_t wave0 = (blockIdx.x * blockDim.x + w * kWave) * 4u;
-  for (uint32_t i0 = wave0; i0 < n; i0 += stride) {
-    const uint32_t i = i0 + (threadIdx.x % kWave) * 4u;
+  for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) * 4u; i < n; i += stride) {
     uint32_t k4[4] = {0u, 0u, 0u, 0u};
     if (i + 3u < n) {
       const uint4 v = *reinterpret_cast<const uint4*>(keys + i);
@@ -113,31 +95,23 @@ __global__ __launch_bounds__(256) void k_rs_upsweep(SortCount cnt, const uint32_
     } else {
       for (int q = 0; q < 4; ++q) k4[q] = (i + q < n) ? keys[i + q] : 0u;
     }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if (i + q < n) {
-        for (int p = 0; p < passes; ++p) {
-          const uint32_t d = (k4[q] >> (8 * p)) & 0xFFu;
-          const uint64_t m = match_digit(d);
-          if (mask_rank(m) == 0u) atomicAdd(&s_h[w][p][d], uint32_t(__popcll(m)));
-        }
-      }
-    }
+    const int m = (i + 3u < n) ? 4 : int(n - i);
+    for (int q = 0; q < m; ++q) atomicAdd(&s_h[w][k4[q] & 0xFFu], 1u);
   }
   __syncthreads();
-  for (int p = 0; p < passes; ++p) {
-    uint32_t c = 0;
-    for (int q = 0; q < kRsWaves; ++q) c += s_h[q][p][threadIdx.x];
-    if (c) atomicAdd(&hist[p * 256 + threadIdx.x], c);
-  }
+  uint32_t c = 0;
+  for (int q = 0; q < kRsWaves; ++q) c += s_h[q][threadIdx.x];
+  if (c) atomicAdd(&hist[threadIdx.x], c);
 }
 
 __global__ __launch_bounds__(kRsThreads) void k_rs_onesweep(SortCount cnt, const uint32_t* __restrict__ kin,
                                                            const uint32_t* __restrict__ vin,
                                                            uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                            const uint32_t* __restrict__ hist, uint32_t* lookback,
-                                                           uint32_t* ticket, int shift, uint32_t* err) {
+                                                           uint32_t* ticket, int shift, uint32_t* err,
+                                                           uint32_t* hist_next) {
   __shared__ uint32_t s_keys[kRsTile];
+  __shared__ uint32_t s_next[256];  // next pass's digit counts of the tile
   __shared__ uint32_t s_vals[kRsTile];
   __shared__ uint32_t s_wcnt[kRsWaves][256];
   __shared__ uint32_t s_dbase[256];
@@ -165,6 +139,7 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_onesweep(SortCount cnt, const
 
 #pragma unroll
     for (int q = 0; q < kRsWaves; ++q) s_wcnt[q][tid] = 0u;
+    s_next[tid] = 0u;
     __syncthreads();
 
     uint32_t key[kRsItems], val[kRsItems], lrank[kRsItems];
@@ -174,6 +149,14 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_onesweep(SortCount cnt, const
       const bool ok = li < valid;
       key[k] = ok ? kin[base + li] : 0xFFFFFFFFu;
       val[k] = ok ? vin[base + li] : 0u;
+    }
+    if (hist_next) {
+      // the next pass's histogram: the same keys, whatever their order
+#pragma unroll
+      for (int k = 0; k < kRsItems; ++k) {
+        const uint32_t li = w * (kWave * kRsItems) + uint32_t(k) * kWave + lane;
+        if (li < valid) atomicAdd(&s_next[(key[k] >> (shift + 8)) & 0xFFu], 1u);
+      }
     }
 #pragma unroll
     for (int k = 0; k < kRsItems; ++k) {
@@ -240,6 +223,7 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_onesweep(SortCount cnt, const
     }
     s_gbase[d] = s_hscan[d] + excl - dbase;
     __syncthreads();
+    if (hist_next && s_next[tid]) atomicAdd(&hist_next[tid], s_next[tid]);
 
     for (uint32_t j = tid; j < valid; j += kRsThreads) {
       const uint32_t k = s_keys[j];
@@ -262,29 +246,33 @@ size_t sort_lookback_words(uint32_t count_max, int passes) {
 }
 
 static void launch_zero(hipStream_t st, int num_cus, SortCount cnt, const SortScratch& sc, int passes,
-                        uint32_t* pad_keys, uint32_t pad_begin, uint32_t pad_end) {
+                        uint32_t* pad_keys, uint32_t pad_begin, uint32_t pad_end, bool zero_hist) {
   uint32_t blocks = (sc.max_tiles * 256u * uint32_t(passes) + 255u) / 256u;
   const uint32_t cap = uint32_t(num_cus) * 4u;
   if (blocks > cap) blocks = cap;
   if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(k_rs_zero, dim3(blocks), dim3(256), 0, st, cnt, sc, passes, pad_keys, pad_begin, pad_end);
+  hipLaunchKernelGGL(k_rs_zero, dim3(blocks), dim3(256), 0, st, cnt, sc, passes, pad_keys, pad_begin, pad_end,
+                     zero_hist ? 1 : 0);
 }
 
 int launch_radix_sort(hipStream_t st, int num_cus, SortCount cnt, uint32_t* k0, uint32_t* v0, uint32_t* k1,
                       uint32_t* v1, int passes, const SortScratch& sc, uint32_t* pad_keys, uint32_t pad_begin,
-                      uint32_t pad_end) {
+                      uint32_t pad_end, bool hist_ready) {
   if (sc.max_tiles == 0 || passes <= 0) return 0;
-  launch_zero(st, num_cus, cnt, sc, passes, pad_keys, pad_begin, pad_end);
-  uint32_t up_blocks = (sc.max_tiles * kRsTile + 4095u) / 4096u;
-  if (up_blocks > uint32_t(num_cus) * 4u) up_blocks = uint32_t(num_cus) * 4u;
-  if (up_blocks < 1) up_blocks = 1;
-  hipLaunchKernelGGL(k_rs_upsweep, dim3(up_blocks), dim3(256), 0, st, cnt, k0, sc.hist, passes);
+  launch_zero(st, num_cus, cnt, sc, passes, pad_keys, pad_begin, pad_end, !hist_ready);
+  if (!hist_ready) {
+    uint32_t up_blocks = (sc.max_tiles * kRsTile + 4095u) / 4096u;
+    if (up_blocks > uint32_t(num_cus) * 4u) up_blocks = uint32_t(num_cus) * 4u;
+    if (up_blocks < 1) up_blocks = 1;
+    hipLaunchKernelGGL(k_rs_upsweep, dim3(up_blocks), dim3(256), 0, st, cnt, k0, sc.hist);
+  }
   uint32_t grid = sc.max_tiles;
   if (grid > uint32_t(num_cus) * 4u) grid = uint32_t(num_cus) * 4u;
   uint32_t *ks = k0, *vs = v0, *kd = k1, *vd = v1;
   for (int p = 0; p < passes; ++p) {
+    uint32_t* next = (p + 1 < passes) ? sc.hist + (p + 1) * 256 : nullptr;
     hipLaunchKernelGGL(k_rs_onesweep, dim3(grid), dim3(kRsThreads), 0, st, cnt, ks, vs, kd, vd, sc.hist + p * 256,
-                       sc.lookback + size_t(p) * sc.max_tiles * 256u, sc.tickets + p, 8 * p, sc.err);
+                       sc.lookback + size_t(p) * sc.max_tiles * 256u, sc.tickets + p, 8 * p, sc.err, next);
     uint32_t* t = ks;
     ks = kd;
     kd = t;

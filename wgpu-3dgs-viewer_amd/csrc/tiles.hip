@@ -23,36 +23,17 @@ constexpr uint32_t kBinTile = kBinThreads * kBinItems;
 uint32_t bin_max_tiles(uint32_t max_instances) { return (max_instances + kBinTile - 1) / kBinTile; }
 
 __global__ __launch_bounds__(256) void k_bin_reset(uint32_t* ranges, uint32_t range_words, uint32_t* status,
-                                                   uint32_t max_bin_tiles, uint32_t* ticket, uint32_t* pair_count) {
+                                                   uint32_t max_bin_tiles, uint32_t* ticket, uint32_t* pair_count,
+                                                   uint32_t* sort_hist, uint32_t hist_words) {
   const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t j = gid; j < range_words; j += stride) ranges[j] = 0u;
   for (uint32_t j = gid; j < max_bin_tiles; j += stride) status[j] = 0u;
+  for (uint32_t j = gid; j < hist_words; j += stride) sort_hist[j] = 0u;
   if (gid == 0) {
     *ticket = 0u;
     *pair_count = 0u;
   }
-}
-
-// Inclusive tile rectangle a splat can touch; false when it touches none.
-// Pixel x is covered only if |x + 0.5 - cx| <= hx, so [cx - hx - 1,
-// cx + hx + 1] is a conservative pixel range.
-__device__ __forceinline__ bool splat_tiles(const Splat2D& s, const TileGrid& g, uint32_t& tx0, uint32_t& ty0,
-                                            uint32_t& tx1, uint32_t& ty1) {
-  if (s.hxy == 0u) return false;  // draws no pixel
-  const float hx = __half2float(__ushort_as_half((unsigned short)(s.hxy & 0xFFFFu)));
-  const float hy = __half2float(__ushort_as_half((unsigned short)(s.hxy >> 16)));
-  const float fx0 = floorf(s.cx - hx - 1.0f), fx1 = floorf(s.cx + hx + 1.0f);
-  const float fy0 = floorf(s.cy - hy - 1.0f), fy1 = floorf(s.cy + hy + 1.0f);
-  const float wmax = float(g.width - 1), hmax = float(g.height - 1);
-  if (!(fx1 >= 0.0f) || !(fx0 <= wmax) || !(fy1 >= 0.0f) || !(fy0 <= hmax)) return false;
-  const uint32_t x0 = uint32_t(fmaxf(fx0, 0.0f)), x1 = uint32_t(fminf(fx1, wmax));
-  const uint32_t y0 = uint32_t(fmaxf(fy0, 0.0f)), y1 = uint32_t(fminf(fy1, hmax));
-  tx0 = x0 / g.tile_w;
-  tx1 = x1 / g.tile_w;
-  ty0 = y0 / g.tile_h;
-  ty1 = y1 / g.tile_h;
-  return true;
 }
 
 __device__ __forceinline__ uint32_t scan256_inclusive(uint32_t x, uint32_t* s_wsum) {
@@ -76,6 +57,8 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(BinArgs a) {
   __shared__ uint32_t s_ktot[kBinItems];
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_base;
+  __shared__ uint32_t s_hist[256];  // digit 0 of this block's stored tile ids
+  if (a.sort_hist) s_hist[threadIdx.x] = 0u;
 
   const uint32_t t = take_ticket(a.ticket, &s_tile);
   const uint32_t v = a.indirect_args[1];
@@ -95,12 +78,18 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(BinArgs a) {
     if (s < v) {
       const uint32_t i = a.indices[s];
       gidx[k] = i;
-      const Splat2D sp = a.splats[i];
-      uint32_t tx0, ty0, tx1, ty1;
-      if (splat_tiles(sp, a.grid, tx0, ty0, tx1, ty1)) {
+      uint2 r;
+      if (a.rects) {
+        r = a.rects[i];  // 8 bytes per instance written by the vertex work
+      } else {
+        const Splat2D sp = a.splats[i];
+        r = tile_rect(sp.cx, sp.cy, sp.hxy, a.grid);
+      }
+      const uint32_t tx0 = r.x & 0xFFFFu, ty0 = r.x >> 16, tx1 = r.y & 0xFFFFu, ty1 = r.y >> 16;
+      if (tx0 <= tx1 && ty0 <= ty1) {
         cnt[k] = (tx1 - tx0 + 1u) * (ty1 - ty0 + 1u);
-        r0[k] = tx0 | (ty0 << 16);
-        r1[k] = tx1 | (ty1 << 16);
+        r0[k] = r.x;
+        r1[k] = r.y;
       }
     }
   }
@@ -139,18 +128,28 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(BinArgs a) {
     for (uint32_t ty = ty0; ty <= ty1; ++ty)
       for (uint32_t tx = tx0; tx <= tx1; ++tx, ++p)
         if (p < a.capacity) {
-          a.pair_keys[p] = ty * a.grid.tiles_x + tx;
+          const uint32_t key = ty * a.grid.tiles_x + tx;
+          a.pair_keys[p] = key;
           a.pair_vals[p] = gidx[k];
+          if (a.sort_hist) atomicAdd(&s_hist[key & 0xFFu], 1u);
         }
+  }
+  if (a.sort_hist) {
+    __syncthreads();
+    if (s_hist[tid]) atomicAdd(&a.sort_hist[tid], s_hist[tid]);
   }
 }
 
 __global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ pair_count, uint32_t capacity,
                                                 const uint32_t* __restrict__ keys, uint32_t* __restrict__ ranges,
-                                                uint32_t* host_total) {
+                                                uint32_t* pair_max, uint32_t* host_total) {
   const uint32_t total = *pair_count;
-  // the pair total, for the host to check the lists' capacity without a copy
-  if (host_total && blockIdx.x == 0 && threadIdx.x == 0) *host_total = total;
+  // the largest pair total since the host last looked, for it to check the
+  // lists' capacity without a copy
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const uint32_t m = atomicMax(pair_max, total);
+    if (host_total) *host_total = m > total ? m : total;
+  }
   const uint32_t p = total < capacity ? total : capacity;
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < p; j += gridDim.x * blockDim.x) {
     const uint32_t k = keys[j];
@@ -160,14 +159,14 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ pai
 }
 
 void launch_bin_reset(hipStream_t st, uint32_t* ranges, uint32_t num_tiles, uint32_t* status, uint32_t max_bin_tiles,
-                      uint32_t* ticket, uint32_t* pair_count) {
+                      uint32_t* ticket, uint32_t* pair_count, uint32_t* sort_hist, uint32_t hist_words) {
   uint32_t words = num_tiles * 2u;
   uint32_t m = words > max_bin_tiles ? words : max_bin_tiles;
   uint32_t blocks = (m + 255u) / 256u;
   if (blocks > 1024u) blocks = 1024u;
   if (blocks < 1u) blocks = 1u;
   hipLaunchKernelGGL(k_bin_reset, dim3(blocks), dim3(256), 0, st, ranges, words, status, max_bin_tiles, ticket,
-                     pair_count);
+                     pair_count, sort_hist, hist_words);
 }
 
 void launch_bin(hipStream_t st, const BinArgs& a) {
@@ -176,12 +175,13 @@ void launch_bin(hipStream_t st, const BinArgs& a) {
 }
 
 void launch_ranges(hipStream_t st, const uint32_t* pair_count, uint32_t capacity, const uint32_t* keys,
-                   uint32_t* ranges, int num_cus, uint32_t* host_total) {
+                   uint32_t* ranges, int num_cus, uint32_t* pair_max, uint32_t* host_total) {
   uint32_t blocks = (capacity + 255u) / 256u;
   const uint32_t cap = uint32_t(num_cus) * 8u;
   if (blocks > cap) blocks = cap;
   if (blocks < 1u) blocks = 1u;
-  hipLaunchKernelGGL(k_ranges, dim3(blocks), dim3(256), 0, st, pair_count, capacity, keys, ranges, host_total);
+  hipLaunchKernelGGL(k_ranges, dim3(blocks), dim3(256), 0, st, pair_count, capacity, keys, ranges, pair_max,
+                     host_total);
 }
 
 }  // namespace gs3d

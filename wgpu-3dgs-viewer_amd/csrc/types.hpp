@@ -29,6 +29,19 @@ enum DisplayMode : int32_t { kModeSplat = 0, kModeEllipse = 1, kModePoint = 2 };
 enum TextureFormat : int32_t {
   kFormatRgba8Unorm = 0,
   kFormatRgba32Float = 1,
+  kFormatDepth32Float = 2,  // depth attachment: one f32 per pixel
+};
+
+// wgpu::CompareFunction of a depth test (same numbering).
+enum CompareFn : int32_t {
+  kCmpNever = 1,
+  kCmpLess = 2,
+  kCmpEqual = 3,
+  kCmpLessEqual = 4,
+  kCmpGreater = 5,
+  kCmpNotEqual = 6,
+  kCmpGreaterEqual = 7,
+  kCmpAlways = 8,
 };
 
 enum BlendMode : int32_t {
@@ -120,5 +133,16 @@ constexpr uint32_t kValueMask = (1u << 30) - 1u;
 // Error bits the kernels raise in the per-viewer status word.
 constexpr uint32_t kErrLookbackSpin = 1u;
 constexpr uint32_t kErrTileOverflow = 2u;
+
+// Screen tiles of a render target.
+struct TileGrid {
+  uint32_t width, height;   // target pixels
+  uint32_t tile_w, tile_h;  // tile pixels
+  uint32_t tiles_x, tiles_y;
+};
+
+inline bool same_grid(const TileGrid& a, const TileGrid& b) {
+  return a.width == b.width && a.height == b.height && a.tile_w == b.tile_w && a.tile_h == b.tile_h;
+}
 
 }  // namespace gs3d

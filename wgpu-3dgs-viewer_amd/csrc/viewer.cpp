@@ -79,13 +79,15 @@ void Device::poll() {
       throw Error(GS3D_ERROR_KERNEL, "a kernel exceeded its look-back wait limit");
     }
     if ((e & kErrTileOverflow) && attempt < 4) {
+      // an overflowed frame drew nothing; later draws of a shared pass
+      // (LoadOp::Load) blended over what was there, so every pending draw
+      // is recorded again, in its order
       bool any = false;
-      for (Renderer* r : pending_)
-        if (r->grow_if_overflowed()) {
-          r->rerender();
-          any = true;
-        }
-      if (any) continue;
+      for (Renderer* r : pending_) any = r->grow_if_overflowed() || any;
+      if (any) {
+        for (Renderer* r : pending_) r->rerender();
+        continue;
+      }
     }
     break;
   }
@@ -376,14 +378,30 @@ uint32_t depth_buffer_len(uint32_t n) { return ((n + kHistoBlockKvs - 1) / kHist
 GaussiansDepthBuffer::GaussiansDepthBuffer(Device& dev, uint32_t n)
     : len_(depth_buffer_len(n)), buf_(dev, size_t(depth_buffer_len(n)) * 4) {}
 
-SplatsBuffer::SplatsBuffer(Device& dev, uint32_t n) : buf_(dev, size_t(n) * sizeof(Splat2D)) {}
+SplatsBuffer::SplatsBuffer(Device& dev, uint32_t n) : dev_(dev), n_(n), buf_(dev, size_t(n) * sizeof(Splat2D)) {}
+
+void SplatsBuffer::enable_depth() {
+  if (depth_.size() == 0 && n_ > 0) depth_ = DeviceBuffer(dev_, size_t(n_) * 4);
+}
+
+void SplatsBuffer::set_rect_grid(const TileGrid& g) {
+  if (rects_.size() == 0 && n_ > 0) rects_ = DeviceBuffer(dev_, size_t(n_) * 8);
+  rect_grid_ = g;
+}
 
 Texture::Texture(Device& dev, uint32_t w, uint32_t h, TextureFormat fmt)
     : device_(&dev), w_(w), h_(h), fmt_(fmt) {
   if (w == 0 || h == 0 || w > 16384 || h > 16384) throw Error(GS3D_ERROR_INVALID_ARGUMENT, "bad texture size");
-  if (fmt != kFormatRgba8Unorm && fmt != kFormatRgba32Float)
+  if (fmt != kFormatRgba8Unorm && fmt != kFormatRgba32Float && fmt != kFormatDepth32Float)
     throw Error(GS3D_ERROR_INVALID_ARGUMENT, "unknown texture format");
-  buf_ = DeviceBuffer(dev, size_t(w) * h * (fmt == kFormatRgba8Unorm ? 4 : 16));
+  buf_ = DeviceBuffer(dev, size_t(w) * h * (fmt == kFormatRgba32Float ? 16 : 4));
+}
+
+void Texture::write(const void* src, size_t bytes) {
+  if (bytes != buf_.size()) throw Error(GS3D_ERROR_INVALID_ARGUMENT, "texture write size mismatch");
+  // draws in flight may still use the texture
+  device_->wait();
+  hip_check(hipMemcpy(buf_.get(), src, bytes, hipMemcpyHostToDevice), "hipMemcpy(texture)");
 }
 
 void Texture::read(void* dst, size_t bytes) const {
@@ -460,7 +478,8 @@ void Preprocessor::preprocess(uint32_t gaussian_count) {
   const uint32_t n = std::min(gaussian_count, gaussians_.len());
   const uint32_t tiles = preprocess_num_tiles(n);
   launch_frame_pre(st_, camera_.dev(), model_.dev(), gt_.dev(), consts_.as<FrameConsts>(),
-                   indirect_args_.dev(), rs_args_.dev(), status_.as<uint32_t>(), tiles, ticket_.as<uint32_t>());
+                   indirect_args_.dev(), rs_args_.dev(), status_.as<uint32_t>(), tiles, ticket_.as<uint32_t>(),
+                   sort_hist_);
   PreprocessArgs a{};
   a.fc = consts_.as<FrameConsts>();
   a.g = gaussians_.streams();
@@ -475,6 +494,13 @@ void Preprocessor::preprocess(uint32_t gaussian_count) {
   a.ticket = ticket_.as<uint32_t>();
   a.num_tiles = tiles;
   a.err = dev_.err();
+  a.selection = selection_;
+  a.splat_depth = splat_depth_;
+  a.rects = splats_.rects_on() ? splats_.rects() : nullptr;
+  a.rect_grid = splats_.rect_grid();
+  a.sort_hist = sort_hist_;
+  a.sort_max = sort_max_;
+  a.invert_selection = invert_selection_ ? 1u : 0u;
   launch_preprocess(st_, gaussians_.pod(), a);
   hip_check(hipGetLastError(), "launch preprocess");
 }
@@ -508,19 +534,19 @@ RadixSorter::RadixSorter(Device& dev, GaussiansDepthBuffer& depth, IndirectIndic
       vals_b_(dev, size_t(indices.len()) * 4),
       scratch_(dev, indices.len(), 4) {}
 
-void RadixSorter::sort(const RadixSortIndirectArgsBuffer& rs_args) {
+void RadixSorter::sort(const RadixSortIndirectArgsBuffer& rs_args, bool hist_ready) {
   // the reference sorts dispatch.x * 3840 entries, payload clamped to N
   // (src/radix_sorter.rs:658-669, 741)
-  sort_count(rs_args.dev(), kHistoBlockKvs);
+  sort_count(rs_args.dev(), kHistoBlockKvs, hist_ready);
 }
 
-void RadixSorter::sort_count(const uint32_t* count_dev, uint32_t count_mul) {
+void RadixSorter::sort_count(const uint32_t* count_dev, uint32_t count_mul, bool hist_ready) {
   const uint32_t n = indices_.len();
   if (n == 0) return;
   SortCount c{count_dev, count_mul, n};
   uint32_t* keys = reinterpret_cast<uint32_t*>(depth_.dev());
   launch_radix_sort(st_, dev_.num_cus(), c, keys, indices_.dev(), keys_b_.as<uint32_t>(),
-                    vals_b_.as<uint32_t>(), 4, scratch_.get(), keys, n, depth_.len());
+                    vals_b_.as<uint32_t>(), 4, scratch_.get(), keys, n, depth_.len(), hist_ready);
   hip_check(hipGetLastError(), "launch sort");
 }
 
@@ -584,8 +610,12 @@ void Renderer::ensure_capacity(uint32_t pairs) {
 }
 
 bool Renderer::grow_if_overflowed() {
-  // the frame has completed: k_ranges wrote its pair total to mapped memory
+  // the frames have completed: k_ranges left the largest pair total since
+  // the last check in mapped memory (and in word 1 of pair_count_)
   const uint32_t total = last_pair_total();
+  *static_cast<volatile uint32_t*>(pair_total_host_) = 0u;
+  const uint32_t zero = 0u;
+  hip_check(hipMemcpy(pair_count_.as<uint32_t>() + 1, &zero, 4, hipMemcpyHostToDevice), "reset pair maximum");
   if (total <= capacity_) return false;
   const uint64_t want = std::min<uint64_t>(uint64_t(total) + total / 4 + 1024, 0x3FFFFFFFu);
   ensure_capacity(uint32_t(want));
@@ -615,7 +645,27 @@ uint64_t Renderer::count_fragments(Texture& target, const IndirectArgsBuffer& in
 }
 
 void Renderer::rerender() {
-  if (last_target_ && last_args_) render(*last_target_, *last_args_, last_standalone_, last_clear_);
+  if (last_target_ && last_args_) render(*last_target_, *last_args_, last_standalone_, last_clear_, last_depth_);
+}
+
+TileGrid Renderer::grid_for(const Texture& target) const {
+  TileGrid g{};
+  g.width = target.width();
+  g.height = target.height();
+  g.tile_w = tile_w_;
+  g.tile_h = tile_h_;
+  g.tiles_x = (g.width + tile_w_ - 1) / tile_w_;
+  g.tiles_y = (g.height + tile_h_ - 1) / tile_h_;
+  return g;
+}
+
+void Renderer::set_depth_stencil(bool enabled, CompareFn cmp, bool write) {
+  if (enabled && (int(cmp) < kCmpNever || int(cmp) > kCmpAlways))
+    throw Error(GS3D_ERROR_INVALID_ARGUMENT, "unknown depth compare function");
+  depth_enabled_ = enabled;
+  depth_cmp_ = cmp;
+  depth_write_ = write;
+  if (enabled) splats_.enable_depth();
 }
 
 static uint32_t ceil_log2(uint32_t x) {
@@ -624,9 +674,18 @@ static uint32_t ceil_log2(uint32_t x) {
   return b;
 }
 
-void Renderer::render(Texture& target, const IndirectArgsBuffer& indirect_args, bool standalone, bool clear) {
+void Renderer::render(Texture& target, const IndirectArgsBuffer& indirect_args, bool standalone, bool clear,
+                      Texture* depth) {
   if (target.format() != format_)
     throw Error(GS3D_ERROR_INVALID_ARGUMENT, "render target format differs from the renderer's format");
+  // a pipeline's depth-stencil state and the pass's depth attachment go
+  // together (wgpu validates the same)
+  if (depth_enabled_ != (depth != nullptr))
+    throw Error(GS3D_ERROR_INVALID_ARGUMENT, depth_enabled_ ? "this renderer needs a depth attachment"
+                                                            : "this renderer has no depth-stencil state");
+  if (depth && (depth->format() != kFormatDepth32Float || depth->width() != target.width() ||
+                depth->height() != target.height()))
+    throw Error(GS3D_ERROR_INVALID_ARGUMENT, "the depth attachment must be Depth32Float of the target's size");
   hipStream_t st = st_;
   StageTimer& tm = active_timer();
   const FrameConsts* fc = shared_consts_;
@@ -636,15 +695,9 @@ void Renderer::render(Texture& target, const IndirectArgsBuffer& indirect_args, 
   }
   if (standalone)
     launch_vertex(st, gaussians_.pod(), fc, gaussians_.streams(), indirect_args.dev(), indices_.dev(), splats_.dev(),
-                  indices_.len());
+                  indices_.len(), depth_enabled_ ? splats_.depth() : nullptr);
 
-  TileGrid g{};
-  g.width = target.width();
-  g.height = target.height();
-  g.tile_w = tile_w_;
-  g.tile_h = tile_h_;
-  g.tiles_x = (g.width + tile_w_ - 1) / tile_w_;
-  g.tiles_y = (g.height + tile_h_ - 1) / tile_h_;
+  const TileGrid g = grid_for(target);
   const uint32_t num_tiles = g.tiles_x * g.tiles_y;
   if (ranges_words_ < num_tiles * 2) {
     dev_.wait();
@@ -654,12 +707,18 @@ void Renderer::render(Texture& target, const IndirectArgsBuffer& indirect_args, 
 
   tm.mark(GS3D_STAGE_BIN, false);
   const uint32_t bin_tiles = bin_max_tiles(indices_.len());
+  const int passes = int((ceil_log2(num_tiles) + 7) / 8);
+  const SortScratch tile_scratch = scratch_->get();
   launch_bin_reset(st, ranges_.as<uint32_t>(), num_tiles, bin_status_.as<uint32_t>(), bin_tiles,
-                   bin_ticket_.as<uint32_t>(), pair_count_.as<uint32_t>());
+                   bin_ticket_.as<uint32_t>(), pair_count_.as<uint32_t>(), tile_scratch.hist,
+                   uint32_t(passes) * 256u);
   BinArgs b{};
   b.indirect_args = indirect_args.dev();
   b.indices = indices_.dev();
   b.splats = splats_.dev();
+  // rectangles written with this frame's splats for this grid (not for a
+  // standalone draw, which recomputes the splats)
+  b.rects = (!standalone && splats_.rects_on() && same_grid(splats_.rect_grid(), g)) ? splats_.rects() : nullptr;
   b.grid = g;
   b.pair_keys = pk_[0].as<uint32_t>();
   b.pair_vals = pv_[0].as<uint32_t>();
@@ -669,16 +728,17 @@ void Renderer::render(Texture& target, const IndirectArgsBuffer& indirect_args, 
   b.ticket = bin_ticket_.as<uint32_t>();
   b.max_tiles = bin_tiles;
   b.err = dev_.err();
+  b.sort_hist = tile_scratch.hist;  // the tile sort's first histogram
   launch_bin(st, b);
   tm.mark(GS3D_STAGE_BIN, true);
 
   tm.mark(GS3D_STAGE_TILE_SORT, false);
-  const int passes = int((ceil_log2(num_tiles) + 7) / 8);
   SortCount c{pair_count_.as<uint32_t>(), 1u, capacity_};
   const int which = launch_radix_sort(st, dev_.num_cus(), c, pk_[0].as<uint32_t>(), pv_[0].as<uint32_t>(),
-                                      pk_[1].as<uint32_t>(), pv_[1].as<uint32_t>(), passes, scratch_->get());
+                                      pk_[1].as<uint32_t>(), pv_[1].as<uint32_t>(), passes, tile_scratch, nullptr, 0,
+                                      0, true);
   launch_ranges(st, pair_count_.as<uint32_t>(), capacity_, pk_[which].as<uint32_t>(), ranges_.as<uint32_t>(),
-                dev_.num_cus(), pair_total_dev_);
+                dev_.num_cus(), pair_count_.as<uint32_t>() + 1, pair_total_dev_);
   tm.mark(GS3D_STAGE_TILE_SORT, true);
 
   tm.mark(GS3D_STAGE_COMPOSITE, false);
@@ -709,10 +769,17 @@ void Renderer::render(Texture& target, const IndirectArgsBuffer& indirect_args, 
   ca.frag_count = counting_ ? frag_count_.as<uint32_t>() : nullptr;
   ca.out = target.dev();
   ca.capacity = capacity_;
-  ca.seg_len = seg_len_;
+  // depth-tested draws keep whole tiles (no split-K segments): a lane's
+  // depth write needs its nearest passing fragment in one pass
+  ca.seg_len = depth ? 0x7FFFFFFFu : seg_len_;
   ca.item_tile = item_tile_.as<uint32_t>();
   ca.item_seg = item_seg_.as<uint32_t>();
   ca.seg_base = seg_base_.as<uint32_t>();
+  ca.pair_count = pair_count_.as<uint32_t>();
+  ca.splat_depth = depth ? splats_.depth() : nullptr;
+  ca.depth = depth ? static_cast<float*>(depth->dev()) : nullptr;
+  ca.depth_cmp = int(depth_cmp_);
+  ca.depth_write = depth_write_ ? 1 : 0;
   ca.plan = plan_.as<uint32_t>();
   ca.plan_counts = plan_counts_.as<uint32_t>();
   ca.partial = partial_.as<float4>();
@@ -725,6 +792,7 @@ void Renderer::render(Texture& target, const IndirectArgsBuffer& indirect_args, 
   last_args_ = &indirect_args;
   last_standalone_ = standalone;
   last_clear_ = clear;
+  last_depth_ = depth;
   dev_.note_render(this);
 }
 
@@ -756,12 +824,46 @@ Viewer::Viewer(Device& dev, TextureFormat format, std::shared_ptr<GaussiansBuffe
   renderer.set_timer(&timer);
 }
 
-void Viewer::render(Texture& target) {
+void Viewer::update_selection(const uint32_t* words, uint32_t count) {
+  const uint32_t need = (gaussians_buffer.len() + 31u) / 32u;
+  if (count != need)
+    throw Error(GS3D_ERROR_INVALID_ARGUMENT,
+                "selection needs " + std::to_string(need) + " words, got " + std::to_string(count));
+  if (need == 0) return;
+  // the previous frame may still read the bitset
+  device.wait();
+  if (selection_buffer.size() == 0) selection_buffer = DeviceBuffer(device, size_t(need) * 4);
+  hip_check(hipMemcpy(selection_buffer.get(), words, size_t(need) * 4, hipMemcpyHostToDevice), "hipMemcpy(selection)");
+  preprocessor.set_selection(selection_buffer.as<uint32_t>());
+}
+
+void Viewer::set_depth_stencil(bool enabled, CompareFn cmp, bool write) {
+  renderer.set_depth_stencil(enabled, cmp, write);
+  preprocessor.set_splat_depth(enabled ? splats_buffer.depth() : nullptr);
+}
+
+void Viewer::render_with_depth(Texture& target, Texture& depth, bool load) {
+  splats_buffer.set_rect_grid(renderer.grid_for(target));
+  // the projection also writes the depth sort's first histogram
+  preprocessor.set_sort_hist(radix_sorter.hist(), radix_sorter.len());
   timer.mark(GS3D_STAGE_PREPROCESS, false);
   preprocessor.preprocess(gaussians_buffer.len());
   timer.mark(GS3D_STAGE_PREPROCESS, true);
   timer.mark(GS3D_STAGE_SORT, false);
-  radix_sorter.sort(radix_sort_indirect_args_buffer);
+  radix_sorter.sort(radix_sort_indirect_args_buffer, true);
+  timer.mark(GS3D_STAGE_SORT, true);
+  renderer.render(target, indirect_args_buffer, false, !load, &depth);
+}
+
+void Viewer::render(Texture& target) {
+  splats_buffer.set_rect_grid(renderer.grid_for(target));
+  // the projection also writes the depth sort's first histogram
+  preprocessor.set_sort_hist(radix_sorter.hist(), radix_sorter.len());
+  timer.mark(GS3D_STAGE_PREPROCESS, false);
+  preprocessor.preprocess(gaussians_buffer.len());
+  timer.mark(GS3D_STAGE_PREPROCESS, true);
+  timer.mark(GS3D_STAGE_SORT, false);
+  radix_sorter.sort(radix_sort_indirect_args_buffer, true);
   timer.mark(GS3D_STAGE_SORT, true);
   renderer.render(target, indirect_args_buffer, false);
 }

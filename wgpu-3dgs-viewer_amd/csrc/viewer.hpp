@@ -239,9 +239,23 @@ class SplatsBuffer {
  public:
   SplatsBuffer(Device& dev, uint32_t n);
   Splat2D* dev() const { return buf_.as<Splat2D>(); }
+  // Per-Gaussian fragment depth, kept only for depth-tested draws (null
+  // until enable_depth).
+  float* depth() const { return depth_.as<float>(); }
+  void enable_depth();
+  // Per-Gaussian tile rectangles on one grid, written with the splats so
+  // that binning reads 8 bytes per instance instead of the whole splat.
+  // Set by whoever knows the next frame's target and tile size.
+  void set_rect_grid(const TileGrid& g);
+  bool rects_on() const { return rects_.size() != 0; }
+  const TileGrid& rect_grid() const { return rect_grid_; }
+  uint2* rects() const { return rects_.as<uint2>(); }
 
  private:
-  DeviceBuffer buf_;
+  Device& dev_;
+  uint32_t n_;
+  DeviceBuffer buf_, depth_, rects_;
+  TileGrid rect_grid_{};
 };
 
 // Render target (wgpu::Texture).
@@ -254,6 +268,8 @@ class Texture {
   void* dev() const { return buf_.get(); }
   size_t bytes() const { return buf_.size(); }
   void read(void* dst, size_t bytes) const;
+  // queue.write_texture: replaces the contents (waits for the device first).
+  void write(const void* src, size_t bytes);
 
  private:
   Device* device_;
@@ -295,6 +311,20 @@ class Preprocessor {
                SplatsBuffer& splats, int queue = 0);
   void preprocess(uint32_t gaussian_count);
   const FrameConsts* frame_consts() const { return consts_.as<FrameConsts>(); }
+  // Selection early-out (preprocess.wesl:69-78): `words` is a device bitset
+  // of the Gaussians (null: none); a Gaussian is skipped when its bit equals
+  // `invert`.
+  void set_selection(const uint32_t* words) { selection_ = words; }
+  // Where the fused vertex work writes each splat's fragment depth (null:
+  // nowhere), for depth-tested draws.
+  void set_splat_depth(float* p) { splat_depth_ = p; }
+  // Histogram of the depth sort's first pass, written with the keys
+  // (`sort_max`: the sorter's capacity; null: not written).
+  void set_sort_hist(uint32_t* hist, uint32_t sort_max) {
+    sort_hist_ = hist;
+    sort_max_ = sort_max;
+  }
+  void set_invert_selection(bool invert) { invert_selection_ = invert; }
 
  private:
   Device& dev_;
@@ -310,6 +340,11 @@ class Preprocessor {
   SplatsBuffer& splats_;
   DeviceBuffer consts_, status_, ticket_;
   uint32_t max_tiles_;
+  const uint32_t* selection_ = nullptr;
+  float* splat_depth_ = nullptr;
+  uint32_t* sort_hist_ = nullptr;
+  uint32_t sort_max_ = 0;
+  bool invert_selection_ = true;  // PreprocessorInvertSelectionBuffer::new: initial value true
 };
 
 // Reusable sort scratch for up to `capacity` pairs and `passes` passes.
@@ -331,9 +366,13 @@ class SortScratchBuffers {
 class RadixSorter {
  public:
   RadixSorter(Device& dev, GaussiansDepthBuffer& depth, IndirectIndicesBuffer& indices, int queue = 0);
-  void sort(const RadixSortIndirectArgsBuffer& rs_args);
+  // hist_ready: the preprocessor that wrote the keys also wrote the first
+  // pass's histogram into hist() (Preprocessor::set_sort_hist).
+  void sort(const RadixSortIndirectArgsBuffer& rs_args, bool hist_ready = false);
   // Sort an explicit device count (count = *count_dev, at most len).
-  void sort_count(const uint32_t* count_dev, uint32_t count_mul);
+  void sort_count(const uint32_t* count_dev, uint32_t count_mul, bool hist_ready = false);
+  uint32_t* hist() const { return scratch_.get().hist; }
+  uint32_t len() const { return indices_.len(); }
 
  private:
   Device& dev_;
@@ -355,7 +394,16 @@ class Renderer {
   // clear: LoadOp::Clear(BLACK) (src/renderer.rs:175); otherwise the splats
   // are blended over what the target holds (LoadOp::Load, as the draws of
   // several models share one pass, src/multi_model.rs:505-527).
-  void render(Texture& target, const IndirectArgsBuffer& indirect_args, bool standalone, bool clear = true);
+  // depth: the depth attachment (Depth32Float) of a draw; required exactly
+  // when the renderer was given a depth-stencil state.
+  void render(Texture& target, const IndirectArgsBuffer& indirect_args, bool standalone, bool clear = true,
+              Texture* depth = nullptr);
+  // Renderer::new's depth_stencil (src/renderer.rs:120-137, 300-305): the
+  // depth compare function and depth write of the pipeline, or none.
+  void set_depth_stencil(bool enabled, CompareFn cmp, bool write);
+  bool depth_enabled() const { return depth_enabled_; }
+  // The tile grid of a frame drawn into `target`.
+  TileGrid grid_for(const Texture& target) const;
   void set_seg_len(uint32_t s) { seg_len_ = s; }
   // Renders once in exact mode counting the fragments that are not
   // discarded (the reference's per-fragment work); waits.
@@ -410,6 +458,10 @@ class Renderer {
   const IndirectArgsBuffer* last_args_ = nullptr;
   bool last_standalone_ = false;
   bool last_clear_ = true;
+  Texture* last_depth_ = nullptr;
+  bool depth_enabled_ = false;
+  CompareFn depth_cmp_ = kCmpLess;
+  bool depth_write_ = false;
 };
 
 // Viewer (src/lib.rs:65-276).
@@ -428,6 +480,17 @@ class Viewer {
     gaussian_transform_buffer.update(size, mode, sh_deg, no_sh0, max_std_dev);
   }
   void render(Texture& target);
+  // SelectionBuffer contents (one bit per Gaussian, ceil(N / 32) words) and
+  // PreprocessorInvertSelectionBuffer::update (src/selection/buffer.rs:
+  // 151-172). The bitset starts all zero and invert true: nothing is hidden.
+  void update_selection(const uint32_t* words, uint32_t count);
+  void update_invert_selection(bool invert) { preprocessor.set_invert_selection(invert); }
+  // Viewer::new_with_options(depth_stencil) (src/lib.rs:100-199, 279-284):
+  // the renderer's depth test, and a frame drawn with a depth attachment
+  // (Renderer::render_with_pass in a pass that has one, src/renderer.rs:
+  // 186-195); load keeps the target's colour (LoadOp::Load).
+  void set_depth_stencil(bool enabled, CompareFn cmp, bool write);
+  void render_with_depth(Texture& target, Texture& depth, bool load);
 
   Device& device;
   int queue;
@@ -445,6 +508,7 @@ class Viewer {
   RadixSorter radix_sorter;
   Renderer renderer;
   StageTimer timer;
+  DeviceBuffer selection_buffer;  // allocated by the first update_selection
 };
 
 // ---- host-side packing helpers ---------------------------------------------
