@@ -397,6 +397,13 @@ gs3d_status gs3d_ply_read_file(const char* path, gs3d_gaussian* out, uint64_t ca
 gs3d_status gs3d_ply_read_memory(const void* data, uint64_t bytes, gs3d_gaussian* out, uint64_t capacity,
                                  uint64_t* count);
 
+/* SPZ ingestion, replacing Gaussians::read_from_file with GaussiansSource::Spz
+ * (examples/simple.rs:157-160): Niantic's gzip-compressed layout, versions 2
+ * and 3, same contract as gs3d_ply_read_file. No coordinate conversion. */
+gs3d_status gs3d_spz_read_file(const char* path, gs3d_gaussian* out, uint64_t capacity, uint64_t* count);
+gs3d_status gs3d_spz_read_memory(const void* data, uint64_t bytes, gs3d_gaussian* out, uint64_t capacity,
+                                 uint64_t* count);
+
 /* Synthetic scenes. config 0: means U[-1,1]^3; 1: means U[-4,4]^3 (both
  * log-scale N(-4,0.5)); 2: 64 anisotropic clusters + 20% on the plane
  * y = -2, log-scale N(-5,0.7). Counter-based RNG: record i depends only on

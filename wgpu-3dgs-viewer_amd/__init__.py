@@ -219,6 +219,8 @@ def _declare(L):
         "gs3d_synth_gaussians": ([i32, u64, u32, u32, vp], i32),
         "gs3d_ply_read_file": ([C.c_char_p, vp, u64, vp], i32),
         "gs3d_ply_read_memory": ([vp, u64, vp, u64, vp], i32),
+        "gs3d_spz_read_file": ([C.c_char_p, vp, u64, vp], i32),
+        "gs3d_spz_read_memory": ([vp, u64, vp, u64, vp], i32),
         "gs3d_model_new": ([vp, vp, u32, i32, pp], i32),
         "gs3d_model_destroy": ([vp], None),
         "gs3d_multi_model_viewer_new": ([vp, i32, pp], i32),
@@ -266,7 +268,8 @@ EXPORTED_SYMBOLS = (
     "gs3d_viewer_read_indirect_indices", "gs3d_viewer_read_gaussians_depth", "gs3d_depth_buffer_len",
     "gs3d_radix_sorter_new", "gs3d_radix_sorter_destroy", "gs3d_radix_sorter_write", "gs3d_radix_sorter_sort",
     "gs3d_radix_sorter_read", "gs3d_camera_pod_new", "gs3d_camera_pod_look_at", "gs3d_gaussian_cov3d",
-    "gs3d_pack_half", "gs3d_synth_gaussians", "gs3d_ply_read_file", "gs3d_ply_read_memory", "gs3d_model_new", "gs3d_model_destroy",
+    "gs3d_pack_half", "gs3d_synth_gaussians", "gs3d_ply_read_file", "gs3d_ply_read_memory",
+    "gs3d_spz_read_file", "gs3d_spz_read_memory", "gs3d_model_new", "gs3d_model_destroy",
     "gs3d_multi_model_viewer_new", "gs3d_multi_model_viewer_destroy", "gs3d_multi_model_viewer_insert_model",
     "gs3d_multi_model_viewer_remove_model", "gs3d_multi_model_viewer_update_camera",
     "gs3d_multi_model_viewer_update_camera_with_pod", "gs3d_multi_model_viewer_update_model_transform",
@@ -320,23 +323,40 @@ def synth_gaussians(config: int, n: int, seed: int, first: int = 0) -> np.ndarra
     return out
 
 
-def read_ply(source) -> np.ndarray:
-    """Gaussians::read_from_file(path, GaussiansSource::Ply) (examples/simple.rs:157-160):
-    a standard 3DGS PLY file (path) or its bytes, as gs3d_gaussian records."""
+def _read_records(source, read_file, read_memory) -> np.ndarray:
     n = C.c_uint64()
     if isinstance(source, (bytes, bytearray, memoryview, np.ndarray)):
         buf = np.frombuffer(bytes(source), np.uint8)
-        _check(lib().gs3d_ply_read_memory(_ptr(buf), buf.nbytes, None, 0, C.byref(n)))
+        _check(read_memory(_ptr(buf), buf.nbytes, None, 0, C.byref(n)))
         out = np.zeros(n.value, GAUSSIAN_DTYPE)
-        _check(lib().gs3d_ply_read_memory(_ptr(buf), buf.nbytes, _ptr(out) if n.value else None, n.value,
-                                          C.byref(n)))
+        if n.value:
+            _check(read_memory(_ptr(buf), buf.nbytes, _ptr(out), n.value, C.byref(n)))
         return out
     path = os.fsencode(source)
-    _check(lib().gs3d_ply_read_file(path, None, 0, C.byref(n)))
+    _check(read_file(path, None, 0, C.byref(n)))
     out = np.zeros(n.value, GAUSSIAN_DTYPE)
     if n.value:
-        _check(lib().gs3d_ply_read_file(path, _ptr(out), n.value, C.byref(n)))
+        _check(read_file(path, _ptr(out), n.value, C.byref(n)))
     return out
+
+
+def read_ply(source) -> np.ndarray:
+    """Gaussians::read_from_file(path, GaussiansSource::Ply) (examples/simple.rs:157-160):
+    a standard 3DGS PLY file (path) or its bytes, as gs3d_gaussian records."""
+    return _read_records(source, lib().gs3d_ply_read_file, lib().gs3d_ply_read_memory)
+
+
+def read_spz(source) -> np.ndarray:
+    """Gaussians::read_from_file(path, GaussiansSource::Spz): an SPZ file (path) or its bytes."""
+    return _read_records(source, lib().gs3d_spz_read_file, lib().gs3d_spz_read_memory)
+
+
+def read_gaussians(path) -> np.ndarray:
+    """The examples' loader (examples/simple.rs:157-160): PLY, else SPZ."""
+    try:
+        return read_ply(path)
+    except Gs3dError:
+        return read_spz(path)
 
 
 def gaussians(pos, rot, scale, color, sh=None) -> np.ndarray:

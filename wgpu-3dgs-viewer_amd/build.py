@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "libgs3d.so")
 ARCH = os.environ.get("GS3D_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["preprocess.hip", "radix_sort.hip", "tiles.hip", "composite.hip", "viewer.cpp", "batch.cpp",
-           "capi.cpp", "synth.cpp", "ply.cpp"]
+           "capi.cpp", "synth.cpp", "ply.cpp", "spz.cpp"]
 
 # -ffp-contract=off: the splat arithmetic follows a fixed operation order
 # (csrc/math.hpp) so that depth keys and visibility match the f32 oracle bit
@@ -59,7 +59,7 @@ def build(verbose: bool = False, out: str = LIB, defines=()) -> str:
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(lambda src: _compile(src, build_dir, defines), SOURCES))
     tmp = out + ".tmp"
-    cmd = [_hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
+    cmd = [_hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

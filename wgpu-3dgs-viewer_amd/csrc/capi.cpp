@@ -498,6 +498,21 @@ gs3d_status gs3d_ply_read_memory(const void* data, uint64_t bytes, gs3d_gaussian
   });
 }
 
+gs3d_status gs3d_spz_read_file(const char* path, gs3d_gaussian* out, uint64_t capacity, uint64_t* count) {
+  return guard([&] {
+    need(path != nullptr && count != nullptr, "null argument");
+    *count = gs3d::spz_read_file(path, out, capacity);
+  });
+}
+
+gs3d_status gs3d_spz_read_memory(const void* data, uint64_t bytes, gs3d_gaussian* out, uint64_t capacity,
+                                 uint64_t* count) {
+  return guard([&] {
+    need(data != nullptr && count != nullptr, "null argument");
+    *count = gs3d::spz_read(static_cast<const uint8_t*>(data), size_t(bytes), out, capacity);
+  });
+}
+
 gs3d_status gs3d_synth_gaussians(int config, uint64_t seed, uint32_t first, uint32_t count, gs3d_gaussian* out) {
   return guard([&] {
     need(config >= 0 && config <= 2, "unknown synthetic config");

@@ -104,7 +104,9 @@ __device__ __forceinline__ uint2 tile_rect(float cx, float cy, uint32_t hxy, con
   if (!(fx1 >= 0.0f) || !(fx0 <= wmax) || !(fy1 >= 0.0f) || !(fy0 <= hmax)) return none;
   const uint32_t x0 = uint32_t(fmaxf(fx0, 0.0f)), x1 = uint32_t(fminf(fx1, wmax));
   const uint32_t y0 = uint32_t(fmaxf(fy0, 0.0f)), y1 = uint32_t(fminf(fy1, hmax));
-  return make_uint2((x0 / g.tile_w) | ((y0 / g.tile_h) << 16), (x1 / g.tile_w) | ((y1 / g.tile_h) << 16));
+  // tile sizes are powers of two (8x8, 16x16, 32x8): shifts, not divisions
+  const uint32_t sx = uint32_t(__builtin_ctz(g.tile_w)), sy = uint32_t(__builtin_ctz(g.tile_h));
+  return make_uint2((x0 >> sx) | ((y0 >> sy) << 16), (x1 >> sx) | ((y1 >> sy) << 16));
 }
 
 // SH colour of one instance (utils.wesl:82-136) for the splat record:

@@ -26,7 +26,12 @@ constexpr int kPreWaves = kPreThreads / kWave;
 #define GS3D_PRE_ITEMS 4
 #endif
 #ifndef GS3D_PRE_WAVES_PER_EU
-#define GS3D_PRE_WAVES_PER_EU 1
+#define GS3D_PRE_WAVES_PER_EU 4
+#endif
+// 1: every item's covariance words are loaded up front with its position;
+// 0: they are loaded where the splat record needs them (fewer registers)
+#ifndef GS3D_PRE_PREFETCH_COV
+#define GS3D_PRE_PREFETCH_COV 0
 #endif
 constexpr int kPreItems = GS3D_PRE_ITEMS;
 constexpr uint32_t kPreTile = kPreThreads * kPreItems;
@@ -67,6 +72,45 @@ __global__ __launch_bounds__(64) void k_frame_consts(const CameraPod* __restrict
   }
 }
 
+// world = model * (pos, 1) and the centre's NDC ((proj * view) * world /
+// w, preprocess.wesl:80-82) in the bit-exact order of math.hpp.
+__device__ __forceinline__ void project_centre(const FrameConsts& fc, float4 pc, float world[4], float& nx,
+                                               float& ny, float& nz) {
+  const float p4[4] = {pc.x, pc.y, pc.z, 1.0f};
+  float clip[4];
+  m4v(fc.model, p4, world);  // model_to_world
+  m4v(fc.pv, world, clip);   // world_to_camera: (proj * view) * world
+  nx = clip[0] / clip[3];
+  ny = clip[1] / clip[3];
+  nz = clip[2] / clip[3];
+}
+
+// preprocess.wesl:87-98: a culled centre is re-tested after moving it
+// towards the screen by the projected major axis.
+__device__ __forceinline__ bool bound_visible(const FrameConsts& fc, const float pos[3], const float c[6], float nx,
+                                              float ny, float nz) {
+  const float sd = fc.max_std_dev;
+  float ax[4];
+  cov2d_axes(fc, pos, c, sd * fc.gsize, ax);
+  const float mx = (ax[0] * sd) / fc.size[0];
+  const float my = (ax[1] * sd) / fc.size[1];
+  const float mlen = sqrtf(mx * mx + my * my);
+  const float nlen = sqrtf((-nx) * (-nx) + (-ny) * (-ny));
+  const float dx = (-nx) / nlen;
+  const float dy = (-ny) / nlen;
+  const float t = fminf(mlen, nlen);
+  return !cull(nx + t * dx, ny + t * dy, nz);
+}
+
+// One workgroup per kPreTile Gaussians (taken by ticket, so that tiles finish
+// roughly in order for the look-back). Three phases:
+//   A  every item: centre NDC, depth key, centre cull;
+//   B  the items whose centre is culled, gathered into a queue so that the
+//      (expensive, rarely needed) bound re-test runs on full waves;
+//   C  the visible count is published for the decoupled look-back, the
+//      visible items' splat records are written, and only then does the
+//      tile look back (its predecessors have mostly resolved meanwhile) and
+//      write the visible indices and keys in Gaussian order.
 template <int POD>
 __global__ __launch_bounds__(kPreThreads) __attribute__((amdgpu_waves_per_eu(GS3D_PRE_WAVES_PER_EU))) void k_preprocess(
     PreprocessArgs a) {
@@ -75,12 +119,17 @@ __global__ __launch_bounds__(kPreThreads) __attribute__((amdgpu_waves_per_eu(GS3
   __shared__ uint32_t s_base;
   __shared__ uint32_t s_total;
   __shared__ uint32_t s_hist[256];  // digit 0 of this tile's depth keys
+  __shared__ uint32_t s_qlen;
+  __shared__ uint16_t s_q[kPreTile];    // slots (k * kPreThreads + tid) whose centre is culled
+  __shared__ uint8_t s_qvis[kPreTile];  // their re-test, by slot
   if (a.sort_hist) s_hist[threadIdx.x] = 0u;
+  if (threadIdx.x == 0) s_qlen = 0u;
 
   const FrameConsts& fc = *a.fc;
   const uint32_t tile = take_ticket(a.ticket, &s_tile);
   const uint32_t tid = threadIdx.x;
   const uint32_t w = tid / kWave;
+  const uint32_t lane = tid % kWave;
   const uint32_t base = tile * kPreTile;
 
   bool vis[kPreItems];
@@ -104,70 +153,87 @@ __global__ __launch_bounds__(kPreThreads) __attribute__((amdgpu_waves_per_eu(GS3
     }
     if (act[k]) {
       pcs[k] = a.g.pc[i];
-      cws[k] = load_cov_words<POD>(a.g, i);
+      if (GS3D_PRE_PREFETCH_COV) cws[k] = load_cov_words<POD>(a.g, i);
     }
   }
 
+  // A
+  bool culled[kPreItems];
 #pragma unroll
   for (int k = 0; k < kPreItems; ++k) {
-    const uint32_t i = base + uint32_t(k) * kPreThreads + tid;
-    vis[k] = false;
     key[k] = 0.0f;
+    culled[k] = false;
+    vis[k] = false;
     if (act[k]) {
-      const float4 pc = pcs[k];
-      const float pos[3] = {pc.x, pc.y, pc.z};
-      const float p4[4] = {pc.x, pc.y, pc.z, 1.0f};
-      float world[4], clip[4];
-      m4v(fc.model, p4, world);  // model_to_world
-      m4v(fc.pv, world, clip);   // world_to_camera: (proj * view) * world
-      const float nx = clip[0] / clip[3];
-      const float ny = clip[1] / clip[3];
-      const float nz = clip[2] / clip[3];
-      bool visible = true;
-      float c[6];
-      cov3d_of<POD>(cws[k], c);
-      if (cull(nx, ny, nz)) {
-        // preprocess.wesl:87-98: re-test the centre moved towards the
-        // screen by the projected major axis.
-        const float sd = fc.max_std_dev;
-        float ax[4];
-        cov2d_axes(fc, pos, c, sd * fc.gsize, ax);
-        const float mx = (ax[0] * sd) / fc.size[0];
-        const float my = (ax[1] * sd) / fc.size[1];
-        const float mlen = sqrtf(mx * mx + my * my);
-        const float nlen = sqrtf((-nx) * (-nx) + (-ny) * (-ny));
-        const float dx = (-nx) / nlen;
-        const float dy = (-ny) / nlen;
-        const float t = fminf(mlen, nlen);
-        visible = !cull(nx + t * dx, ny + t * dy, nz);
-      }
-      if (visible) {
-        key[k] = 1.0f - nz;  // preprocess.wesl:105
-        compute_splat<POD>(fc, a.g, i, pos, __float_as_uint(pc.w), world, c, a.splats + i,
-                           a.splat_depth ? a.splat_depth + i : nullptr, a.rects ? a.rects + i : nullptr,
-                           &a.rect_grid);
-      }
-      vis[k] = visible;
+      float world[4], nx, ny, nz;
+      project_centre(fc, pcs[k], world, nx, ny, nz);
+      key[k] = 1.0f - nz;  // preprocess.wesl:105
+      culled[k] = cull(nx, ny, nz);
+      vis[k] = !culled[k];
     }
-    masks[k] = __ballot(vis[k]);
-    if (lane_id() == 0) s_cnt[k][w] = uint32_t(__popcll(masks[k]));
+    const uint64_t m = __ballot(culled[k]);
+    uint32_t qb = 0;
+    if (lane == 0 && m) qb = atomicAdd(&s_qlen, uint32_t(__popcll(m)));
+    qb = __shfl(qb, 0, kWave);
+    if (culled[k]) s_q[qb + mask_rank(m)] = uint16_t(uint32_t(k) * kPreThreads + tid);
   }
   __syncthreads();
 
+  // B
+  const uint32_t qlen = s_qlen;
+  for (uint32_t j = tid; j < qlen; j += kPreThreads) {
+    const uint32_t e = s_q[j];
+    const uint32_t i = base + e;
+    const float4 pc = a.g.pc[i];
+    float world[4], nx, ny, nz, c[6];
+    project_centre(fc, pc, world, nx, ny, nz);
+    cov3d_of<POD>(load_cov_words<POD>(a.g, i), c);
+    const float pos[3] = {pc.x, pc.y, pc.z};
+    s_qvis[e] = bound_visible(fc, pos, c, nx, ny, nz) ? 1u : 0u;
+  }
+  __syncthreads();
+
+  // the tile's visible count is known: publish it now and look back only
+  // after the splat records, so that predecessors have resolved meanwhile
+#pragma unroll
+  for (int k = 0; k < kPreItems; ++k) {
+    if (culled[k]) vis[k] = s_qvis[uint32_t(k) * kPreThreads + tid] != 0u;
+    masks[k] = __ballot(vis[k]);
+    if (lane == 0) s_cnt[k][w] = uint32_t(__popcll(masks[k]));
+  }
+  __syncthreads();
+  uint32_t run = 0;  // meaningful in thread 0
   if (tid == 0) {
     // exclusive scan over (item, wave) = ascending Gaussian index order
-    uint32_t run = 0;
     for (int k = 0; k < kPreItems; ++k)
       for (int ww = 0; ww < kPreWaves; ++ww) {
         const uint32_t c = s_cnt[k][ww];
         s_cnt[k][ww] = run;
         run += c;
       }
+    status_store(a.status + tile, (tile == 0 ? kFlagInclusive : kFlagAggregate) | run);
+  }
+
+  // C: the splat records of the visible items
+#pragma unroll
+  for (int k = 0; k < kPreItems; ++k) {
+    if (vis[k]) {
+      const uint32_t i = base + uint32_t(k) * kPreThreads + tid;
+      const float4 pc = pcs[k];
+      const float pos[3] = {pc.x, pc.y, pc.z};
+      const float p4[4] = {pc.x, pc.y, pc.z, 1.0f};
+      float world[4], c[6];
+      m4v(fc.model, p4, world);
+      cov3d_of<POD>(GS3D_PRE_PREFETCH_COV ? cws[k] : load_cov_words<POD>(a.g, i), c);
+      compute_splat<POD>(fc, a.g, i, pos, __float_as_uint(pc.w), world, c, a.splats + i,
+                         a.splat_depth ? a.splat_depth + i : nullptr, a.rects ? a.rects + i : nullptr,
+                         &a.rect_grid);
+    }
+  }
+
+  if (tid == 0) {
     uint32_t excl = 0;
-    if (tile == 0) {
-      status_store(a.status, kFlagInclusive | run);
-    } else {
-      status_store(a.status + tile, kFlagAggregate | run);
+    if (tile > 0) {
       excl = lookback_exclusive(a.status, tile, a.err);
       status_store(a.status + tile, kFlagInclusive | (excl + run));
     }

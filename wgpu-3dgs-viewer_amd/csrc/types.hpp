@@ -137,7 +137,7 @@ constexpr uint32_t kErrTileOverflow = 2u;
 // Screen tiles of a render target.
 struct TileGrid {
   uint32_t width, height;   // target pixels
-  uint32_t tile_w, tile_h;  // tile pixels
+  uint32_t tile_w, tile_h;  // tile pixels (powers of two)
   uint32_t tiles_x, tiles_y;
 };
 

@@ -521,6 +521,9 @@ uint32_t pack_color(const uint8_t c[4]);
 // returned; otherwise capacity must hold every vertex.
 uint64_t ply_read(const uint8_t* data, size_t size, gs3d_gaussian* out, uint64_t capacity);
 uint64_t ply_read_file(const char* path, gs3d_gaussian* out, uint64_t capacity);
+// SPZ ingestion (spz.cpp), same contract.
+uint64_t spz_read(const uint8_t* data, size_t size, gs3d_gaussian* out, uint64_t capacity);
+uint64_t spz_read_file(const char* path, gs3d_gaussian* out, uint64_t capacity);
 
 // Synthetic scenes (synth.cpp).
 void synth_gaussians(int config, uint64_t seed, uint32_t first, uint32_t count, gs3d_gaussian* out);
