@@ -205,7 +205,7 @@ void BatchRenderer::render(const CameraPod* pods, bool use_graph) {
 void BatchRenderer::finish() {
   dev_.wait();
   const uint32_t e = dev_.take_errors();
-  if (e & kErrLookbackSpin) throw Error(GS3D_ERROR_KERNEL, "a kernel exceeded its look-back wait limit");
+  throw_kernel_errors(e);
   bool regrown = false;
   for (auto& v : viewers_)
     if (v->renderer.grow_if_overflowed()) {
@@ -399,7 +399,7 @@ ServeStats Server::run(const std::vector<ServeRequest>& reqs, const std::vector<
     }
   }
   const uint32_t e = dev_.take_errors();
-  if (e & kErrLookbackSpin) throw Error(GS3D_ERROR_KERNEL, "a kernel exceeded its look-back wait limit");
+  throw_kernel_errors(e);
   st.seconds = std::max(last_done - first_arrival, 1e-9);
   st.frames_per_s = double(st.completed) / st.seconds;
   st.mean_in_flight = in_flight_area / std::max(last_t, 1e-9);

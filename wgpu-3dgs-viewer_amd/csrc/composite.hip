@@ -30,11 +30,13 @@
 #include "kernels.hpp"
 #include "types.hpp"
 
+
 namespace gs3d {
 
 constexpr int kCompMaxThreads = 256;
 constexpr int kCompMaxWaves = kCompMaxThreads / kWave;
 constexpr float kLog2e = 1.4426950408889634f;
+constexpr float kSqrtLog2e = 1.2011224087864498f;
 
 // Lanes stop once what is left behind contributes at most 1e-4 of a colour.
 constexpr float kMinTransmittance = 1e-4f;
@@ -62,15 +64,16 @@ constexpr uint32_t kListPad = 4;
 // the exact path per 8x8 block (lists 0 .. NT / 64 - 1). Sized per tile
 // shape, so that 8x8 tiles (one wave) take a quarter of a 256-lane tile's
 // LDS and as many workgroups fit a CU.
-template <int NT>
+template <int NT, bool Z = true>
 struct StageT {
   static constexpr int kWaves = NT / kWave;
   static constexpr int kQuads = 4 * kWaves;
   static constexpr uint32_t kDummy = NT;
-  float4 geo[NT + 1];  // splat: cx, cy, ca*log2e, 2*cb*log2e   point: cx, cy, hx, hy
-  float4 col[NT + 1];  // cc*log2e, opacity (fast splat: log2 opacity), r, g
-  float blue[NT + 1];
-  float z[NT + 1];  // fragment depth, for depth-tested draws
+  // splat / ellipse: |qo|^2 log2(e) = (d.u')^2 + (d.v')^2, u' = u sqrt(log2(e)), v' likewise
+  float4 geo[NT + 1];  // splat: cx, cy, u'.x, u'.y   point: cx, cy, hx, hy
+  float4 col[NT + 1];  // splat: v'.x, v'.y, opacity (fast splat: log2 opacity), r   point: 0, 0, opacity, r
+  float2 gb[NT + 1];   // g, b
+  float z[Z ? NT + 1 : 1];  // fragment depth, for depth-tested draws (Z)
   uint16_t list[kQuads][NT + kListPad];
   uint32_t wcount[kWaves][kWaves];  // exact path: [staging wave][target wave]
   uint64_t qbal[kWaves][kQuads];    // fast path: [staging wave][quadrant] ballots
@@ -79,14 +82,14 @@ using Stage = StageT<kCompMaxThreads>;
 
 // The fast path stages log2(opacity) for splat mode (its alpha is then one
 // exp2 of log2(opacity) - |qo|^2 log2(e)): the dummy's opacity 0 is -inf.
-template <int NT>
-__device__ __forceinline__ void init_dummy(StageT<NT>& s, int mode) {
-  constexpr uint32_t kDummy = StageT<NT>::kDummy;
+template <int NT, bool Z>
+__device__ __forceinline__ void init_dummy(StageT<NT, Z>& s, int mode) {
+  constexpr uint32_t kDummy = StageT<NT, Z>::kDummy;
   if (threadIdx.x == 0) {
     s.geo[kDummy] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    s.col[kDummy] = make_float4(0.0f, mode == kModeSplat ? -__builtin_inff() : 0.0f, 0.0f, 0.0f);
-    s.blue[kDummy] = 0.0f;
-    s.z[kDummy] = 0.0f;
+    s.col[kDummy] = make_float4(0.0f, 0.0f, mode == kModeSplat ? -__builtin_inff() : 0.0f, 0.0f);
+    s.gb[kDummy] = make_float2(0.0f, 0.0f);
+    if (Z) s.z[kDummy] = 0.0f;
   }
 }
 
@@ -114,6 +117,30 @@ __device__ __forceinline__ Tiling setup_tile(const TileGrid& g, uint32_t tile) {
   return t;
 }
 
+// Stage one splat record into slot tid. `log2op`: store log2(opacity) (the
+// fast path's splat mode) instead of the opacity.
+template <int NT, bool Z>
+__device__ __forceinline__ void stage_splat(StageT<NT, Z>& s, uint32_t tid, const Splat2D& r, bool clamp_src, int mode,
+                                            bool log2op) {
+  float cr = h_lo(r.rg), cg = h_hi(r.rg), cb = h_lo(r.ba);
+  const float op = h_hi(r.ba);
+  if (clamp_src) {
+    cr = clamp01(cr);
+    cg = clamp01(cg);
+    cb = clamp01(cb);
+  }
+  if (mode == kModePoint) {
+    s.geo[tid] = make_float4(r.cx, r.cy, r.ux, r.uy);
+    s.col[tid] = make_float4(0.0f, 0.0f, op, cr);
+  } else {
+    const float ux = r.ux * kSqrtLog2e, uy = r.uy * kSqrtLog2e;
+    s.geo[tid] = make_float4(r.cx, r.cy, ux, uy);
+    // log2(0) = -inf: exp2 gives alpha 0
+    s.col[tid] = make_float4(-r.rho * uy, r.rho * ux, log2op ? __log2f(op) : op, cr);
+  }
+  s.gb[tid] = make_float2(cg, cb);
+}
+
 // Stage splat k (or nothing when !valid) into slot threadIdx.x and build the
 // per-wave lists. Contains __syncthreads; every lane must call it.
 __device__ __forceinline__ void stage_batch(Stage& s, const Tiling& t, const Splat2D* splats, const float* zsrc,
@@ -122,21 +149,7 @@ __device__ __forceinline__ void stage_batch(Stage& s, const Tiling& t, const Spl
   uint32_t mask = 0u;
   if (valid) {
     const Splat2D r = splats[k];
-    float cr = h_lo(r.rg), cg = h_hi(r.rg), cb = h_lo(r.ba);
-    const float op = h_hi(r.ba);
-    if (clamp_src) {
-      cr = clamp01(cr);
-      cg = clamp01(cg);
-      cb = clamp01(cb);
-    }
-    if (mode == kModePoint) {
-      s.geo[tid] = make_float4(r.cx, r.cy, r.ca, r.cb);
-      s.col[tid] = make_float4(0.0f, op, cr, cg);
-    } else {
-      s.geo[tid] = make_float4(r.cx, r.cy, r.ca * kLog2e, 2.0f * r.cb * kLog2e);
-      s.col[tid] = make_float4(r.cc * kLog2e, op, cr, cg);
-    }
-    s.blue[tid] = cb;
+    stage_splat(s, tid, r, clamp_src, mode, false);
     if (zsrc) s.z[tid] = zsrc[k];
     // which 8x8 blocks of the tile the bound [c - h, c + h] can touch (pixel
     // centres of block (i, j) span [o + 8i + 0.5, o + 8i + 7.5]; half a pixel
@@ -170,32 +183,20 @@ __device__ __forceinline__ void stage_batch(Stage& s, const Tiling& t, const Spl
 
 // Stage splat k (or nothing) into slot threadIdx.x and build one ordered list
 // per 4x4 quadrant of every wave's 8x8 block: quadrant (qx, qy) of wave w is
-// list 4w + qx + 2qy. Contains __syncthreads; every lane must call it.
-template <int NT>
-__device__ __forceinline__ void stage_batch_quads(StageT<NT>& s, const Tiling& t, const Splat2D* splats,
+// list 4w + qx + 2qy. (Testing each quadrant against the splat's ellipse,
+// as the binning does per tile, shortens the lists but costs more in staging
+// than it saves in blending: measured.) Contains __syncthreads; every lane
+// must call it.
+template <int NT, bool Z>
+__device__ __forceinline__ void stage_batch_quads(StageT<NT, Z>& s, const Tiling& t, const Splat2D* splats,
                                                   const float* zsrc, bool valid, uint32_t k, bool clamp_src,
                                                   int mode) {
   const uint32_t tid = threadIdx.x, lane = tid % kWave, sw = tid / kWave;
   uint32_t mask = 0u;
   if (valid) {
     const Splat2D r = splats[k];
-    float cr = h_lo(r.rg), cg = h_hi(r.rg), cb = h_lo(r.ba);
-    const float op = h_hi(r.ba);
-    if (clamp_src) {
-      cr = clamp01(cr);
-      cg = clamp01(cg);
-      cb = clamp01(cb);
-    }
-    if (mode == kModePoint) {
-      s.geo[tid] = make_float4(r.cx, r.cy, r.ca, r.cb);
-      s.col[tid] = make_float4(0.0f, op, cr, cg);
-    } else {
-      s.geo[tid] = make_float4(r.cx, r.cy, r.ca * kLog2e, 2.0f * r.cb * kLog2e);
-      // log2(0) = -inf: exp2 gives alpha 0
-      s.col[tid] = make_float4(r.cc * kLog2e, mode == kModeSplat ? __log2f(op) : op, cr, cg);
-    }
-    s.blue[tid] = cb;
-    if (zsrc) s.z[tid] = zsrc[k];
+    stage_splat(s, tid, r, clamp_src, mode, mode == kModeSplat);
+    if (Z && zsrc) s.z[tid] = zsrc[k];
     // Quadrant column c of the tile holds pixel centres [4c + 0.5, 4c + 3.5]
     // (tile-relative); the bound [x0, x1] may touch it when x1 >= 4c and
     // x0 <= 4c + 4 (half a pixel of margin). Same for rows. Quadrant (c, r)
@@ -210,7 +211,7 @@ __device__ __forceinline__ void stage_batch_quads(StageT<NT>& s, const Tiling& t
     for (int c = c0; c <= c1; ++c) colmask |= 1u << (4 * (c >> 1) + (c & 1));
     for (int rr = r0; rr <= r1; ++rr) mask |= colmask << (4 * int(t.blocks_x) * (rr >> 1) + 2 * (rr & 1));
   }
-  constexpr int kQuads = StageT<NT>::kQuads;
+  constexpr int kQuads = StageT<NT, Z>::kQuads;
   uint64_t bal[kQuads];
 #pragma unroll
   for (int q = 0; q < kQuads; ++q) bal[q] = __ballot((mask >> q) & 1u);
@@ -248,10 +249,11 @@ __device__ __forceinline__ float fragment(const Stage& s, uint32_t q, const Tili
   const float dy = t.fpy - g.y;
   if (mode == kModePoint) return (dx >= -g.z && dx < g.z && dy >= -g.w && dy < g.w) ? 1.0f : -1.0f;
   // |qo|^2 log2(e)
-  const float r2 = dx * (g.z * dx + g.w * dy) + col.x * dy * dy;
+  const float t1 = dx * g.z + dy * g.w, t2 = dx * col.x + dy * col.y;
+  const float r2 = t1 * t1 + t2 * t2;
   if (!(r2 <= sd2l)) return -1.0f;
-  if (mode == kModeEllipse) return col.y + (1.0f - col.y) * ((r2 > ell2l) ? 1.0f : 0.0f);
-  return col.y * __builtin_amdgcn_exp2f(-r2);
+  if (mode == kModeEllipse) return col.z + (1.0f - col.z) * ((r2 > ell2l) ? 1.0f : 0.0f);
+  return col.z * __builtin_amdgcn_exp2f(-r2);
 }
 
 // Depth test of a fragment at depth z against the attachment's value d.
@@ -279,14 +281,15 @@ __device__ __forceinline__ float fragment_alpha(float4 g, float4 c, const Tiling
   const float dy = t.fpy - g.y;
   if (MODE == kModePoint) {
     kept = dx >= -g.z && dx < g.z && dy >= -g.w && dy < g.w;
-    return kept ? c.y : 0.0f;
+    return kept ? c.z : 0.0f;
   }
-  const float r2 = dx * (g.z * dx + g.w * dy) + c.x * dy * dy;
+  const float t1 = dx * g.z + dy * g.w, t2 = dx * c.x + dy * c.y;
+  const float r2 = t1 * t1 + t2 * t2;
   float al;
   if (MODE == kModeEllipse)
-    al = c.y + (1.0f - c.y) * ((r2 > ell2l) ? 1.0f : 0.0f);
+    al = c.z + (1.0f - c.z) * ((r2 > ell2l) ? 1.0f : 0.0f);
   else
-    al = __builtin_amdgcn_exp2f(c.y - r2);  // c.y = log2(opacity)
+    al = __builtin_amdgcn_exp2f(c.z - r2);  // c.z = log2(opacity)
   kept = r2 <= sd2l;
   return kept ? al : 0.0f;
 }
@@ -315,7 +318,7 @@ static_assert(kListPad % kUnroll == 0, "a step reads at most kListPad entries pa
 // blending (their colour only gets closer to the exact sum) until the whole
 // wave is done.
 template <int MODE, bool DEPTH, int NT>
-__device__ __forceinline__ void blend_quads(const StageT<NT>& s, const Tiling& t, uint32_t w, float sd2l, float ell2l,
+__device__ __forceinline__ void blend_quads(const StageT<NT, DEPTH>& s, const Tiling& t, uint32_t w, float sd2l, float ell2l,
                                             float& T, float& cr, float& cg, float& cb, bool& done, DepthLane& dl) {
 #pragma clang fp contract(fast)
   const uint32_t lane = threadIdx.x % kWave;
@@ -329,10 +332,10 @@ __device__ __forceinline__ void blend_quads(const StageT<NT>& s, const Tiling& t
   // the same for the whole wave: keep the loop bound in a scalar register
   const uint32_t wmax =
       __builtin_amdgcn_readfirstlane(max(max(lens[0], lens[1]), max(lens[2], lens[3])));
-  constexpr uint32_t kDummyOff = StageT<NT>::kDummy * 16u;
+  constexpr uint32_t kDummyOff = StageT<NT, DEPTH>::kDummy * 16u;
   const char* geo = reinterpret_cast<const char*>(s.geo);
   const char* col = reinterpret_cast<const char*>(s.col);
-  const char* blue = reinterpret_cast<const char*>(s.blue);
+  const char* gbs = reinterpret_cast<const char*>(s.gb);
   const char* zs = reinterpret_cast<const char*>(s.z);
   for (uint32_t j = 0; j < wmax; j += kUnroll) {
     if ((j & 31u) == 0u && __all(done)) break;
@@ -348,12 +351,13 @@ __device__ __forceinline__ void blend_quads(const StageT<NT>& s, const Tiling& t
       }
     }
     float4 g[kUnroll], c[kUnroll];
-    float bl[kUnroll], zz[kUnroll];
+    float2 gb[kUnroll];
+    float zz[kUnroll];
 #pragma unroll
     for (int k = 0; k < kUnroll; ++k) {
       g[k] = *reinterpret_cast<const float4*>(geo + q[k]);
       c[k] = *reinterpret_cast<const float4*>(col + q[k]);
-      bl[k] = *reinterpret_cast<const float*>(blue + (q[k] >> 2));
+      gb[k] = *reinterpret_cast<const float2*>(gbs + (q[k] >> 1));
       if (DEPTH) zz[k] = *reinterpret_cast<const float*>(zs + (q[k] >> 2));
     }
 #pragma unroll
@@ -372,9 +376,9 @@ __device__ __forceinline__ void blend_quads(const StageT<NT>& s, const Tiling& t
         }
       }
       const float wgt = T * al;
-      cr += wgt * c[k].z;
-      cg += wgt * c[k].w;
-      cb += wgt * bl[k];
+      cr += wgt * c[k].w;
+      cg += wgt * gb[k].x;
+      cb += wgt * gb[k].y;
       T = __builtin_fmaf(-T, al, T);
     }
     done = T < kMinTransmittance;
@@ -531,11 +535,12 @@ __global__ __launch_bounds__(kPlanTiles) void k_plan(CompositeArgs a) {
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     // all items, never more than the list holds (the bound is exact for lists
-    // built from at most `capacity` pairs); multi-segment items; work ticket
+    // built from at most `capacity` pairs); multi-segment items
     const uint32_t all = ha + la;
     a.plan[0] = all < max_items ? all : max_items;
     a.plan[1] = ha;
-    a.plan[2] = 0u;
+    // the work ticket, in a cache line of its own
+    a.plan[kPlanTicketWord] = 0u;
   }
 }
 
@@ -543,7 +548,7 @@ template <bool DEPTH, int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_composite_fast(CompositeArgs a) {
 #pragma clang fp contract(fast)
   if (*a.pair_count > a.capacity) return;  // truncated lists: the frame is drawn again
-  __shared__ StageT<NT> s;
+  __shared__ StageT<NT, DEPTH> s;
   __shared__ uint32_t s_item;
   const FrameConsts& fc = *a.fc;
   const int mode = fc.display_mode;
@@ -552,11 +557,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
   const float ell2l = e * e * kLog2e;
   const bool q8 = a.format == kFormatRgba8Unorm;
   const uint32_t nt = blockDim.x, tid = threadIdx.x, w = tid / kWave;
+  // read once: loads from the ticket's cache line while every workgroup
+  // takes tickets from it wait behind the atomics (measured: 3x slower)
   const uint32_t items = a.plan[0], heavy = a.plan[1];
   init_dummy(s, mode);
 
   for (;;) {
-    const uint32_t item = take_ticket(a.plan + 2, &s_item);
+    const uint32_t item = take_ticket(a.plan + kPlanTicketWord, &s_item);
     if (item >= items) break;
     const uint32_t tile = a.item_tile[item], seg = a.item_seg[item];
     const Tiling t = setup_tile(a.grid, tile);
@@ -675,9 +682,10 @@ __global__ __launch_bounds__(kCompMaxThreads) void k_composite_exact(CompositeAr
       }
       ++frags;
       const float om = 1.0f - al;
-      dr = col.z * al + dr * om;
-      dg = col.w * al + dg * om;
-      db = s.blue[q] * al + db * om;
+      const float2 gb = s.gb[q];
+      dr = col.w * al + dr * om;
+      dg = gb.x * al + dg * om;
+      db = gb.y * al + db * om;
       da = al + da * om;
       if (q8) {
         dr = float(unorm8(dr)) / 255.0f;

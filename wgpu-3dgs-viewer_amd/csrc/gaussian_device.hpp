@@ -15,11 +15,14 @@ namespace gs3d {
 // Gaussian in each stream):
 //   pc   : float4  pos.xyz, color rgba8 bits in .w            16 B
 //   kPodSingle: cov  : float2[3]  cov3d xx,xy,xz,yy,yz,zz       24 B
-//               sh   : float4[12] 45 SH floats + 3 pad          192 B
+//               sh   : 12 planes of float4: 45 SH floats + 3 pad 192 B
 //   kPodHalf:   rs   : uint4  rot xyzw f16, scale xyz f16, pad   16 B
-//               sh   : uint4[6] 45 SH halves + 3 pad             96 B
-// The culled majority only reads `pc` and the covariance stream; the SH
-// stream is read for visible splats only.
+//               sh   : 6 planes of uint4: 45 SH halves + 3 pad   96 B
+// The SH stream is plane-major: 16-byte chunk q of Gaussian i is element
+// q * n + i, so one load instruction of a wave reads one contiguous 1 KiB
+// run of a plane, and lower SH degrees touch only their planes. The culled
+// majority only reads `pc` and the covariance stream; the SH stream is read
+// for visible splats only.
 struct GaussianStreams {
   const float4* pc;
   const float2* cov;  // kPodSingle
@@ -88,25 +91,133 @@ __device__ __forceinline__ void cov3d_of(const CovWords& w, float c[6]) {
   }
 }
 
-// Inclusive tile rectangle a splat can touch, packed (tx0 | ty0 << 16,
-// tx1 | ty1 << 16); (0xFFFFFFFF, 0) when it touches none. A pixel x is
-// covered only if |x + 0.5 - cx| <= hx, so [cx - hx - 1, cx + hx + 1] is a
+// Whether the ellipse |qo|^2 = (d.u)^2 + (d.v)^2 <= lim meets a box of
+// offsets d from its centre, for u = (ux, uy) and v = rho (-uy, ux) (a
+// Splat2D's axes). As a quadratic form, a dx^2 + 2 b dx dy + c dy^2 with
+// a = |(ux, vx)|^2, b = ux uy + vx vy, c = |(uy, vy)|^2, it is convex with
+// its minimum at the centre, so when the centre is outside the box the box's
+// minimum lies on a face that the segment from it to the centre leaves
+// through: the face nearer the centre in x and/or in y. On the face x = xe
+// the minimum over dy is at -b xe / c, clamped to the face (likewise in y);
+// the form is evaluated there as the sum of squares. A misplaced minimiser
+// (the hardware reciprocal) raises the value only by second-order terms.
+// The comparison carries a margin of 1e-4 of the largest value the squares'
+// terms reach over the box, far above the rounding of this evaluation and of
+// the blend's, so a fragment the blend keeps is never dropped. Degenerate or
+// non-finite axes answer true.
+struct EllipseTest {
+  float ux, uy, vx, vy;
+  float ba, bc;  // b / a, b / c
+  float lim;
+  bool ok;
+};
+
+__device__ __forceinline__ EllipseTest ellipse_test(float ux, float uy, float rho, float lim) {
+  EllipseTest e;
+  e.ux = ux;
+  e.uy = uy;
+  e.vx = -rho * uy;
+  e.vy = rho * ux;
+  const float a = ux * ux + e.vx * e.vx, b = ux * uy + e.vx * e.vy, c = uy * uy + e.vy * e.vy;
+  e.ok = a > 0.0f && c > 0.0f && a < __builtin_inff() && c < __builtin_inff();
+  e.ba = b * __builtin_amdgcn_rcpf(a);
+  e.bc = b * __builtin_amdgcn_rcpf(c);
+  e.lim = lim;
+  return e;
+}
+
+__device__ __forceinline__ bool ellipse_meets_box(const EllipseTest& e, float bx0, float bx1, float by0, float by1) {
+  const bool fx = bx0 > 0.0f || bx1 < 0.0f;
+  const bool fy = by0 > 0.0f || by1 < 0.0f;
+  if ((!fx && !fy) || !e.ok) return true;  // the centre is inside, or no test
+  const float xe = bx0 > 0.0f ? bx0 : bx1;
+  const float ye = by0 > 0.0f ? by0 : by1;
+  float m = __builtin_inff();
+  if (fx) {
+    const float dy = fminf(fmaxf(-e.bc * xe, by0), by1);
+    const float t1 = xe * e.ux + dy * e.uy, t2 = xe * e.vx + dy * e.vy;
+    m = t1 * t1 + t2 * t2;
+  }
+  if (fy) {
+    const float dx = fminf(fmaxf(-e.ba * ye, bx0), bx1);
+    const float t1 = dx * e.ux + ye * e.uy, t2 = dx * e.vx + ye * e.vy;
+    m = fminf(m, t1 * t1 + t2 * t2);
+  }
+  const float mx = fmaxf(fabsf(bx0), fabsf(bx1)), my = fmaxf(fabsf(by0), fabsf(by1));
+  const float e1 = fabsf(e.ux) * mx + fabsf(e.uy) * my, e2 = fabsf(e.vx) * mx + fabsf(e.vy) * my;
+  return m <= e.lim + 1e-4f * (e1 * e1 + e2 * e2);
+}
+
+// Inclusive tile rectangle a splat can touch and, for rectangles of at most
+// 4 x 4 tiles, which of those tiles its ellipse |qo|^2 <= sd2 reaches:
+//   x = tx0 | ty0 << 12 | mask bits 0-7 << 24
+//   y = tx1 | ty1 << 12 | mask bits 8-15 << 24
+// with bit 4 j + i of the mask for tile (tx0 + i, ty0 + j); point-mode
+// splats (rho = 0) set the bits of every tile of the rectangle. Larger
+// rectangles keep every tile (mask unused). (0xFFFFFFFF, 0) when it touches
+// none. Tile coordinates take 12
+// bits: textures are at most 16384 pixels wide and tiles at least 8. A pixel
+// x is covered only if |x + 0.5 - cx| <= hx, so [cx - hx - 1, cx + hx + 1] is a
 // conservative pixel range; hxy holds the f16 half extents rounded up (0:
 // draws no pixel).
-__device__ __forceinline__ uint2 tile_rect(float cx, float cy, uint32_t hxy, const TileGrid& g) {
+// The mask bits of every tile of a w x h rectangle (w, h <= 4).
+__device__ __forceinline__ uint32_t rect_bits(uint32_t w, uint32_t h) {
+  return ((1u << w) - 1u) * (0x1111u & ((1u << (4u * h)) - 1u));
+}
+
+__device__ __forceinline__ uint2 tile_rect(const Splat2D& s, float sd2, const TileGrid& g) {
   const uint2 none = make_uint2(0xFFFFFFFFu, 0u);
-  if (hxy == 0u) return none;
-  const float hx = __half2float(__ushort_as_half((unsigned short)(hxy & 0xFFFFu)));
-  const float hy = __half2float(__ushort_as_half((unsigned short)(hxy >> 16)));
-  const float fx0 = floorf(cx - hx - 1.0f), fx1 = floorf(cx + hx + 1.0f);
-  const float fy0 = floorf(cy - hy - 1.0f), fy1 = floorf(cy + hy + 1.0f);
+  if (s.hxy == 0u) return none;
+  const float hx = __half2float(__ushort_as_half((unsigned short)(s.hxy & 0xFFFFu)));
+  const float hy = __half2float(__ushort_as_half((unsigned short)(s.hxy >> 16)));
+  const float fx0 = floorf(s.cx - hx - 1.0f), fx1 = floorf(s.cx + hx + 1.0f);
+  const float fy0 = floorf(s.cy - hy - 1.0f), fy1 = floorf(s.cy + hy + 1.0f);
   const float wmax = float(g.width - 1), hmax = float(g.height - 1);
   if (!(fx1 >= 0.0f) || !(fx0 <= wmax) || !(fy1 >= 0.0f) || !(fy0 <= hmax)) return none;
   const uint32_t x0 = uint32_t(fmaxf(fx0, 0.0f)), x1 = uint32_t(fminf(fx1, wmax));
   const uint32_t y0 = uint32_t(fmaxf(fy0, 0.0f)), y1 = uint32_t(fminf(fy1, hmax));
   // tile sizes are powers of two (8x8, 16x16, 32x8): shifts, not divisions
   const uint32_t sx = uint32_t(__builtin_ctz(g.tile_w)), sy = uint32_t(__builtin_ctz(g.tile_h));
-  return make_uint2((x0 >> sx) | ((y0 >> sy) << 16), (x1 >> sx) | ((y1 >> sy) << 16));
+  const uint32_t tx0 = x0 >> sx, tx1 = x1 >> sx, ty0 = y0 >> sy, ty1 = y1 >> sy;
+  const bool small = tx1 - tx0 < 4u && ty1 - ty0 < 4u;
+  uint32_t mask = small ? rect_bits(tx1 - tx0 + 1u, ty1 - ty0 + 1u) : 0xFFFFu;
+  // (a rectangle of one tile keeps it: the bound rarely misses all of it)
+  if (s.rho > 0.0f && small && (tx0 != tx1 || ty0 != ty1)) {
+    const EllipseTest e = ellipse_test(s.ux, s.uy, s.rho, sd2 * 1.0001f);
+    mask = 0u;
+    for (uint32_t ty = ty0; ty <= ty1; ++ty) {
+      // pixel centres of the tile: [ty th + 0.5, (ty + 1) th - 0.5]
+      const float by0 = float(ty * g.tile_h) + 0.5f - s.cy, by1 = by0 + float(g.tile_h - 1u);
+      for (uint32_t tx = tx0; tx <= tx1; ++tx) {
+        const float bx0 = float(tx * g.tile_w) + 0.5f - s.cx, bx1 = bx0 + float(g.tile_w - 1u);
+        if (ellipse_meets_box(e, bx0, bx1, by0, by1)) mask |= 1u << (4u * (ty - ty0) + (tx - tx0));
+      }
+    }
+  }
+  return make_uint2(tx0 | (ty0 << 12) | ((mask & 0xFFu) << 24), tx1 | (ty1 << 12) | ((mask >> 8) << 24));
+}
+
+// The tiles of a packed rectangle (tile_rect): count and bounds.
+struct TileRect {
+  uint32_t tx0, ty0, tx1, ty1, mask;
+  bool small;  // at most 4 x 4 tiles: `mask` selects among them
+};
+
+__device__ __forceinline__ TileRect unpack_rect(uint2 r) {
+  TileRect t;
+  t.tx0 = r.x & 0xFFFu;
+  t.ty0 = (r.x >> 12) & 0xFFFu;
+  t.tx1 = r.y & 0xFFFu;
+  t.ty1 = (r.y >> 12) & 0xFFFu;
+  t.mask = (r.x >> 24) | ((r.y >> 24) << 8);
+  t.small = t.tx1 - t.tx0 < 4u && t.ty1 - t.ty0 < 4u;
+  return t;
+}
+
+__device__ __forceinline__ uint32_t rect_tiles(const TileRect& t) {
+  if (t.tx0 > t.tx1 || t.ty0 > t.ty1) return 0u;
+  if (t.small) return uint32_t(__popc(t.mask & rect_bits(t.tx1 - t.tx0 + 1u, t.ty1 - t.ty0 + 1u)));
+  return (t.tx1 - t.tx0 + 1u) * (t.ty1 - t.ty0 + 1u);
 }
 
 // SH colour of one instance (utils.wesl:82-136) for the splat record:
@@ -136,12 +247,12 @@ __device__ __forceinline__ void sh_color(const GaussianStreams& g, uint32_t i, f
     if (j < 45 && j / 3 < kmax) rgb[j % 3] += basis[j / 3] * v;
   };
   if constexpr (POD == kPodSingle) {
-    const float4* p = g.sh32 + size_t(i) * 12;
+    const float4* p = g.sh32 + i;
     const int nq = (3 * kmax + 3) / 4;
     float4 v[12];
 #pragma unroll
     for (int q = 0; q < 12; ++q)
-      if (q < nq) v[q] = p[q];
+      if (q < nq) v[q] = p[size_t(q) * g.n];
 #pragma unroll
     for (int q = 0; q < 12; ++q) {
       if (q >= nq) break;
@@ -151,12 +262,12 @@ __device__ __forceinline__ void sh_color(const GaussianStreams& g, uint32_t i, f
       add(4 * q + 3, v[q].w);
     }
   } else {
-    const uint4* p = g.sh16 + size_t(i) * 6;
+    const uint4* p = g.sh16 + i;
     const int nq = (3 * kmax + 7) / 8;
     uint4 v[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q)
-      if (q < nq) v[q] = p[q];
+      if (q < nq) v[q] = p[size_t(q) * g.n];
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
       if (q >= nq) break;
@@ -214,7 +325,7 @@ __device__ __forceinline__ void compute_splat(const FrameConsts& fc, const Gauss
   Splat2D s;
   s.cx = (proj[0] * iw + 1.0f) * 0.5f * fc.size[0];
   s.cy = (1.0f - proj[1] * iw) * 0.5f * fc.size[1];
-  s.ca = s.cb = s.cc = 0.0f;
+  s.ux = s.uy = s.rho = 0.0f;
   float hx = 0.0f, hy = 0.0f;
   bool drawable = false;
   // depth clipping of the rasterizer: 0 <= z_clip <= w_clip
@@ -227,8 +338,8 @@ __device__ __forceinline__ void compute_splat(const FrameConsts& fc, const Gauss
     const float q = 0.01f * fc.gsize;
     hx = q * aspect * ivl * 0.5f * fc.size[0];
     hy = q * ivl * 0.5f * fc.size[1];
-    s.ca = hx;
-    s.cb = hy;
+    s.ux = hx;
+    s.uy = hy;
     alpha = 1.0f;
     drawable = zclip_ok;
   } else {
@@ -277,12 +388,12 @@ __device__ __forceinline__ void compute_splat(const FrameConsts& fc, const Gauss
       ok = (maj > 0.0f) && (mnr > 0.0f);
       if (ok) {
         // qo = (d.u, d.v) with u = 2 (M.x, -M.y) / |M|^2, v = 2 (m.x, -m.y) / |m|^2,
-        // M = maj (dx, dy), m = mnr (dy, -dx); |qo|^2 as a quadratic form in d
-        const float im = 2.0f * __builtin_amdgcn_rcpf(maj), in = 2.0f * __builtin_amdgcn_rcpf(mnr);
-        const float ux = dx * im, uy = -dy * im, vx = dy * in, vy = dx * in;
-        s.ca = ux * ux + vx * vx;
-        s.cb = ux * uy + vx * vy;
-        s.cc = uy * uy + vy * vy;
+        // M = maj (dx, dy), m = mnr (dy, -dx): u = (2 / maj) (dx, -dy) and
+        // v = (2 / mnr) (dy, dx) = (maj / mnr) (-u.y, u.x)
+        const float im = 2.0f * __builtin_amdgcn_rcpf(maj);
+        s.ux = dx * im;
+        s.uy = -dy * im;
+        s.rho = maj / mnr;
         hx = 0.5f * sd * __builtin_amdgcn_sqrtf(maj * maj * dx * dx + mnr * mnr * dy * dy);
         hy = 0.5f * sd * __builtin_amdgcn_sqrtf(maj * maj * dy * dy + mnr * mnr * dx * dx);
       }
@@ -293,7 +404,7 @@ __device__ __forceinline__ void compute_splat(const FrameConsts& fc, const Gauss
   s.ba = pack_half2(rgb[2], alpha);
   s.hxy = drawable ? pack_half2_up(hx, hy) : 0u;
   *out = s;
-  if (rect_out) *rect_out = tile_rect(s.cx, s.cy, s.hxy, *rect_grid);
+  if (rect_out) *rect_out = tile_rect(s, fc.max_std_dev * fc.max_std_dev, *rect_grid);
 }
 
 }  // namespace gs3d

@@ -96,6 +96,7 @@ struct BinArgs {
   const uint32_t* indices;        // sorted Gaussian indices
   const Splat2D* splats;
   const uint2* rects;  // per-Gaussian tile rectangles on `grid` (null: from the splats)
+  const FrameConsts* fc;  // max_std_dev, for rectangles made from the splats
   TileGrid grid;
   uint32_t* pair_keys;
   uint32_t* pair_vals;
@@ -135,7 +136,7 @@ struct CompositeArgs {
   float* depth;
   int depth_cmp;
   int depth_write;
-  uint32_t* plan;      // {items, multi-segment items, work ticket}
+  uint32_t* plan;      // kPlanWords words: {items, multi-segment items}, the work ticket at kPlanTicketWord
   uint32_t* plan_counts;  // [composite_plan_blocks][2] per-block item counts
   float4* partial;     // [composite_max_heavy_items][tile pixels]
 };
@@ -149,10 +150,15 @@ void launch_bin(hipStream_t st, const BinArgs& a);
 // it; host_total: device alias of mapped host memory that receives that
 // maximum (or null)
 void launch_ranges(hipStream_t st, const uint32_t* pair_count, uint32_t capacity, const uint32_t* keys,
-                   uint32_t* ranges, int num_cus, uint32_t* pair_max, uint32_t* host_total);
+                   uint32_t* ranges, uint32_t num_tiles, int num_cus, uint32_t* pair_max, uint32_t* host_total,
+                   uint32_t* err);
 uint32_t composite_max_items(uint32_t num_tiles, uint32_t capacity, uint32_t seg_len);
 uint32_t composite_max_heavy_items(uint32_t capacity, uint32_t seg_len);
 uint32_t composite_plan_blocks(uint32_t num_tiles);
+// words of CompositeArgs::plan; the work ticket sits in the second 128-byte
+// line, apart from the counts every workgroup reads
+constexpr uint32_t kPlanWords = 64;
+constexpr uint32_t kPlanTicketWord = 32;
 // main_begin/main_end, when not null, are recorded around the main
 // compositing kernel (for per-kernel timing).
 void launch_composite(hipStream_t st, const CompositeArgs& a, int num_cus, hipEvent_t main_begin = nullptr,

@@ -68,29 +68,23 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(BinArgs a) {
   const uint32_t base = t * kBinTile;
 
   uint32_t cnt[kBinItems], excl[kBinItems], gidx[kBinItems];
-  uint32_t r0[kBinItems], r1[kBinItems];  // packed (tx0 | ty0 << 16), (tx1 | ty1 << 16)
+  uint2 rr[kBinItems];  // packed rectangles (tile_rect)
 #pragma unroll
   for (int k = 0; k < kBinItems; ++k) {
     const uint32_t s = base + uint32_t(k) * kBinThreads + tid;
     cnt[k] = 0u;
     gidx[k] = 0u;
-    r0[k] = r1[k] = 0u;
+    rr[k] = make_uint2(0xFFFFFFFFu, 0u);
     if (s < v) {
       const uint32_t i = a.indices[s];
       gidx[k] = i;
-      uint2 r;
       if (a.rects) {
-        r = a.rects[i];  // 8 bytes per instance written by the vertex work
+        rr[k] = a.rects[i];  // 8 bytes per instance written by the vertex work
       } else {
-        const Splat2D sp = a.splats[i];
-        r = tile_rect(sp.cx, sp.cy, sp.hxy, a.grid);
+        const float sd = a.fc->max_std_dev;
+        rr[k] = tile_rect(a.splats[i], sd * sd, a.grid);
       }
-      const uint32_t tx0 = r.x & 0xFFFFu, ty0 = r.x >> 16, tx1 = r.y & 0xFFFFu, ty1 = r.y >> 16;
-      if (tx0 <= tx1 && ty0 <= ty1) {
-        cnt[k] = (tx1 - tx0 + 1u) * (ty1 - ty0 + 1u);
-        r0[k] = r.x;
-        r1[k] = r.y;
-      }
+      cnt[k] = rect_tiles(unpack_rect(rr[k]));
     }
   }
   // exclusive scan in (item, thread) order == sorted instance order
@@ -124,15 +118,18 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(BinArgs a) {
   for (int k = 0; k < kBinItems; ++k) {
     if (cnt[k] == 0u) continue;
     uint32_t p = off0 + excl[k];
-    const uint32_t tx0 = r0[k] & 0xFFFFu, ty0 = r0[k] >> 16, tx1 = r1[k] & 0xFFFFu, ty1 = r1[k] >> 16;
-    for (uint32_t ty = ty0; ty <= ty1; ++ty)
-      for (uint32_t tx = tx0; tx <= tx1; ++tx, ++p)
+    const TileRect r = unpack_rect(rr[k]);
+    for (uint32_t ty = r.ty0; ty <= r.ty1; ++ty)
+      for (uint32_t tx = r.tx0; tx <= r.tx1; ++tx) {
+        if (r.small && !((r.mask >> (4u * (ty - r.ty0) + (tx - r.tx0))) & 1u)) continue;
         if (p < a.capacity) {
           const uint32_t key = ty * a.grid.tiles_x + tx;
           a.pair_keys[p] = key;
           a.pair_vals[p] = gidx[k];
           if (a.sort_hist) atomicAdd(&s_hist[key & 0xFFu], 1u);
         }
+        ++p;
+      }
   }
   if (a.sort_hist) {
     __syncthreads();
@@ -142,7 +139,8 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(BinArgs a) {
 
 __global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ pair_count, uint32_t capacity,
                                                 const uint32_t* __restrict__ keys, uint32_t* __restrict__ ranges,
-                                                uint32_t* pair_max, uint32_t* host_total) {
+                                                uint32_t num_tiles, uint32_t* pair_max, uint32_t* host_total,
+                                                uint32_t* err) {
   const uint32_t total = *pair_count;
   // the largest pair total since the host last looked, for it to check the
   // lists' capacity without a copy
@@ -153,6 +151,10 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ pai
   const uint32_t p = total < capacity ? total : capacity;
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < p; j += gridDim.x * blockDim.x) {
     const uint32_t k = keys[j];
+    if (k >= num_tiles) {  // never written by k_bin: report, do not index
+      atomicOr(err, kErrTileKey);
+      continue;
+    }
     if (j == 0 || keys[j - 1] != k) ranges[2 * k] = j;
     if (j == p - 1 || keys[j + 1] != k) ranges[2 * k + 1] = j + 1;
   }
@@ -175,13 +177,14 @@ void launch_bin(hipStream_t st, const BinArgs& a) {
 }
 
 void launch_ranges(hipStream_t st, const uint32_t* pair_count, uint32_t capacity, const uint32_t* keys,
-                   uint32_t* ranges, int num_cus, uint32_t* pair_max, uint32_t* host_total) {
+                   uint32_t* ranges, uint32_t num_tiles, int num_cus, uint32_t* pair_max, uint32_t* host_total,
+                   uint32_t* err) {
   uint32_t blocks = (capacity + 255u) / 256u;
   const uint32_t cap = uint32_t(num_cus) * 8u;
   if (blocks > cap) blocks = cap;
   if (blocks < 1u) blocks = 1u;
-  hipLaunchKernelGGL(k_ranges, dim3(blocks), dim3(256), 0, st, pair_count, capacity, keys, ranges, pair_max,
-                     host_total);
+  hipLaunchKernelGGL(k_ranges, dim3(blocks), dim3(256), 0, st, pair_count, capacity, keys, ranges, num_tiles,
+                     pair_max, host_total, err);
 }
 
 }  // namespace gs3d

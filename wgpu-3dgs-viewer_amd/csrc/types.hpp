@@ -109,15 +109,19 @@ struct FrameConsts {
 // Per visible Gaussian, the 2D splat the compositor draws (the pixel-space
 // restatement of what vert_main hands the rasterizer, render.wesl:75-130).
 // Indexed by Gaussian index; 32 bytes (one aligned 32-byte HBM sector).
-//   splat / ellipse: |qo|^2 = ca*dx^2 + 2*cb*dx*dy + cc*dy^2 for d = pixel
-//                    centre - (cx, cy), where qo is the interpolated
-//                    quad offset of render.wesl:116-121
-//   point          : ca, cb hold the half extents in pixels (f32)
+//   splat / ellipse: |qo|^2 = (d.u)^2 + (d.v)^2 for d = pixel centre -
+//                    (cx, cy), where qo is the interpolated quad offset of
+//                    render.wesl:116-121: u = (ux, uy) = 2 (M.x, -M.y) /
+//                    |M|^2 for the major axis M, and v = rho (-uy, ux) with
+//                    rho = |M| / |m| (v is the minor axis' term, u rotated
+//                    by 90 degrees and scaled). A sum of squares, as the
+//                    shader's, so it never goes negative for thin splats.
+//   point          : ux, uy hold the half extents in pixels (f32)
 // rg, ba: colour and opacity as f16 pairs; hxy: f16 pair of the half extents
 // of the pixel bound, rounded up, 0 when the splat draws no pixel.
 struct alignas(16) Splat2D {
   float cx, cy;
-  float ca, cb, cc;
+  float ux, uy, rho;
   uint32_t rg;
   uint32_t ba;
   uint32_t hxy;
@@ -133,6 +137,7 @@ constexpr uint32_t kValueMask = (1u << 30) - 1u;
 // Error bits the kernels raise in the per-viewer status word.
 constexpr uint32_t kErrLookbackSpin = 1u;
 constexpr uint32_t kErrTileOverflow = 2u;
+constexpr uint32_t kErrTileKey = 4u;  // a tile pair with a tile id outside the grid
 
 // Screen tiles of a render target.
 struct TileGrid {

@@ -66,6 +66,11 @@ void Device::note_render(Renderer* r) {
 
 void Device::forget_renderer(Renderer* r) { pending_.erase(std::remove(pending_.begin(), pending_.end(), r), pending_.end()); }
 
+void throw_kernel_errors(uint32_t e) {
+  if (e & kErrLookbackSpin) throw Error(GS3D_ERROR_KERNEL, "a kernel exceeded its look-back wait limit");
+  if (e & kErrTileKey) throw Error(GS3D_ERROR_KERNEL, "a tile pair held a tile id outside the grid");
+}
+
 void Device::poll() {
   make_current();
   for (int attempt = 0;; ++attempt) {
@@ -74,9 +79,9 @@ void Device::poll() {
     hip_check(hipMemcpy(&e, err_, 4, hipMemcpyDeviceToHost), "read error word");
     if (e == 0) break;
     hip_check(hipMemset(err_, 0, 4), "clear error word");
-    if (e & kErrLookbackSpin) {
+    if (e & (kErrLookbackSpin | kErrTileKey)) {
       pending_.clear();
-      throw Error(GS3D_ERROR_KERNEL, "a kernel exceeded its look-back wait limit");
+      throw_kernel_errors(e);
     }
     if ((e & kErrTileOverflow) && attempt < 4) {
       // an overflowed frame drew nothing; later draws of a shared pass
@@ -309,6 +314,18 @@ GaussiansBuffer::GaussiansBuffer(Device& dev, const gs3d_gaussian* g, uint32_t n
   const uint32_t chunk = 1u << 18;
   std::vector<float> pc, cov, sh32;
   std::vector<uint16_t> rs, sh16;
+  std::vector<uint8_t> plane;
+  // the SH stream is plane-major (16-byte chunk q of Gaussian i at q * n + i):
+  // records [b, b + m) of `planes` chunks each, Gaussian-major in `src`
+  auto upload_sh_planes = [&](const void* src, uint32_t m, uint32_t b, int planes) {
+    const uint8_t* s8 = static_cast<const uint8_t*>(src);
+    plane.resize(size_t(m) * 16 * size_t(planes));
+    for (int q = 0; q < planes; ++q)
+      for (uint32_t j = 0; j < m; ++j)
+        std::memcpy(&plane[(size_t(q) * m + j) * 16], s8 + (size_t(j) * planes + size_t(q)) * 16, 16);
+    for (int q = 0; q < planes; ++q)
+      sh_.upload(&plane[size_t(q) * m * 16], size_t(m) * 16, (size_t(q) * n + b) * 16);
+  };
   for (uint32_t b = 0; b < n; b += chunk) {
     const uint32_t m = std::min(chunk, n - b);
     pc.assign(size_t(m) * 4, 0.0f);
@@ -330,7 +347,7 @@ GaussiansBuffer::GaussiansBuffer(Device& dev, const gs3d_gaussian* g, uint32_t n
         std::memcpy(&sh32[48 * size_t(j)], &q.sh[0][0], 45 * sizeof(float));
       }
       cov_.upload(cov.data(), size_t(m) * 24, size_t(b) * 24);
-      sh_.upload(sh32.data(), size_t(m) * 192, size_t(b) * 192);
+      upload_sh_planes(sh32.data(), m, b, 12);
     } else {
       rs.assign(size_t(m) * 8, 0);
       sh16.assign(size_t(m) * 48, 0);
@@ -341,7 +358,7 @@ GaussiansBuffer::GaussiansBuffer(Device& dev, const gs3d_gaussian* g, uint32_t n
         for (int k = 0; k < 45; ++k) sh16[48 * size_t(j) + k] = float_to_half((&q.sh[0][0])[k]);
       }
       rs_.upload(rs.data(), size_t(m) * 16, size_t(b) * 16);
-      sh_.upload(sh16.data(), size_t(m) * 96, size_t(b) * 96);
+      upload_sh_planes(sh16.data(), m, b, 6);
     }
     // the staging vectors are reused: wait for the copies of this chunk
     dev.wait();
@@ -716,6 +733,7 @@ void Renderer::render(Texture& target, const IndirectArgsBuffer& indirect_args, 
   b.indirect_args = indirect_args.dev();
   b.indices = indices_.dev();
   b.splats = splats_.dev();
+  b.fc = fc;
   // rectangles written with this frame's splats for this grid (not for a
   // standalone draw, which recomputes the splats)
   b.rects = (!standalone && splats_.rects_on() && same_grid(splats_.rect_grid(), g)) ? splats_.rects() : nullptr;
@@ -738,7 +756,7 @@ void Renderer::render(Texture& target, const IndirectArgsBuffer& indirect_args, 
                                       pk_[1].as<uint32_t>(), pv_[1].as<uint32_t>(), passes, tile_scratch, nullptr, 0,
                                       0, true);
   launch_ranges(st, pair_count_.as<uint32_t>(), capacity_, pk_[which].as<uint32_t>(), ranges_.as<uint32_t>(),
-                dev_.num_cus(), pair_count_.as<uint32_t>() + 1, pair_total_dev_);
+                num_tiles, dev_.num_cus(), pair_count_.as<uint32_t>() + 1, pair_total_dev_, dev_.err());
   tm.mark(GS3D_STAGE_TILE_SORT, true);
 
   tm.mark(GS3D_STAGE_COMPOSITE, false);
@@ -754,7 +772,7 @@ void Renderer::render(Texture& target, const IndirectArgsBuffer& indirect_args, 
   grow(item_tile_, items * 4);
   grow(item_seg_, items * 4);
   grow(seg_base_, size_t(num_tiles) * 4);
-  grow(plan_, 16);
+  grow(plan_, kPlanWords * sizeof(uint32_t));
   grow(plan_counts_, size_t(composite_plan_blocks(num_tiles)) * 8);
   grow(partial_, heavy * tile_px * sizeof(float4));
   CompositeArgs ca{};

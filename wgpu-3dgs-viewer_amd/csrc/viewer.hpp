@@ -30,6 +30,10 @@ void hip_check(hipError_t e, const char* what);
 class Renderer;
 
 // wgpu::Device + wgpu::Queue: one HIP device and one stream.
+// Raises the error a kernel reported through the device's error word (the
+// look-back wait limit, a tile id outside the grid); overflow is not one.
+void throw_kernel_errors(uint32_t e);
+
 class Device {
  public:
   explicit Device(int ordinal);
