@@ -29,6 +29,10 @@ constexpr int kRsWaves = kRsThreads / kWave;
 #define GS3D_SORT_ITEMS 16
 #endif
 constexpr int kRsItems = GS3D_SORT_ITEMS;
+// resident onesweep workgroups per CU (the persistent grid)
+#ifndef GS3D_SORT_WG_PER_CU
+#define GS3D_SORT_WG_PER_CU 4
+#endif
 constexpr uint32_t kRsTile = kRsThreads * kRsItems;
 
 __device__ __forceinline__ uint32_t sort_n(const SortCount& c) {
@@ -267,7 +271,7 @@ int launch_radix_sort(hipStream_t st, int num_cus, SortCount cnt, uint32_t* k0, 
     hipLaunchKernelGGL(k_rs_upsweep, dim3(up_blocks), dim3(256), 0, st, cnt, k0, sc.hist);
   }
   uint32_t grid = sc.max_tiles;
-  if (grid > uint32_t(num_cus) * 4u) grid = uint32_t(num_cus) * 4u;
+  if (grid > uint32_t(num_cus) * GS3D_SORT_WG_PER_CU) grid = uint32_t(num_cus) * GS3D_SORT_WG_PER_CU;
   uint32_t *ks = k0, *vs = v0, *kd = k1, *vd = v1;
   for (int p = 0; p < passes; ++p) {
     uint32_t* next = (p + 1 < passes) ? sc.hist + (p + 1) * 256 : nullptr;
