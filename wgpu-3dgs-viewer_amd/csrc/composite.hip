@@ -69,9 +69,11 @@ struct StageT {
   static constexpr int kWaves = NT / kWave;
   static constexpr int kQuads = 4 * kWaves;
   static constexpr uint32_t kDummy = NT;
-  // splat / ellipse: |qo|^2 log2(e) = (d.u')^2 + (d.v')^2, u' = u sqrt(log2(e)), v' likewise
-  float4 geo[NT + 1];  // splat: cx, cy, u'.x, u'.y   point: cx, cy, hx, hy
-  float4 col[NT + 1];  // splat: v'.x, v'.y, opacity (fast splat: log2 opacity), r   point: 0, 0, opacity, r
+  // splat / ellipse: |qo|^2 log2(e) = (p.u' - k1)^2 + (p.v' - k2)^2 for the
+  // pixel centre p relative to the tile's 32 x 16 cell, u' = u sqrt(log2(e)),
+  // v' likewise, k1 = c.u', k2 = c.v' for the splat's centre c (cell-relative)
+  float4 geo[NT + 1];  // splat: u'.x, u'.y, v'.x, v'.y   point: c.x, c.y, hx, hy
+  float4 col[NT + 1];  // splat: k1, k2, opacity (fast splat: log2 opacity), r   point: 0, 0, opacity, r
   float2 gb[NT + 1];   // g, b
   float z[Z ? NT + 1 : 1];  // fragment depth, for depth-tested draws (Z)
   uint16_t list[kQuads][NT + kListPad];
@@ -99,6 +101,10 @@ struct Tiling {
   float fpx, fpy;
   uint32_t waves, blocks_x;
   float ox, oy;  // pixel origin of the tile
+  // origin of the 32 x 16 cell holding the tile (every tile shape is a
+  // sub-tile of one): splat centres and pixel centres are taken relative to
+  // it, so a pixel's arithmetic does not depend on the tile shape
+  float sx, sy;
 };
 
 __device__ __forceinline__ Tiling setup_tile(const TileGrid& g, uint32_t tile) {
@@ -112,16 +118,19 @@ __device__ __forceinline__ Tiling setup_tile(const TileGrid& g, uint32_t tile) {
   t.px = tx * g.tile_w + (w % t.blocks_x) * 8 + (lane & 7u);
   t.py = ty * g.tile_h + (w / t.blocks_x) * 8 + (lane >> 3);
   t.inside = t.px < g.width && t.py < g.height;
-  t.fpx = float(t.px) + 0.5f;
-  t.fpy = float(t.py) + 0.5f;
+  const uint32_t cx0 = (tx * g.tile_w) & ~31u, cy0 = (ty * g.tile_h) & ~15u;
+  t.sx = float(cx0);
+  t.sy = float(cy0);
+  t.fpx = float(t.px - cx0) + 0.5f;
+  t.fpy = float(t.py - cy0) + 0.5f;
   return t;
 }
 
 // Stage one splat record into slot tid. `log2op`: store log2(opacity) (the
 // fast path's splat mode) instead of the opacity.
 template <int NT, bool Z>
-__device__ __forceinline__ void stage_splat(StageT<NT, Z>& s, uint32_t tid, const Splat2D& r, bool clamp_src, int mode,
-                                            bool log2op) {
+__device__ __forceinline__ void stage_splat(StageT<NT, Z>& s, uint32_t tid, const Splat2D& r, const Tiling& t,
+                                            bool clamp_src, int mode, bool log2op) {
   float cr = h_lo(r.rg), cg = h_hi(r.rg), cb = h_lo(r.ba);
   const float op = h_hi(r.ba);
   if (clamp_src) {
@@ -129,14 +138,16 @@ __device__ __forceinline__ void stage_splat(StageT<NT, Z>& s, uint32_t tid, cons
     cg = clamp01(cg);
     cb = clamp01(cb);
   }
+  const float cx = r.cx - t.sx, cy = r.cy - t.sy;
   if (mode == kModePoint) {
-    s.geo[tid] = make_float4(r.cx, r.cy, r.ux, r.uy);
+    s.geo[tid] = make_float4(cx, cy, r.ux, r.uy);
     s.col[tid] = make_float4(0.0f, 0.0f, op, cr);
   } else {
     const float ux = r.ux * kSqrtLog2e, uy = r.uy * kSqrtLog2e;
-    s.geo[tid] = make_float4(r.cx, r.cy, ux, uy);
+    const float vx = -r.rho * uy, vy = r.rho * ux;
+    s.geo[tid] = make_float4(ux, uy, vx, vy);
     // log2(0) = -inf: exp2 gives alpha 0
-    s.col[tid] = make_float4(-r.rho * uy, r.rho * ux, log2op ? __log2f(op) : op, cr);
+    s.col[tid] = make_float4(cx * ux + cy * uy, cx * vx + cy * vy, log2op ? __log2f(op) : op, cr);
   }
   s.gb[tid] = make_float2(cg, cb);
 }
@@ -149,7 +160,7 @@ __device__ __forceinline__ void stage_batch(Stage& s, const Tiling& t, const Spl
   uint32_t mask = 0u;
   if (valid) {
     const Splat2D r = splats[k];
-    stage_splat(s, tid, r, clamp_src, mode, false);
+    stage_splat(s, tid, r, t, clamp_src, mode, false);
     if (zsrc) s.z[tid] = zsrc[k];
     // which 8x8 blocks of the tile the bound [c - h, c + h] can touch (pixel
     // centres of block (i, j) span [o + 8i + 0.5, o + 8i + 7.5]; half a pixel
@@ -195,7 +206,7 @@ __device__ __forceinline__ void stage_batch_quads(StageT<NT, Z>& s, const Tiling
   uint32_t mask = 0u;
   if (valid) {
     const Splat2D r = splats[k];
-    stage_splat(s, tid, r, clamp_src, mode, mode == kModeSplat);
+    stage_splat(s, tid, r, t, clamp_src, mode, mode == kModeSplat);
     if (Z && zsrc) s.z[tid] = zsrc[k];
     // Quadrant column c of the tile holds pixel centres [4c + 0.5, 4c + 3.5]
     // (tile-relative); the bound [x0, x1] may touch it when x1 >= 4c and
@@ -245,11 +256,13 @@ __device__ __forceinline__ float fragment(const Stage& s, uint32_t q, const Tili
                                           float ell2l, float4& col) {
   const float4 g = s.geo[q];
   col = s.col[q];
-  const float dx = t.fpx - g.x;
-  const float dy = t.fpy - g.y;
-  if (mode == kModePoint) return (dx >= -g.z && dx < g.z && dy >= -g.w && dy < g.w) ? 1.0f : -1.0f;
+  if (mode == kModePoint) {
+    const float dx = t.fpx - g.x;
+    const float dy = t.fpy - g.y;
+    return (dx >= -g.z && dx < g.z && dy >= -g.w && dy < g.w) ? 1.0f : -1.0f;
+  }
   // |qo|^2 log2(e)
-  const float t1 = dx * g.z + dy * g.w, t2 = dx * col.x + dy * col.y;
+  const float t1 = (t.fpx * g.x + t.fpy * g.y) - col.x, t2 = (t.fpx * g.z + t.fpy * g.w) - col.y;
   const float r2 = t1 * t1 + t2 * t2;
   if (!(r2 <= sd2l)) return -1.0f;
   if (mode == kModeEllipse) return col.z + (1.0f - col.z) * ((r2 > ell2l) ? 1.0f : 0.0f);
@@ -277,14 +290,15 @@ template <int MODE>
 __device__ __forceinline__ float fragment_alpha(float4 g, float4 c, const Tiling& t, float sd2l, float ell2l,
                                                 bool& kept) {
 #pragma clang fp contract(fast)
-  const float dx = t.fpx - g.x;
-  const float dy = t.fpy - g.y;
   if (MODE == kModePoint) {
+    const float dx = t.fpx - g.x;
+    const float dy = t.fpy - g.y;
     kept = dx >= -g.z && dx < g.z && dy >= -g.w && dy < g.w;
     return kept ? c.z : 0.0f;
   }
-  const float t1 = dx * g.z + dy * g.w, t2 = dx * c.x + dy * c.y;
-  const float r2 = t1 * t1 + t2 * t2;
+  const float t1 = __builtin_fmaf(t.fpx, g.x, __builtin_fmaf(t.fpy, g.y, -c.x));
+  const float t2 = __builtin_fmaf(t.fpx, g.z, __builtin_fmaf(t.fpy, g.w, -c.y));
+  const float r2 = __builtin_fmaf(t1, t1, t2 * t2);
   float al;
   if (MODE == kModeEllipse)
     al = c.z + (1.0f - c.z) * ((r2 > ell2l) ? 1.0f : 0.0f);
@@ -318,7 +332,7 @@ static_assert(kListPad % kUnroll == 0, "a step reads at most kListPad entries pa
 // blending (their colour only gets closer to the exact sum) until the whole
 // wave is done.
 template <int MODE, bool DEPTH, int NT>
-__device__ __forceinline__ void blend_quads(const StageT<NT, DEPTH>& s, const Tiling& t, uint32_t w, float sd2l, float ell2l,
+__device__ __forceinline__ void blend_quads(StageT<NT, DEPTH>& s, const Tiling& t, uint32_t w, float sd2l, float ell2l,
                                             float& T, float& cr, float& cg, float& cb, bool& done, DepthLane& dl) {
 #pragma clang fp contract(fast)
   const uint32_t lane = threadIdx.x % kWave;
@@ -333,6 +347,17 @@ __device__ __forceinline__ void blend_quads(const StageT<NT, DEPTH>& s, const Ti
   const uint32_t wmax =
       __builtin_amdgcn_readfirstlane(max(max(lens[0], lens[1]), max(lens[2], lens[3])));
   constexpr uint32_t kDummyOff = StageT<NT, DEPTH>::kDummy * 16u;
+  // pad this wave's four lists with the dummy up to the longest one (whole
+  // steps), so that every step reads list entries without a bounds check;
+  // only this wave reads its lists, and its LDS accesses run in order
+  {
+    const uint32_t wend = (wmax + uint32_t(kUnroll) - 1u) & ~uint32_t(kUnroll - 1);
+    const uint32_t qlane = (lane & 3u) | (((lane >> 3) & 3u) << 2);
+    for (uint32_t e = mylen + qlane; e < wend; e += 16u) s.list[myq][e] = uint16_t(kDummyOff);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
   const char* geo = reinterpret_cast<const char*>(s.geo);
   const char* col = reinterpret_cast<const char*>(s.col);
   const char* gbs = reinterpret_cast<const char*>(s.gb);
@@ -341,13 +366,13 @@ __device__ __forceinline__ void blend_quads(const StageT<NT, DEPTH>& s, const Ti
     if ((j & 31u) == 0u && __all(done)) break;
     uint32_t q[kUnroll];  // byte offsets of the slots in geo / col
     if (kUnroll == 1) {
-      q[0] = j < mylen ? uint32_t(s.list[myq][j]) : kDummyOff;
+      q[0] = s.list[myq][j];
     } else {
 #pragma unroll
       for (int k = 0; k < kUnroll; k += 2) {
         const uint32_t qq = *reinterpret_cast<const uint32_t*>(&s.list[myq][j + k]);
-        q[k] = (j + k < mylen) ? (qq & 0xFFFFu) : kDummyOff;
-        q[k + 1 < kUnroll ? k + 1 : 0] = (j + k + 1 < mylen) ? (qq >> 16) : kDummyOff;
+        q[k] = qq & 0xFFFFu;
+        q[k + 1 < kUnroll ? k + 1 : 0] = qq >> 16;
       }
     }
     float4 g[kUnroll], c[kUnroll];
