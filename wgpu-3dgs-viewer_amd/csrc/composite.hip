@@ -223,22 +223,27 @@ __device__ __forceinline__ void stage_batch_quads(StageT<NT, Z>& s, const Tiling
     for (int rr = r0; rr <= r1; ++rr) mask |= colmask << (4 * int(t.blocks_x) * (rr >> 1) + 2 * (rr & 1));
   }
   constexpr int kQuads = StageT<NT, Z>::kQuads;
-  uint64_t bal[kQuads];
 #pragma unroll
-  for (int q = 0; q < kQuads; ++q) bal[q] = __ballot((mask >> q) & 1u);
-  if (lane == 0) {
-#pragma unroll
-    for (int q = 0; q < kQuads; ++q) s.qbal[sw][q] = bal[q];
+  for (int q = 0; q < kQuads; ++q) {
+    const uint64_t bal = __ballot((mask >> q) & 1u);
+    if (lane == 0) s.qbal[sw][q] = bal;
   }
   __syncthreads();
-  // each lane appends its slot to the lists of its quadrants (usually one or
-  // two): offset = the earlier staging waves' counts + rank in this wave
-  for (uint32_t m = mask; m != 0u; m &= m - 1u) {
-    const uint32_t q = uint32_t(__builtin_ctz(m));
-    uint32_t off = 0;
-    for (uint32_t p = 0; p < sw; ++p) off += uint32_t(__popcll(s.qbal[p][q]));
+  // this wave's first entry in each list: the earlier staging waves' counts
+  // (lane q < kQuads holds list q's)
+  uint32_t first = 0;
+  if (lane < uint32_t(kQuads))
+    for (uint32_t p = 0; p < sw; ++p) first += uint32_t(__popcll(s.qbal[p][lane]));
+  // every lane appends its slot to the lists of its quadrants at first +
+  // its rank in this wave's ballot; one uniform step per list (a per-lane
+  // walk over its set bits would run as long as the wave's largest splat)
+#pragma unroll
+  for (int q = 0; q < kQuads; ++q) {
+    const uint32_t off = __builtin_amdgcn_readlane(first, q);
     // quadrant lists hold the slot's byte offset in geo / col (16 per slot)
-    s.list[q][off + mask_rank(s.qbal[sw][q])] = uint16_t(tid * 16u);
+    // (the ballot again, rather than kept live across the barrier)
+    const uint64_t bal = __ballot((mask >> q) & 1u);
+    if ((mask >> q) & 1u) s.list[q][off + mask_rank(bal)] = uint16_t(tid * 16u);
   }
   __syncthreads();
 }
