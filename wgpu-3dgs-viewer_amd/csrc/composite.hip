@@ -193,8 +193,8 @@ __device__ __forceinline__ void stage_batch(Stage& s, const Tiling& t, const Spl
 }
 
 // Stage splat k (or nothing) into slot threadIdx.x and build one ordered list
-// per 4x4 quadrant of every wave's 8x8 block: quadrant (qx, qy) of wave w is
-// list 4w + qx + 2qy. (Testing each quadrant against the splat's ellipse,
+// per 4x4 quadrant of the tile (row-major over the tile's quadrants; each
+// wave blends the four of its 8x8 block). (Testing each quadrant against the splat's ellipse,
 // as the binning does per tile, shortens the lists but costs more in staging
 // than it saves in blending: measured.) Contains __syncthreads; every lane
 // must call it.
@@ -211,16 +211,17 @@ __device__ __forceinline__ void stage_batch_quads(StageT<NT, Z>& s, const Tiling
     // Quadrant column c of the tile holds pixel centres [4c + 0.5, 4c + 3.5]
     // (tile-relative); the bound [x0, x1] may touch it when x1 >= 4c and
     // x0 <= 4c + 4 (half a pixel of margin). Same for rows. Quadrant (c, r)
-    // is list 4w + (c & 1) + 2 (r & 1) of wave w = c / 2 + blocks_x (r / 2).
+    // is list r ncols + c (row-major over the tile's quadrants).
     const float hx = h_lo(r.hxy), hy = h_hi(r.hxy);
     const float x0 = r.cx - hx - t.ox, x1 = r.cx + hx - t.ox;
     const float y0 = r.cy - hy - t.oy, y1 = r.cy + hy - t.oy;
     const int ncols = int(t.blocks_x) * 2, nrows = int(t.waves / t.blocks_x) * 2;
     const int c0 = max(0, int(ceilf((x0 - 4.0f) * 0.25f))), c1 = min(ncols - 1, int(floorf(x1 * 0.25f)));
     const int r0 = max(0, int(ceilf((y0 - 4.0f) * 0.25f))), r1 = min(nrows - 1, int(floorf(y1 * 0.25f)));
-    uint32_t colmask = 0u;
-    for (int c = c0; c <= c1; ++c) colmask |= 1u << (4 * (c >> 1) + (c & 1));
-    for (int rr = r0; rr <= r1; ++rr) mask |= colmask << (4 * int(t.blocks_x) * (rr >> 1) + 2 * (rr & 1));
+    if (c0 <= c1 && r0 <= r1) {
+      const uint32_t colbits = (2u << c1) - (1u << c0);  // bits c0 .. c1
+      for (int rr = r0; rr <= r1; ++rr) mask |= colbits << (rr * ncols);
+    }
   }
   constexpr int kQuads = StageT<NT, Z>::kQuads;
 #pragma unroll
@@ -341,13 +342,17 @@ __device__ __forceinline__ void blend_quads(StageT<NT, DEPTH>& s, const Tiling& 
                                             float& T, float& cr, float& cg, float& cb, bool& done, DepthLane& dl) {
 #pragma clang fp contract(fast)
   const uint32_t lane = threadIdx.x % kWave;
-  const uint32_t myq = 4u * w + (((lane >> 2) & 1u) | ((lane >> 5) << 1));
+  // this wave's 8x8 block holds quadrant columns 2 bx, 2 bx + 1 and rows
+  // 2 by, 2 by + 1; the lane's quadrant is (qx, qy) within it
+  const uint32_t ncols = 2u * t.blocks_x, bx = w % t.blocks_x, by = w / t.blocks_x;
+  const uint32_t q00 = 2u * by * ncols + 2u * bx;
+  const uint32_t myq = q00 + ((lane >> 5) ? ncols : 0u) + ((lane >> 2) & 1u);
   uint32_t lens[4] = {0u, 0u, 0u, 0u};
   for (uint32_t p = 0; p < t.waves; ++p) {
 #pragma unroll
-    for (uint32_t qd = 0; qd < 4; ++qd) lens[qd] += uint32_t(__popcll(s.qbal[p][4 * w + qd]));
+    for (uint32_t qd = 0; qd < 4; ++qd) lens[qd] += uint32_t(__popcll(s.qbal[p][q00 + (qd >> 1) * ncols + (qd & 1u)]));
   }
-  const uint32_t mylen = lens[myq & 3u];
+  const uint32_t mylen = lens[((lane >> 2) & 1u) | ((lane >> 5) << 1)];
   // the same for the whole wave: keep the loop bound in a scalar register
   const uint32_t wmax =
       __builtin_amdgcn_readfirstlane(max(max(lens[0], lens[1]), max(lens[2], lens[3])));
