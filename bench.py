@@ -51,15 +51,15 @@ R_OUT = 40
 # Depth sort: 4 passes x (read key+value 8, write key+value 8) per key, plus
 # the one histogram read of the keys (4).
 SORT_BYTES_PER_KEY = 68
-# Compositing: floating-point operations per blended fragment (offset 2,
-# quadratic form 5, exp2 1, alpha 2, colour+alpha blend 8, transmittance 2).
+# Compositing: floating-point operations per blended fragment (r^2 from the
+# two axis terms 9, exp2 1, alpha 1, weight 1, colour 6, transmittance 2).
 FLOPS_PER_FRAGMENT = 20
 # HBM bytes per launch of k_composite_fast on this workload: rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE, each scaled by what the same counter reads for a
 # 1 GiB copy (x2.000 and x1.000); profiles/traffic_clustered-6M-1440p-f16_plain.md.
 # Mostly the 32-byte splat records gathered per (tile, splat) pair, one
 # 128-byte line each.
-COMPOSITE_TRAFFIC_BYTES = 989098672
+COMPOSITE_TRAFFIC_BYTES = 988567291
 
 MIX_RES = [(640, 360), (1280, 720), (1920, 1080), (3840, 2160)]
 
