@@ -38,7 +38,9 @@ typedef enum gs3d_status {
     GS3D_ERROR_DEVICE = 4,
     /* MultiModelViewerAccessError::ModelNotFound (src/error.rs:47-50) */
     GS3D_ERROR_MODEL_NOT_FOUND = 5,
-    /* a kernel reported an internal error (look-back wait limit) */
+    /* a kernel reported an internal error (look-back wait limit exceeded, or
+     * a tile pair with a tile id outside the grid); no counterpart in the
+     * reference, whose recording is infallible */
     GS3D_ERROR_KERNEL = 6
 } gs3d_status;
 
