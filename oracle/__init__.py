@@ -79,6 +79,8 @@ def _declare(L):
     L.orc_perspective_rh.argtypes = [C.c_float, C.c_float, C.c_float, C.c_float, fp]
     L.orc_cov3d_f32.argtypes = [fp, fp, fp]
     L.orc_cov3d_f64.argtypes = [dp, dp, dp]
+    L.orc_cov3d_f32_n.argtypes = [C.c_uint64, fp, fp, fp]
+    L.orc_cov3d_f64_n.argtypes = [C.c_uint64, dp, dp, dp]
     L.orc_model_mats.argtypes = [fp, fp, fp, fp, fp, fp]
     L.orc_preprocess_f32.argtypes = [F, C.c_uint32, fp, fp, u32p, fp]
     L.orc_preprocess_f32.restype = C.c_uint32
@@ -172,18 +174,16 @@ def cov3d_f32(scale, rot):
     """(n,3) scale, (n,4) xyzw rotation -> (n,6) cov3d in f32."""
     scale, rot = _f32(scale, (-1, 3)), _f32(rot, (-1, 4))
     out = np.zeros((scale.shape[0], 6), np.float32)
-    L = lib()
-    for i in range(scale.shape[0]):
-        L.orc_cov3d_f32(_p(scale[i], C.c_float), _p(rot[i], C.c_float), _p(out[i], C.c_float))
+    if scale.shape[0]:
+        lib().orc_cov3d_f32_n(scale.shape[0], _p(scale, C.c_float), _p(rot, C.c_float), _p(out, C.c_float))
     return out
 
 
 def cov3d_f64(scale, rot):
     scale, rot = _f64(scale, (-1, 3)), _f64(rot, (-1, 4))
     out = np.zeros((scale.shape[0], 6), np.float64)
-    L = lib()
-    for i in range(scale.shape[0]):
-        L.orc_cov3d_f64(_p(scale[i], C.c_double), _p(rot[i], C.c_double), _p(out[i], C.c_double))
+    if scale.shape[0]:
+        lib().orc_cov3d_f64_n(scale.shape[0], _p(scale, C.c_double), _p(rot, C.c_double), _p(out, C.c_double))
     return out
 
 

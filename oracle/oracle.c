@@ -124,6 +124,12 @@ void orc_perspective_rh(float fov_y, float aspect, float z_near, float z_far, fl
 
 void orc_cov3d_f32(const float scale[3], const float rot[4], float out[6]) { COV3D(float, scale, rot, out); }
 void orc_cov3d_f64(const double scale[3], const double rot[4], double out[6]) { COV3D(double, scale, rot, out); }
+void orc_cov3d_f32_n(uint64_t n, const float* scale, const float* rot, float* out) {
+    for (uint64_t i = 0; i < n; ++i) orc_cov3d_f32(scale + 3 * i, rot + 4 * i, out + 6 * i);
+}
+void orc_cov3d_f64_n(uint64_t n, const double* scale, const double* rot, double* out) {
+    for (uint64_t i = 0; i < n; ++i) orc_cov3d_f64(scale + 3 * i, rot + 4 * i, out + 6 * i);
+}
 
 void orc_model_mats(const float pos[3], const float rot[4], const float scale[3], float model[16],
                     float sr[9], float inv_sr[9]) {

@@ -113,6 +113,9 @@ void orc_perspective_rh(float fov_y, float aspect, float z_near, float z_far, fl
 /* ---- core restatements (wgpu-3dgs-core 0.6.0, absent: parity unpinned) ---- */
 void orc_cov3d_f32(const float scale[3], const float rot_xyzw[4], float out[6]);
 void orc_cov3d_f64(const double scale[3], const double rot_xyzw[4], double out[6]);
+/* the same over n records (scale n x 3, rot n x 4, out n x 6) */
+void orc_cov3d_f32_n(uint64_t n, const float* scale, const float* rot_xyzw, float* out);
+void orc_cov3d_f64_n(uint64_t n, const double* scale, const double* rot_xyzw, double* out);
 void orc_model_mats(const float pos[3], const float rot_xyzw[4], const float scale[3],
                     float model[16], float sr[9], float inv_sr[9]);
 

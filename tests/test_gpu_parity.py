@@ -11,7 +11,7 @@ import math
 import numpy as np
 import pytest
 
-from helpers import CONFIG0, CONFIG1, camera_pod, max_abs_diff, oracle_inputs, oracle_inputs_half, psnr
+from helpers import CONFIG0, CONFIG1, CONFIG2, camera_pod, max_abs_diff, oracle_inputs, oracle_inputs_half, psnr
 
 pytestmark = pytest.mark.gpu
 
@@ -225,6 +225,37 @@ def test_config1_full_frame(gs, orc, device):
     img = target.read()
     d, p = max_abs_diff(img, oimg), psnr(img, oimg)
     print(f"config1: V={v} max diff {d}/255 PSNR {p:.2f} dB", flush=True)
+    assert d <= 2 and p >= 45.0
+
+
+def test_config2_full_frame_half(gs, orc, device):
+    """The benchmark workload itself: 6M clustered Gaussians in the f16
+    storage form at 2560x1440, one frame against the oracle fed the same
+    f16-rounded inputs (visible set, depth keys and order bit-exact; image
+    within 2/255 and 45 dB)."""
+    config, n, seed, w, h = CONFIG2
+    _case(f"config2 {n} {w}x{h} f16")
+    g = gs.synth_gaussians(config, n, seed)
+    pod = camera_pod(gs, config, w, h, azimuth=0.0)
+    viewer = gs.Viewer(device, gs.TextureFormat.RGBA8_UNORM, g, gs.GaussianPod.HALF)
+    viewer.update_camera_with_pod(pod)
+    target = gs.Texture(device, w, h, gs.TextureFormat.RGBA8_UNORM)
+    viewer.render(target)
+    device.poll()
+    img = target.read()
+    v = int(viewer.indirect_args()[1])
+    idx = viewer.indirect_indices(v)
+    keys = viewer.gaussians_depth(v).view(np.uint32)
+    viewer.close()
+    frame = orc.frame_from_pod(pod.to_bytes())
+    pos, color, sh, cov = oracle_inputs_half(orc, g)
+    del g
+    oidx, okey, oimg = orc.full_frame(frame, pos, color, sh, cov, w, h, orc.TARGET_RGBA8_F32, threads=16)
+    assert v == len(oidx)
+    assert np.array_equal(idx, oidx)
+    assert np.array_equal(keys, okey.view(np.uint32))
+    d, p = max_abs_diff(img, oimg), psnr(img, oimg)
+    print(f"config2: V={v} max diff {d}/255 PSNR {p:.2f} dB", flush=True)
     assert d <= 2 and p >= 45.0
 
 
