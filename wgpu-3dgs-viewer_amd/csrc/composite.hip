@@ -6,12 +6,10 @@
 // 16x16 or 32x8 pixels); each wave owns an 8x8 block of the tile. The tile's
 // splats (its run of the tile-sorted pairs, in draw order) are staged through
 // LDS in batches of one splat per lane. While staging, every splat's pixel
-// bound is tested against the 4x4 quadrants of each wave's 8x8 block (fast)
-// or against the 8x8 blocks (exact), and each quadrant (block) receives, by
-// ballot compaction, the ordered list of the staged splats that can touch it.
-// Most splats of dense scenes cover a few pixels: a wave steps only through
-// the longest of its four quadrant lists, each 16-lane quadrant blending its
-// own list.
+// bound is tested against the tile's 4x4 quadrants, and each quadrant
+// receives, by ballot compaction, the ordered list of the staged splats that
+// can touch it. Most splats of dense scenes cover a few pixels: each 16-lane
+// quadrant blends only its own list.
 //
 // fast  : front to back in f32 with transmittance. The over operator is
 //         associative, so a tile whose list is longer than one segment is
@@ -51,18 +49,16 @@ __device__ __forceinline__ float clamp01(float x) { return fminf(fmaxf(x, 0.0f),
 __device__ __forceinline__ float h_lo(uint32_t w) { return __half2float(__ushort_as_half((unsigned short)(w & 0xFFFFu))); }
 __device__ __forceinline__ float h_hi(uint32_t w) { return __half2float(__ushort_as_half((unsigned short)(w >> 16))); }
 
-// One batch of staged splats plus, per wave, the ordered slots that touch
-// the wave's 8x8 block.
+// One batch of staged splats plus, per 4x4 quadrant of the tile, the ordered
+// slots that touch it.
 // Slot NT (one past the staged splats) holds a splat of opacity 0 (blending
 // it changes nothing): lanes whose list is shorter than the wave's longest
 // blend it. List rows carry kListPad spare entries so that a step may read
 // past a list's end.
 constexpr uint32_t kListPad = 4;
 
-// LDS of a workgroup of NT lanes (one per pixel of its tile). The fast path
-// lists splats per 4x4 quadrant of each wave's 8x8 block (4 NT / 64 lists);
-// the exact path per 8x8 block (lists 0 .. NT / 64 - 1). Sized per tile
-// shape, so that 8x8 tiles (one wave) take a quarter of a 256-lane tile's
+// LDS of a workgroup of NT lanes (one per pixel of its tile), with one list
+// per 4x4 quadrant (NT / 16 lists). Sized per tile shape, so that 8x8 tiles (one wave) take a quarter of a 256-lane tile's
 // LDS and as many workgroups fit a CU.
 template <int NT, bool Z = true>
 struct StageT {
@@ -77,7 +73,6 @@ struct StageT {
   float2 gb[NT + 1];   // g, b
   float z[Z ? NT + 1 : 1];  // fragment depth, for depth-tested draws (Z)
   uint16_t list[kQuads][NT + kListPad];
-  uint32_t wcount[kWaves][kWaves];  // exact path: [staging wave][target wave]
   uint64_t qbal[kWaves][kQuads];    // fast path: [staging wave][quadrant] ballots
 };
 using Stage = StageT<kCompMaxThreads>;
@@ -152,46 +147,6 @@ __device__ __forceinline__ void stage_splat(StageT<NT, Z>& s, uint32_t tid, cons
   s.gb[tid] = make_float2(cg, cb);
 }
 
-// Stage splat k (or nothing when !valid) into slot threadIdx.x and build the
-// per-wave lists. Contains __syncthreads; every lane must call it.
-__device__ __forceinline__ void stage_batch(Stage& s, const Tiling& t, const Splat2D* splats, const float* zsrc,
-                                            bool valid, uint32_t k, bool clamp_src, int mode) {
-  const uint32_t tid = threadIdx.x, lane = tid % kWave, sw = tid / kWave;
-  uint32_t mask = 0u;
-  if (valid) {
-    const Splat2D r = splats[k];
-    stage_splat(s, tid, r, t, clamp_src, mode, false);
-    if (zsrc) s.z[tid] = zsrc[k];
-    // which 8x8 blocks of the tile the bound [c - h, c + h] can touch (pixel
-    // centres of block (i, j) span [o + 8i + 0.5, o + 8i + 7.5]; half a pixel
-    // of margin)
-    const float hx = h_lo(r.hxy), hy = h_hi(r.hxy);
-    const float x0 = r.cx - hx - t.ox, x1 = r.cx + hx - t.ox;
-    const float y0 = r.cy - hy - t.oy, y1 = r.cy + hy - t.oy;
-    for (uint32_t w = 0; w < t.waves; ++w) {
-      const float bx = float((w % t.blocks_x) * 8), by = float((w / t.blocks_x) * 8);
-      const bool hit = !(x1 < bx) && !(x0 > bx + 8.0f) && !(y1 < by) && !(y0 > by + 8.0f);
-      mask |= hit ? (1u << w) : 0u;
-    }
-  }
-  uint64_t bal[kCompMaxWaves];
-#pragma unroll
-  for (int w = 0; w < kCompMaxWaves; ++w) {
-    bal[w] = __ballot((mask >> w) & 1u);
-    if (lane == 0) s.wcount[sw][w] = uint32_t(__popcll(bal[w]));
-  }
-  __syncthreads();
-#pragma unroll
-  for (int w = 0; w < kCompMaxWaves; ++w) {
-    if ((mask >> w) & 1u) {
-      uint32_t off = 0;
-      for (uint32_t q = 0; q < sw; ++q) off += s.wcount[q][w];
-      s.list[w][off + mask_rank(bal[w])] = uint16_t(tid);
-    }
-  }
-  __syncthreads();
-}
-
 // Stage splat k (or nothing) into slot threadIdx.x and build one ordered list
 // per 4x4 quadrant of the tile (row-major over the tile's quadrants; each
 // wave blends the four of its 8x8 block). (Testing each quadrant against the splat's ellipse,
@@ -201,12 +156,12 @@ __device__ __forceinline__ void stage_batch(Stage& s, const Tiling& t, const Spl
 template <int NT, bool Z>
 __device__ __forceinline__ void stage_batch_quads(StageT<NT, Z>& s, const Tiling& t, const Splat2D* splats,
                                                   const float* zsrc, bool valid, uint32_t k, bool clamp_src,
-                                                  int mode) {
+                                                  int mode, bool log2op) {
   const uint32_t tid = threadIdx.x, lane = tid % kWave, sw = tid / kWave;
   uint32_t mask = 0u;
   if (valid) {
     const Splat2D r = splats[k];
-    stage_splat(s, tid, r, t, clamp_src, mode, mode == kModeSplat);
+    stage_splat(s, tid, r, t, clamp_src, mode, log2op);
     if (Z && zsrc) s.z[tid] = zsrc[k];
     // Quadrant column c of the tile holds pixel centres [4c + 0.5, 4c + 3.5]
     // (tile-relative); the bound [x0, x1] may touch it when x1 >= 4c and
@@ -247,12 +202,6 @@ __device__ __forceinline__ void stage_batch_quads(StageT<NT, Z>& s, const Tiling
     if ((mask >> q) & 1u) s.list[q][off + mask_rank(bal)] = uint16_t(tid * 16u);
   }
   __syncthreads();
-}
-
-__device__ __forceinline__ uint32_t wave_list_len(const Stage& s, const Tiling& t, uint32_t w) {
-  uint32_t n = 0;
-  for (uint32_t q = 0; q < t.waves; ++q) n += s.wcount[q][w];
-  return n;
 }
 
 // Fragment alpha (render.wesl:143-166) of staged slot q at this lane's pixel
@@ -618,7 +567,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
       const bool valid = tid < cnt;
       // nearest first
       stage_batch_quads(s, t, a.splats, DEPTH ? a.splat_depth : nullptr, valid,
-                        valid ? a.pair_vals[bend - 1 - tid] : 0u, q8, mode);
+                        valid ? a.pair_vals[bend - 1 - tid] : 0u, q8, mode, mode == kModeSplat);
       if (mode == kModeSplat)
         blend_quads<kModeSplat, DEPTH, NT>(s, t, w, sd2l, ell2l, T, cr, cg, cb, done, dl);
       else if (mode == kModeEllipse)
@@ -673,7 +622,10 @@ __global__ __launch_bounds__(kCompMaxThreads) void k_composite_exact(CompositeAr
   const float e = fc.max_std_dev - 0.1f;
   const float ell2l = e * e * kLog2e;
   const bool q8 = a.format == kFormatRgba8Unorm;
-  const uint32_t nt = blockDim.x, tid = threadIdx.x, w = tid / kWave;
+  const uint32_t nt = blockDim.x, tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
+  // the lane's 4x4 quadrant list (row-major over the tile's quadrants)
+  const uint32_t ncols = 2u * t.blocks_x, bx = w % t.blocks_x, by = w / t.blocks_x;
+  const uint32_t myq = (2u * by + (lane >> 5)) * ncols + 2u * bx + ((lane >> 2) & 1u);
 
   uint32_t frags = 0;
   float dr = 0.0f, dg = 0.0f, db = 0.0f, da = 1.0f;
@@ -700,12 +652,13 @@ __global__ __launch_bounds__(kCompMaxThreads) void k_composite_exact(CompositeAr
     __syncthreads();
     const bool valid = tid < cnt;
     // farthest first
-    stage_batch(s, t, a.splats, dtest ? a.splat_depth : nullptr, valid, valid ? a.pair_vals[b0 + tid] : 0u, q8,
-                mode);
+    stage_batch_quads(s, t, a.splats, dtest ? a.splat_depth : nullptr, valid, valid ? a.pair_vals[b0 + tid] : 0u,
+                      q8, mode, false);
     if (!t.inside) continue;
-    const uint32_t len = wave_list_len(s, t, w);
+    uint32_t len = 0;
+    for (uint32_t p = 0; p < t.waves; ++p) len += uint32_t(__popcll(s.qbal[p][myq]));
     for (uint32_t j = 0; j < len; ++j) {
-      const uint32_t q = s.list[w][j];
+      const uint32_t q = uint32_t(s.list[myq][j]) >> 4;  // lists hold byte offsets
       float4 col;
       const float al = fragment(s, q, t, mode, sd2l, ell2l, col);
       if (al < 0.0f) continue;
