@@ -28,11 +28,9 @@
 #include "kernels.hpp"
 #include "types.hpp"
 
-
 namespace gs3d {
 
 constexpr int kCompMaxThreads = 256;
-constexpr int kCompMaxWaves = kCompMaxThreads / kWave;
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr float kSqrtLog2e = 1.2011224087864498f;
 
@@ -58,8 +56,9 @@ __device__ __forceinline__ float h_hi(uint32_t w) { return __half2float(__ushort
 constexpr uint32_t kListPad = 4;
 
 // LDS of a workgroup of NT lanes (one per pixel of its tile), with one list
-// per 4x4 quadrant (NT / 16 lists). Sized per tile shape, so that 8x8 tiles (one wave) take a quarter of a 256-lane tile's
-// LDS and as many workgroups fit a CU.
+// per 4x4 quadrant (NT / 16 lists). Sized per tile shape, so that 8x8 tiles
+// (one wave) take a quarter of a 256-lane tile's LDS and as many workgroups
+// fit a CU.
 template <int NT, bool Z = true>
 struct StageT {
   static constexpr int kWaves = NT / kWave;
@@ -73,7 +72,7 @@ struct StageT {
   float2 gb[NT + 1];   // g, b
   float z[Z ? NT + 1 : 1];  // fragment depth, for depth-tested draws (Z)
   uint16_t list[kQuads][NT + kListPad];
-  uint64_t qbal[kWaves][kQuads];    // fast path: [staging wave][quadrant] ballots
+  uint64_t qbal[kWaves][kQuads];  // [staging wave][quadrant] ballots
 };
 using Stage = StageT<kCompMaxThreads>;
 
@@ -149,10 +148,10 @@ __device__ __forceinline__ void stage_splat(StageT<NT, Z>& s, uint32_t tid, cons
 
 // Stage splat k (or nothing) into slot threadIdx.x and build one ordered list
 // per 4x4 quadrant of the tile (row-major over the tile's quadrants; each
-// wave blends the four of its 8x8 block). (Testing each quadrant against the splat's ellipse,
-// as the binning does per tile, shortens the lists but costs more in staging
-// than it saves in blending: measured.) Contains __syncthreads; every lane
-// must call it.
+// wave blends the four of its 8x8 block). (Testing each quadrant against the
+// splat's ellipse, as the binning does per tile, shortens the lists but costs
+// more in staging than it saves in blending: measured.) Contains
+// __syncthreads; every lane must call it.
 template <int NT, bool Z>
 __device__ __forceinline__ void stage_batch_quads(StageT<NT, Z>& s, const Tiling& t, const Splat2D* splats,
                                                   const float* zsrc, bool valid, uint32_t k, bool clamp_src,
@@ -196,8 +195,8 @@ __device__ __forceinline__ void stage_batch_quads(StageT<NT, Z>& s, const Tiling
 #pragma unroll
   for (int q = 0; q < kQuads; ++q) {
     const uint32_t off = __builtin_amdgcn_readlane(first, q);
-    // quadrant lists hold the slot's byte offset in geo / col (16 per slot)
-    // (the ballot again, rather than kept live across the barrier)
+    // the ballot again, rather than kept live across the barrier; quadrant
+    // lists hold the slot's byte offset in geo / col (16 per slot)
     const uint64_t bal = __ballot((mask >> q) & 1u);
     if ((mask >> q) & 1u) s.list[q][off + mask_rank(bal)] = uint16_t(tid * 16u);
   }
