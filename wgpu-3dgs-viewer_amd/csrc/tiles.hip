@@ -17,7 +17,11 @@
 namespace gs3d {
 
 constexpr int kBinThreads = 256;
-constexpr int kBinItems = 4;
+// instances per thread of a binning tile
+#ifndef GS3D_BIN_ITEMS
+#define GS3D_BIN_ITEMS 8
+#endif
+constexpr int kBinItems = GS3D_BIN_ITEMS;
 constexpr uint32_t kBinTile = kBinThreads * kBinItems;
 
 uint32_t bin_max_tiles(uint32_t max_instances) { return (max_instances + kBinTile - 1) / kBinTile; }
