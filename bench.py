@@ -10,7 +10,8 @@ Sections (all run by default; --sections picks a subset):
            replayed; tile size autotuned per view. One step = one batch.
   plain    the same model and resolution through one Viewer, one frame per
            call: frames/s, per-frame latency p50/p99, per-stage GPU times,
-           and the per-kernel timing the roofline figures come from.
+           and the per-kernel timing the roofline figures come from (always
+           run, whatever --sections names).
   mix      4 resident models of 0.5M/1M/3M/8M Gaussians (clustered scene,
            f16), 32 views per batch at resolutions drawn from
            {640x360, 1280x720, 1920x1080, 3840x2160}, cameras on spheres of
@@ -284,6 +285,7 @@ def main():
     ap.add_argument("--tile", default="16x16", help="tile size of the plain section")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=400.0)
+    ap.add_argument("--serve-queues", type=int, default=8, help="device queues of the serving loop")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -308,10 +310,9 @@ def main():
     tile = tuple(int(x) for x in args.tile.split("x"))
     out_extra = {}
 
-    plain = None
-    if "plain" in sections or "batched" in sections:
-        plain = run_plain(gs, dev, g, w, h, args.steps, args.warmup, tile)
-        log(f"plain: {plain['fps']:.1f} frames/s")
+    # the plain section always runs: the rooflines and stage times come from it
+    plain = run_plain(gs, dev, g, w, h, args.steps, args.warmup, tile)
+    log(f"plain: {plain['fps']:.1f} frames/s")
     batched = None
     if "batched" in sections:
         model = gs.Model(dev, g, gs.GaussianPod.HALF)
@@ -322,7 +323,7 @@ def main():
         out_extra["mix"] = run_mix(gs, dev, max(args.steps // 4, 4), 2, args.queues)
         log(f"mix: {out_extra['mix']['fps']:.1f} frames/s")
     if "serve" in sections:
-        out_extra["serve"] = run_serve(gs, dev, args.serve_seconds, args.serve_rate, 8)
+        out_extra["serve"] = run_serve(gs, dev, args.serve_seconds, args.serve_rate, args.serve_queues)
         log(f"serve: p50 {out_extra['serve']['p50_ms']:.2f} ms p99 {out_extra['serve']['p99_ms']:.2f} ms")
 
     # rooflines from the plain section's HIP events (same model, resolution
