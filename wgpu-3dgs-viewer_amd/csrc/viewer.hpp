@@ -17,6 +17,11 @@
 #include "kernels.hpp"
 #include "types.hpp"
 
+// splats per segment of a long tile list in the fast blend (split-K)
+#ifndef GS3D_COMPOSITE_SEG_LEN
+#define GS3D_COMPOSITE_SEG_LEN 2048
+#endif
+
 namespace gs3d {
 
 // Error carrying a gs3d_status (the reference's error enums, src/error.rs).
@@ -449,7 +454,7 @@ class Renderer {
   DeviceBuffer ranges_, bin_status_, bin_ticket_, pair_count_;
   uint32_t ranges_words_ = 0;
   // compositing work list (fast blend)
-  uint32_t seg_len_ = 512;
+  uint32_t seg_len_ = GS3D_COMPOSITE_SEG_LEN;
   DeviceBuffer item_tile_, item_seg_, seg_base_, plan_, plan_counts_, partial_;
   uint32_t* pair_total_host_ = nullptr;  // pinned, mapped
   uint32_t* pair_total_dev_ = nullptr;   // its device alias
