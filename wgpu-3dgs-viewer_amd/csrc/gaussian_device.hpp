@@ -148,6 +148,11 @@ __device__ __forceinline__ bool ellipse_meets_box(const EllipseTest& e, float bx
   return m <= e.lim + 1e-4f * (e1 * e1 + e2 * e2);
 }
 
+// The mask bits of every tile of a w x h rectangle (w, h <= 4).
+__device__ __forceinline__ uint32_t rect_bits(uint32_t w, uint32_t h) {
+  return ((1u << w) - 1u) * (0x1111u & ((1u << (4u * h)) - 1u));
+}
+
 // Inclusive tile rectangle a splat can touch and, for rectangles of at most
 // 4 x 4 tiles, which of those tiles its ellipse |qo|^2 <= sd2 reaches:
 //   x = tx0 | ty0 << 12 | mask bits 0-7 << 24
@@ -155,15 +160,10 @@ __device__ __forceinline__ bool ellipse_meets_box(const EllipseTest& e, float bx
 // with bit 4 j + i of the mask for tile (tx0 + i, ty0 + j); point-mode
 // splats (rho = 0) set the bits of every tile of the rectangle. Larger
 // rectangles keep every tile (mask unused). (0xFFFFFFFF, 0) when it touches
-// none. Tile coordinates take 12
-// bits: textures are at most 16384 pixels wide and tiles at least 8. A pixel
-// x is covered only if |x + 0.5 - cx| <= hx, so [cx - hx - 1, cx + hx + 1] is a
-// conservative pixel range; hxy holds the f16 half extents rounded up (0:
-// draws no pixel).
-// The mask bits of every tile of a w x h rectangle (w, h <= 4).
-__device__ __forceinline__ uint32_t rect_bits(uint32_t w, uint32_t h) {
-  return ((1u << w) - 1u) * (0x1111u & ((1u << (4u * h)) - 1u));
-}
+// none. Tile coordinates take 12 bits: textures are at most 16384 pixels wide
+// and tiles at least 8. A pixel x is covered only if |x + 0.5 - cx| <= hx, so
+// [cx - hx - 1, cx + hx + 1] is a conservative pixel range; hxy holds the f16
+// half extents rounded up (0: draws no pixel).
 
 __device__ __forceinline__ uint2 tile_rect(const Splat2D& s, float sd2, const TileGrid& g) {
   const uint2 none = make_uint2(0xFFFFFFFFu, 0u);
@@ -343,10 +343,10 @@ __device__ __forceinline__ void compute_splat(const FrameConsts& fc, const Gauss
     alpha = 1.0f;
     drawable = zclip_ok;
   } else {
-    // cov2d (utils.wesl:25-48) with the zero entries of J dropped
-    const float p4[4] = {pos[0], pos[1], pos[2], 1.0f};
-    float t[4];
-    m4v(fc.vm, p4, t);
+    // cov2d (utils.wesl:25-48) with the zero entries of J dropped; its
+    // t = (view * model) * pos is the view-space centre computed above as
+    // view * (model * pos) (the same point, rounded in another order)
+    const float* t = view;
     const float itz = __builtin_amdgcn_rcpf(t[2]);
     const float j00 = fc.focal[0] * itz, j11 = fc.focal[1] * itz;
     const float j20 = -fc.focal[0] * t[0] * itz * itz, j21 = -fc.focal[1] * t[1] * itz * itz;
