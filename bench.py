@@ -60,7 +60,7 @@ FLOPS_PER_FRAGMENT = 20
 # 1 GiB copy (x2.000 and x1.000); profiles/traffic_clustered-6M-1440p-f16_plain.md.
 # Mostly the 32-byte splat records gathered per (tile, splat) pair, one
 # 128-byte line each.
-COMPOSITE_TRAFFIC_BYTES = 988567291
+COMPOSITE_TRAFFIC_BYTES = 947750072
 
 MIX_RES = [(640, 360), (1280, 720), (1920, 1080), (3840, 2160)]
 
