@@ -46,20 +46,23 @@ FP32_PEAK_TFLOPS = 157.3
 
 # Algorithmic bytes of the projection kernel per Gaussian read (f16 pod:
 # position+colour 16, packed f16 rotation/scale 16, f16 SH 96) and per visible
-# Gaussian written (depth key 4 + index 4 + 32-byte splat record).
+# Gaussian written (depth key 4 + index 4 + 32-byte splat record + 8-byte tile
+# rectangle for the binning).
 R_IN = {"single": 220, "half": 128}
-R_OUT = 40
+R_OUT = 48
 # Depth sort: 4 passes x (read key+value 8, write key+value 8) per key, plus
 # the one histogram read of the keys (4).
 SORT_BYTES_PER_KEY = 68
 # Compositing: floating-point operations per blended fragment (r^2 from the
 # two axis terms 9, exp2 1, alpha 1, weight 1, colour 6, transmittance 2).
 FLOPS_PER_FRAGMENT = 20
-# HBM bytes per launch of k_composite_fast on this workload: rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE, each scaled by what the same counter reads for a
-# 1 GiB copy (x2.000 and x1.000); profiles/traffic_clustered-6M-1440p-f16_plain.md.
-# Mostly the 32-byte splat records gathered per (tile, splat) pair, one
-# 128-byte line each.
+# HBM bytes per launch on this workload: rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE, each scaled by what the same counter reads for a 1 GiB copy
+# (x2.000 and x1.000); profiles/traffic_clustered-6M-1440p-f16_plain.md.
+# k_preprocess: the Gaussian streams plus the sector overhead of the scattered
+# per-Gaussian splat/rectangle writes; k_composite_fast: mostly the 32-byte
+# splat records gathered per (tile, splat) pair, one 128-byte line each.
+PREPROCESS_TRAFFIC_BYTES = 1101224960
 COMPOSITE_TRAFFIC_BYTES = 947750072
 
 MIX_RES = [(640, 360), (1280, 720), (1920, 1080), (3840, 2160)]
@@ -333,22 +336,35 @@ def main():
     comp_ms = float(stage_ms[5])
     comp_flops = plain["fragments_mean"] * FLOPS_PER_FRAGMENT
     comp_tflops = comp_flops / (comp_ms * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "achieved": round(comp_tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(comp_tflops / FP32_PEAK_TFLOPS, 4), "traffic": COMPOSITE_TRAFFIC_BYTES,
-                "kernel": "k_composite_fast",
-                "note": "vector fp32 (exp2 + blend per fragment); no MFMA, compositing is not a contraction",
-                "avg_launch_ms": round(comp_ms, 4),
-                "fragments_per_launch": int(plain["fragments_mean"]),
-                "flops_per_fragment": FLOPS_PER_FRAGMENT}
+    composite = {"bound": "mfma", "achieved": round(comp_tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                 "frac": round(comp_tflops / FP32_PEAK_TFLOPS, 4), "traffic": COMPOSITE_TRAFFIC_BYTES,
+                 "kernel": "k_composite_fast",
+                 "note": "vector fp32 (exp2 + blend per fragment); no MFMA, compositing is not a contraction",
+                 "avg_launch_ms": round(comp_ms, 4),
+                 "fragments_per_launch": int(plain["fragments_mean"]),
+                 "flops_per_fragment": FLOPS_PER_FRAGMENT}
+    # the dominant kernel of the frame: the projection pass (k_preprocess),
+    # priced on the bytes it must move per launch
+    pre_ms = float(stage_ms[6])
     pre_bytes = n * R_IN["half"] + v_mean * R_OUT
-    pre_gbs = pre_bytes / (float(stage_ms[0]) * 1e-3) / 1e9
+    pre_gbs = pre_bytes / (pre_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(pre_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(pre_gbs / HBM_PEAK_GBS, 4), "traffic": PREPROCESS_TRAFFIC_BYTES,
+                "kernel": "k_preprocess<1>",
+                "avg_launch_ms": round(pre_ms, 4),
+                "bytes_per_launch": int(pre_bytes),
+                "bytes_per_unit": {"gaussian_read": R_IN["half"], "visible_written": R_OUT},
+                "units_per_launch": {"gaussians": n, "visible": int(v_mean)}}
     sort_gbs = SORT_BYTES_PER_KEY * v_mean / (float(stage_ms[1]) * 1e-3) / 1e9
-    hbm = {"preprocess": {"achieved_gbs": round(pre_gbs, 1), "frac": round(pre_gbs / HBM_PEAK_GBS, 4),
-                          "bytes_per_launch": int(pre_bytes), "ms": round(float(stage_ms[0]), 4)},
+    stage_pre_gbs = pre_bytes / (float(stage_ms[0]) * 1e-3) / 1e9
+    hbm = {"preprocess_stage": {"achieved_gbs": round(stage_pre_gbs, 1), "frac": round(stage_pre_gbs / HBM_PEAK_GBS, 4),
+                                "bytes_per_launch": int(pre_bytes), "ms": round(float(stage_ms[0]), 4),
+                                "note": "k_frame_pre + k_preprocess"},
            "depth_sort": {"achieved_gbs": round(sort_gbs, 1), "frac": round(sort_gbs / HBM_PEAK_GBS, 4),
                           "bytes_per_sort": int(SORT_BYTES_PER_KEY * v_mean), "ms": round(float(stage_ms[1]), 4)}}
 
-    names = ["preprocess", "sort", "bin", "tile_sort", "composite", "composite_main_kernel"]
+    names = ["preprocess", "sort", "bin", "tile_sort", "composite", "composite_main_kernel",
+             "preprocess_main_kernel"]
     if batched is not None:
         value, ms_step, steps = batched["fps"], batched["ms_per_step"], args.steps
         form = f"batched: {args.views} views per batch, HIP graph over {args.queues} queues, tile autotuned"
@@ -375,6 +391,7 @@ def main():
                   "latency_ms": plain["latency_ms"], "tile": args.tile,
                   "stage_ms": {nm: round(float(x), 4) for nm, x in zip(names, stage_ms)}},
         "roofline": roofline,
+        "composite_roofline": composite,
         "hbm_rooflines": hbm,
     }
     if batched is not None:

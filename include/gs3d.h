@@ -130,7 +130,9 @@ typedef enum gs3d_stage {
     GS3D_STAGE_COMPOSITE = 4,
     /* the main compositing kernel alone, an interval inside COMPOSITE */
     GS3D_STAGE_COMPOSITE_MAIN = 5,
-    GS3D_STAGE_COUNT = 6
+    /* the projection kernel alone, an interval inside PREPROCESS */
+    GS3D_STAGE_PREPROCESS_MAIN = 6,
+    GS3D_STAGE_COUNT = 7
 } gs3d_stage;
 
 typedef struct gs3d_device gs3d_device;

@@ -96,6 +96,21 @@ def test_empty_model_renders_black(gs, device):
     assert np.all(img[..., :3] == 0) and np.all(img[..., 3] == 255)
 
 
+def test_stage_times_nested_and_identical_image(gs, device):
+    _case("stage times")
+    g, pod, w, h = _small_scene(gs)
+    v0, ref = _render(gs, device, g, pod, w, h)
+    v0.close()
+    v, img = _render(gs, device, g, pod, w, h, setup=lambda vv: vv.set_profiling(True))
+    ms = v.stage_times()
+    v.close()
+    assert np.array_equal(img, ref)
+    assert ms.shape == (7,) and np.all(ms >= 0.0) and np.all(np.isfinite(ms))
+    # the kernel-only spans lie inside their stages
+    assert 0.0 < ms[6] <= ms[0]
+    assert 0.0 < ms[5] <= ms[4]
+
+
 def test_model_size_exceeds_device_limit(gs, device):
     _case("model size limit")
     g = gs.synth_gaussians(0, 1000, 0)

@@ -537,7 +537,7 @@ class Viewer:
         _check(lib().gs3d_viewer_set_profiling(self._h, int(bool(on))))
 
     def stage_times(self) -> np.ndarray:
-        out = np.zeros(6, np.float32)
+        out = np.zeros(7, np.float32)
         _check(lib().gs3d_viewer_stage_times(self._h, _ptr(out)))
         return out
 

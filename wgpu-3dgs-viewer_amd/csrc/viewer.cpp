@@ -491,12 +491,15 @@ Preprocessor::Preprocessor(Device& dev, const CameraBuffer& camera, const ModelT
     throw Error(GS3D_ERROR_MODEL_SIZE_EXCEEDS_DEVICE_LIMIT, "model size exceeds the device limit");
 }
 
-void Preprocessor::preprocess(uint32_t gaussian_count) {
+void Preprocessor::frame_pre(uint32_t gaussian_count) {
   const uint32_t n = std::min(gaussian_count, gaussians_.len());
-  const uint32_t tiles = preprocess_num_tiles(n);
   launch_frame_pre(st_, camera_.dev(), model_.dev(), gt_.dev(), consts_.as<FrameConsts>(),
-                   indirect_args_.dev(), rs_args_.dev(), status_.as<uint32_t>(), tiles, ticket_.as<uint32_t>(),
-                   sort_hist_);
+                   indirect_args_.dev(), rs_args_.dev(), status_.as<uint32_t>(), preprocess_num_tiles(n),
+                   ticket_.as<uint32_t>(), sort_hist_);
+}
+
+void Preprocessor::project(uint32_t gaussian_count) {
+  const uint32_t n = std::min(gaussian_count, gaussians_.len());
   PreprocessArgs a{};
   a.fc = consts_.as<FrameConsts>();
   a.g = gaussians_.streams();
@@ -509,7 +512,7 @@ void Preprocessor::preprocess(uint32_t gaussian_count) {
   a.dispatch_args = rs_args_.dev();
   a.status = status_.as<uint32_t>();
   a.ticket = ticket_.as<uint32_t>();
-  a.num_tiles = tiles;
+  a.num_tiles = preprocess_num_tiles(n);
   a.err = dev_.err();
   a.selection = selection_;
   a.splat_depth = splat_depth_;
@@ -520,6 +523,11 @@ void Preprocessor::preprocess(uint32_t gaussian_count) {
   a.invert_selection = invert_selection_ ? 1u : 0u;
   launch_preprocess(st_, gaussians_.pod(), a);
   hip_check(hipGetLastError(), "launch preprocess");
+}
+
+void Preprocessor::preprocess(uint32_t gaussian_count) {
+  frame_pre(gaussian_count);
+  project(gaussian_count);
 }
 
 // ---- sort ----------------------------------------------------------------------
@@ -865,7 +873,10 @@ void Viewer::render_with_depth(Texture& target, Texture& depth, bool load) {
   // the projection also writes the depth sort's first histogram
   preprocessor.set_sort_hist(radix_sorter.hist(), radix_sorter.len());
   timer.mark(GS3D_STAGE_PREPROCESS, false);
-  preprocessor.preprocess(gaussians_buffer.len());
+  preprocessor.frame_pre(gaussians_buffer.len());
+  timer.mark(GS3D_STAGE_PREPROCESS_MAIN, false);
+  preprocessor.project(gaussians_buffer.len());
+  timer.mark(GS3D_STAGE_PREPROCESS_MAIN, true);
   timer.mark(GS3D_STAGE_PREPROCESS, true);
   timer.mark(GS3D_STAGE_SORT, false);
   radix_sorter.sort(radix_sort_indirect_args_buffer, true);
@@ -878,7 +889,10 @@ void Viewer::render(Texture& target) {
   // the projection also writes the depth sort's first histogram
   preprocessor.set_sort_hist(radix_sorter.hist(), radix_sorter.len());
   timer.mark(GS3D_STAGE_PREPROCESS, false);
-  preprocessor.preprocess(gaussians_buffer.len());
+  preprocessor.frame_pre(gaussians_buffer.len());
+  timer.mark(GS3D_STAGE_PREPROCESS_MAIN, false);
+  preprocessor.project(gaussians_buffer.len());
+  timer.mark(GS3D_STAGE_PREPROCESS_MAIN, true);
   timer.mark(GS3D_STAGE_PREPROCESS, true);
   timer.mark(GS3D_STAGE_SORT, false);
   radix_sorter.sort(radix_sort_indirect_args_buffer, true);

@@ -319,6 +319,10 @@ class Preprocessor {
                RadixSortIndirectArgsBuffer& rs_args, IndirectIndicesBuffer& indices, GaussiansDepthBuffer& depth,
                SplatsBuffer& splats, int queue = 0);
   void preprocess(uint32_t gaussian_count);
+  // preprocess() in two steps: the per-frame constants and resets, then the
+  // projection kernel (timed alone by the viewer's profiling)
+  void frame_pre(uint32_t gaussian_count);
+  void project(uint32_t gaussian_count);
   const FrameConsts* frame_consts() const { return consts_.as<FrameConsts>(); }
   // Selection early-out (preprocess.wesl:69-78): `words` is a device bitset
   // of the Gaussians (null: none); a Gaussian is skipped when its bit equals
