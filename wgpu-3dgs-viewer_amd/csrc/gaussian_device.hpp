@@ -91,61 +91,59 @@ __device__ __forceinline__ void cov3d_of(const CovWords& w, float c[6]) {
   }
 }
 
-// Whether the ellipse |qo|^2 = (d.u)^2 + (d.v)^2 <= lim meets a box of
-// offsets d from its centre, for u = (ux, uy) and v = rho (-uy, ux) (a
-// Splat2D's axes). As a quadratic form, a dx^2 + 2 b dx dy + c dy^2 with
-// a = |(ux, vx)|^2, b = ux uy + vx vy, c = |(uy, vy)|^2, it is convex with
-// its minimum at the centre, so when the centre is outside the box the box's
-// minimum lies on a face that the segment from it to the centre leaves
-// through: the face nearer the centre in x and/or in y. On the face x = xe
-// the minimum over dy is at -b xe / c, clamped to the face (likewise in y);
-// the form is evaluated there as the sum of squares. A misplaced minimiser
-// (the hardware reciprocal) raises the value only by second-order terms.
-// The comparison carries a margin of 1e-4 of the largest value the squares'
-// terms reach over the box, far above the rounding of this evaluation and of
-// the blend's, so a fragment the blend keeps is never dropped. Degenerate or
-// non-finite axes answer true.
-struct EllipseTest {
-  float ux, uy, vx, vy;
-  float ba, bc;  // b / a, b / c
-  float lim;
+// The ellipse |qo|^2 = (d.u)^2 + (d.v)^2 <= lim of a splat's offsets d from
+// its centre, for u = (ux, uy) and v = rho (-uy, ux) (a Splat2D's axes), is
+// the quadratic form A dx^2 + 2 B dx dy + C dy^2 <= lim with A = |(ux, vx)|^2,
+// B = ux uy + vx vy, C = |(uy, vy)|^2 and determinant D^2, D = rho |u|^2
+// (formed from the axes, not as A C - B^2, which cancels for thin splats).
+// Its extents are |dx| <= xe = sqrt(lim C) / D and |dy| <= ye = sqrt(lim A) / D;
+// the rightmost point lies at dy = yr = -(B / C) xe, the leftmost at -yr.
+// Within a horizontal band by0 <= dy <= by1 the ellipse is convex, so it meets
+// a tile's box of pixel-centre offsets [bx0, bx1] x [by0, by1] exactly when
+// [bx0, bx1] meets the band's x-interval [xl, xr]: xr is the right root
+// x+(dy) = (-B dy + sqrt(A lim - D^2 dy^2)) / A, concave in dy, at dy = yr
+// clamped to the band, and xl likewise at -yr. One row of tiles thus costs
+// two square roots instead of one test per tile. The interval is widened by
+// 0.4% of the extents plus 1e-3 pixel, above the rounding of this evaluation
+// (the discriminant's cancellation near a tip moves a root by at most ~1e-3
+// of sqrt(lim / A) <= xe) and of the blend's, so a fragment the blend keeps is
+// never dropped.
+struct EllipseRows {
+  float A, B, iA, alim, det2;  // det2 = D^2
+  float yr, ye, tol;
   bool ok;
 };
 
-__device__ __forceinline__ EllipseTest ellipse_test(float ux, float uy, float rho, float lim) {
-  EllipseTest e;
-  e.ux = ux;
-  e.uy = uy;
-  e.vx = -rho * uy;
-  e.vy = rho * ux;
-  const float a = ux * ux + e.vx * e.vx, b = ux * uy + e.vx * e.vy, c = uy * uy + e.vy * e.vy;
-  e.ok = a > 0.0f && c > 0.0f && a < __builtin_inff() && c < __builtin_inff();
-  e.ba = b * __builtin_amdgcn_rcpf(a);
-  e.bc = b * __builtin_amdgcn_rcpf(c);
-  e.lim = lim;
+__device__ __forceinline__ EllipseRows ellipse_rows(float ux, float uy, float rho, float lim) {
+  EllipseRows e;
+  const float vx = -rho * uy, vy = rho * ux;
+  e.A = ux * ux + vx * vx;
+  e.B = ux * uy + vx * vy;
+  const float C = uy * uy + vy * vy;
+  const float D = rho * (ux * ux + uy * uy);
+  e.ok = e.A > 0.0f && C > 0.0f && D > 0.0f && e.A < __builtin_inff() && C < __builtin_inff() &&
+         D < __builtin_inff();
+  const float iD = __builtin_amdgcn_rcpf(D);
+  e.iA = __builtin_amdgcn_rcpf(e.A);
+  const float xe = __builtin_amdgcn_sqrtf(lim * C) * iD;
+  e.ye = __builtin_amdgcn_sqrtf(lim * e.A) * iD;
+  e.yr = -(e.B * __builtin_amdgcn_rcpf(C)) * xe;
+  e.alim = e.A * lim;
+  e.det2 = D * D;
+  e.tol = 4e-3f * (xe + e.ye) + 1e-3f;
   return e;
 }
 
-__device__ __forceinline__ bool ellipse_meets_box(const EllipseTest& e, float bx0, float bx1, float by0, float by1) {
-  const bool fx = bx0 > 0.0f || bx1 < 0.0f;
-  const bool fy = by0 > 0.0f || by1 < 0.0f;
-  if ((!fx && !fy) || !e.ok) return true;  // the centre is inside, or no test
-  const float xe = bx0 > 0.0f ? bx0 : bx1;
-  const float ye = by0 > 0.0f ? by0 : by1;
-  float m = __builtin_inff();
-  if (fx) {
-    const float dy = fminf(fmaxf(-e.bc * xe, by0), by1);
-    const float t1 = xe * e.ux + dy * e.uy, t2 = xe * e.vx + dy * e.vy;
-    m = t1 * t1 + t2 * t2;
-  }
-  if (fy) {
-    const float dx = fminf(fmaxf(-e.ba * ye, bx0), bx1);
-    const float t1 = dx * e.ux + ye * e.uy, t2 = dx * e.vx + ye * e.vy;
-    m = fminf(m, t1 * t1 + t2 * t2);
-  }
-  const float mx = fmaxf(fabsf(bx0), fabsf(bx1)), my = fmaxf(fabsf(by0), fabsf(by1));
-  const float e1 = fabsf(e.ux) * mx + fabsf(e.uy) * my, e2 = fabsf(e.vx) * mx + fabsf(e.vy) * my;
-  return m <= e.lim + 1e-4f * (e1 * e1 + e2 * e2);
+// The band's x-interval [xl, xr] (widened by tol); false when the band
+// misses the ellipse's rows.
+__device__ __forceinline__ bool ellipse_row_span(const EllipseRows& e, float by0, float by1, float& xl, float& xr) {
+  if (by0 > e.ye + e.tol || by1 < -e.ye - e.tol) return false;
+  const float yR = fminf(fmaxf(e.yr, by0), by1), yL = fminf(fmaxf(-e.yr, by0), by1);
+  const float dR = __builtin_amdgcn_sqrtf(fmaxf(e.alim - e.det2 * (yR * yR), 0.0f));
+  const float dL = __builtin_amdgcn_sqrtf(fmaxf(e.alim - e.det2 * (yL * yL), 0.0f));
+  xr = (dR - e.B * yR) * e.iA + e.tol;
+  xl = (-dL - e.B * yL) * e.iA - e.tol;
+  return true;
 }
 
 // The mask bits of every tile of a w x h rectangle (w, h <= 4).
@@ -183,14 +181,21 @@ __device__ __forceinline__ uint2 tile_rect(const Splat2D& s, float sd2, const Ti
   uint32_t mask = small ? rect_bits(tx1 - tx0 + 1u, ty1 - ty0 + 1u) : 0xFFFFu;
   // (a rectangle of one tile keeps it: the bound rarely misses all of it)
   if (s.rho > 0.0f && small && (tx0 != tx1 || ty0 != ty1)) {
-    const EllipseTest e = ellipse_test(s.ux, s.uy, s.rho, sd2 * 1.0001f);
-    mask = 0u;
-    for (uint32_t ty = ty0; ty <= ty1; ++ty) {
-      // pixel centres of the tile: [ty th + 0.5, (ty + 1) th - 0.5]
-      const float by0 = float(ty * g.tile_h) + 0.5f - s.cy, by1 = by0 + float(g.tile_h - 1u);
-      for (uint32_t tx = tx0; tx <= tx1; ++tx) {
-        const float bx0 = float(tx * g.tile_w) + 0.5f - s.cx, bx1 = bx0 + float(g.tile_w - 1u);
-        if (ellipse_meets_box(e, bx0, bx1, by0, by1)) mask |= 1u << (4u * (ty - ty0) + (tx - tx0));
+    const EllipseRows e = ellipse_rows(s.ux, s.uy, s.rho, sd2 * 1.0001f);
+    if (e.ok) {
+      const float itw = 1.0f / float(g.tile_w);  // exact: a power of two
+      mask = 0u;
+      for (uint32_t ty = ty0; ty <= ty1; ++ty) {
+        // pixel centres of the tile row: [ty th + 0.5, (ty + 1) th - 0.5]
+        const float by0 = float(ty * g.tile_h) + 0.5f - s.cy, by1 = by0 + float(g.tile_h - 1u);
+        float xl, xr;
+        if (!ellipse_row_span(e, by0, by1, xl, xr)) continue;
+        // columns tx whose pixel centres [tx tw + 0.5, tx tw + tw - 0.5] - cx
+        // meet [xl, xr], clamped to the rectangle
+        const float fhi = fminf(floorf((s.cx + xr - 0.5f) * itw), float(tx1));
+        const float flo = fmaxf(ceilf((s.cx + xl + 0.5f) * itw) - 1.0f, float(tx0));
+        if (flo <= fhi)
+          mask |= ((2u << (uint32_t(fhi) - tx0)) - (1u << (uint32_t(flo) - tx0))) << (4u * (ty - ty0));
       }
     }
   }
