@@ -32,27 +32,46 @@ __device__ __forceinline__ void status_store(uint32_t* p, uint32_t v) {
 // error into a reported error instead of a hang.
 constexpr uint32_t kSpinLimit = 1u << 20;
 
-// Decoupled look-back (Merrill & Garland 2016) for one value. `status[t]`
-// already holds this tile's aggregate; walks predecessors until an inclusive
-// prefix is found and returns the exclusive prefix of tile t.
-__device__ __forceinline__ uint32_t lookback_exclusive(uint32_t* status, uint32_t t, uint32_t* err) {
-  uint32_t excl = 0;
-  int64_t j = int64_t(t) - 1;
-  uint32_t spins = 0;
-  while (j >= 0) {
-    const uint32_t s = status_load(status + j);
+// Sum over the wave's 64 lanes (every lane active), in every lane.
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// Decoupled look-back (Merrill & Garland 2016) for one value, run by a whole
+// wave (all 64 lanes active; the result in every lane). `status[t]` already
+// holds this tile's aggregate; returns the exclusive prefix of tile t. Each
+// step loads the 64 nearest unexamined predecessors' words at once (one
+// round trip instead of one per predecessor), adds the aggregates up to the
+// nearest inclusive prefix and stops there or, when a predecessor has not
+// published yet, adds those before it and continues from it.
+__device__ __forceinline__ uint32_t lookback_exclusive_wave(uint32_t* status, uint32_t t, uint32_t* err) {
+  const uint32_t lane = lane_id();
+  uint32_t excl = 0, spins = 0;
+  int64_t hi = int64_t(t) - 1;
+  while (hi >= 0) {
+    const int64_t j = hi - int64_t(lane);
+    // below tile 0: an inclusive prefix of 0
+    const uint32_t s = j >= 0 ? status_load(status + j) : kFlagInclusive;
     const uint32_t flag = s & kFlagMask;
-    if (flag == 0u) {
+    const uint64_t pending = __ballot(flag == 0u);
+    const uint64_t incl = __ballot(flag == kFlagInclusive);
+    const uint32_t first_pending = pending ? uint32_t(__builtin_ctzll(pending)) : 64u;
+    const uint32_t first_incl = incl ? uint32_t(__builtin_ctzll(incl)) : 64u;
+    if (first_incl < first_pending) {
+      excl += wave_sum(lane <= first_incl ? (s & kValueMask) : 0u);
+      break;
+    }
+    excl += wave_sum(lane < first_pending ? (s & kValueMask) : 0u);
+    hi -= int64_t(first_pending);
+    if (first_pending == 0u) {
       __builtin_amdgcn_s_sleep(1);
       if (++spins > kSpinLimit) {
-        atomicOr(err, kErrLookbackSpin);
+        if (lane == 0) atomicOr(err, kErrLookbackSpin);
         break;
       }
-      continue;
     }
-    excl += s & kValueMask;
-    if (flag == kFlagInclusive) break;
-    --j;
   }
   return excl;
 }

@@ -75,7 +75,27 @@ __device__ __forceinline__ CovWords load_cov_words(const GaussianStreams& g, uin
   return w;
 }
 
-template <int POD>
+// cov3d_from_scale_rot contracted (the record's covariance; the bound
+// re-test of the cull keeps the uncontracted one).
+__device__ __forceinline__ void cov3d_fast(const float s[3], const float q[4], float out[6]) {
+#pragma clang fp contract(fast)
+  float rm[9], mm[9];
+  quat_to_mat3(q, rm);
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) mm[c * 3 + r] = rm[c * 3 + r] * s[c];
+  const int ia[6] = {0, 0, 0, 1, 1, 2};
+  const int ib[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+  for (int e = 0; e < 6; ++e)
+    out[e] = mm[0 * 3 + ia[e]] * mm[0 * 3 + ib[e]] + mm[1 * 3 + ia[e]] * mm[1 * 3 + ib[e]] +
+             mm[2 * 3 + ia[e]] * mm[2 * 3 + ib[e]];
+}
+
+// The cov3d of one Gaussian from its covariance words; kFast contracts the
+// half form's product (for the splat record only).
+template <int POD, bool kFast = false>
 __device__ __forceinline__ void cov3d_of(const CovWords& w, float c[6]) {
   if constexpr (POD == kPodSingle) {
     c[0] = w.a.x;
@@ -87,7 +107,10 @@ __device__ __forceinline__ void cov3d_of(const CovWords& w, float c[6]) {
   } else {
     const float q[4] = {half_lo(w.rs.x), half_hi(w.rs.x), half_lo(w.rs.y), half_hi(w.rs.y)};
     const float s[3] = {half_lo(w.rs.z), half_hi(w.rs.z), half_lo(w.rs.w)};
-    cov3d_from_scale_rot(s, q, c);
+    if constexpr (kFast)
+      cov3d_fast(s, q, c);
+    else
+      cov3d_from_scale_rot(s, q, c);
   }
 }
 
@@ -183,7 +206,7 @@ __device__ __forceinline__ uint2 tile_rect(const Splat2D& s, float sd2, const Ti
   if (s.rho > 0.0f && small && (tx0 != tx1 || ty0 != ty1)) {
     const EllipseRows e = ellipse_rows(s.ux, s.uy, s.rho, sd2 * 1.0001f);
     if (e.ok) {
-      const float itw = 1.0f / float(g.tile_w);  // exact: a power of two
+      const float itw = __uint_as_float((127u - sx) << 23);  // 1 / tile_w, a power of two
       mask = 0u;
       for (uint32_t ty = ty0; ty <= ty1; ++ty) {
         // pixel centres of the tile row: [ty th + 0.5, (ty + 1) th - 0.5]
@@ -225,16 +248,61 @@ __device__ __forceinline__ uint32_t rect_tiles(const TileRect& t) {
   return (t.tx1 - t.tx0 + 1u) * (t.ty1 - t.ty0 + 1u);
 }
 
-// SH colour of one instance (utils.wesl:82-136) for the splat record:
-// every 16-byte chunk of coefficients is requested first (all in flight at
-// once), the 15 basis values of the view direction are formed meanwhile,
-// then each chunk is folded into the three channel sums as packed words,
-// without unpacking all 45 coefficients at once. The sums run in
-// coefficient order, not the shader's association (the record is compared
-// within a tolerance, not bit for bit).
+// First R rows of the column-major 4x4 matrix m times v, contracted to fused
+// multiply-adds (for the splat record, which is not compared bit for bit).
+template <int R>
+__device__ __forceinline__ void mv4_fast(const float* m, const float v[4], float* out) {
+#pragma clang fp contract(fast)
+#pragma unroll
+  for (int r = 0; r < R; ++r) out[r] = m[0 * 4 + r] * v[0] + m[1 * 4 + r] * v[1] + m[2 * 4 + r] * v[2] + m[3 * 4 + r] * v[3];
+}
+
+__device__ __forceinline__ void mv3_fast(const float* m, const float v[3], float out[3]) {
+#pragma clang fp contract(fast)
+#pragma unroll
+  for (int r = 0; r < 3; ++r) out[r] = m[0 * 3 + r] * v[0] + m[1 * 3 + r] * v[1] + m[2 * 3 + r] * v[2];
+}
+
+// The SH stream's words of one Gaussian, requested ahead of their use.
 template <int POD>
-__device__ __forceinline__ void sh_color(const GaussianStreams& g, uint32_t i, float x, float y, float z, int sh_deg,
-                                         float rgb[3]) {
+struct ShWords {
+  uint4 v[6];  // kPodHalf: 45 halves + 3 pad
+};
+template <>
+struct ShWords<kPodSingle> {
+  float4 v[12];
+};
+
+__device__ __forceinline__ int sh_bands(int sh_deg) { return sh_deg >= 3 ? 15 : (sh_deg == 2 ? 8 : (sh_deg == 1 ? 3 : 0)); }
+
+// Requests the 16-byte chunks of coefficients the SH degree uses (all in
+// flight at once; lower degrees touch only their planes).
+template <int POD>
+__device__ __forceinline__ void sh_load(const GaussianStreams& g, uint32_t i, int sh_deg, ShWords<POD>& w) {
+  const int kmax = sh_bands(sh_deg);
+  if constexpr (POD == kPodSingle) {
+    const float4* p = g.sh32 + i;
+    const int nq = (3 * kmax + 3) / 4;
+#pragma unroll
+    for (int q = 0; q < 12; ++q)
+      if (q < nq) w.v[q] = p[size_t(q) * g.n];
+  } else {
+    const uint4* p = g.sh16 + i;
+    const int nq = (3 * kmax + 7) / 8;
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+      if (q < nq) w.v[q] = p[size_t(q) * g.n];
+  }
+}
+
+// SH colour of one instance (utils.wesl:82-136) for the splat record: the
+// 15 basis values of the view direction, then each loaded chunk folded into
+// the three channel sums as packed words, without unpacking all 45
+// coefficients at once. The sums run in coefficient order, not the shader's
+// association (the record is compared within a tolerance, not bit for bit).
+template <int POD>
+__device__ __forceinline__ void sh_fold(const ShWords<POD>& w, float x, float y, float z, int sh_deg,
+                                        float rgb[3]) {
 #pragma clang fp contract(fast)
   if (sh_deg < 1) return;
   const float xx = x * x, yy = y * y, zz = z * z;
@@ -247,49 +315,40 @@ __device__ __forceinline__ void sh_color(const GaussianStreams& g, uint32_t i, f
       (ShC::c3_5 * z) * (xx - yy), (ShC::c3_6 * x) * (xx - 3.0f * yy)};
   // coefficient j (0..44) is band-k coefficient k = j / 3 of channel j % 3;
   // bands start at k = 0 (degree 1), 3 (degree 2) and 8 (degree 3)
-  const int kmax = sh_deg >= 3 ? 15 : (sh_deg == 2 ? 8 : 3);
+  const int kmax = sh_bands(sh_deg);
   auto add = [&](int j, float v) {
     if (j < 45 && j / 3 < kmax) rgb[j % 3] += basis[j / 3] * v;
   };
   if constexpr (POD == kPodSingle) {
-    const float4* p = g.sh32 + i;
     const int nq = (3 * kmax + 3) / 4;
-    float4 v[12];
-#pragma unroll
-    for (int q = 0; q < 12; ++q)
-      if (q < nq) v[q] = p[size_t(q) * g.n];
 #pragma unroll
     for (int q = 0; q < 12; ++q) {
-      if (q >= nq) break;
-      add(4 * q + 0, v[q].x);
-      add(4 * q + 1, v[q].y);
-      add(4 * q + 2, v[q].z);
-      add(4 * q + 3, v[q].w);
+      if (q >= nq) continue;
+      add(4 * q + 0, w.v[q].x);
+      add(4 * q + 1, w.v[q].y);
+      add(4 * q + 2, w.v[q].z);
+      add(4 * q + 3, w.v[q].w);
     }
   } else {
-    const uint4* p = g.sh16 + i;
     const int nq = (3 * kmax + 7) / 8;
-    uint4 v[6];
-#pragma unroll
-    for (int q = 0; q < 6; ++q)
-      if (q < nq) v[q] = p[size_t(q) * g.n];
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
-      if (q >= nq) break;
-      add(8 * q + 0, half_lo(v[q].x));
-      add(8 * q + 1, half_hi(v[q].x));
-      add(8 * q + 2, half_lo(v[q].y));
-      add(8 * q + 3, half_hi(v[q].y));
-      add(8 * q + 4, half_lo(v[q].z));
-      add(8 * q + 5, half_hi(v[q].z));
-      add(8 * q + 6, half_lo(v[q].w));
-      add(8 * q + 7, half_hi(v[q].w));
+      if (q >= nq) continue;
+      add(8 * q + 0, half_lo(w.v[q].x));
+      add(8 * q + 1, half_hi(w.v[q].x));
+      add(8 * q + 2, half_lo(w.v[q].y));
+      add(8 * q + 3, half_hi(w.v[q].y));
+      add(8 * q + 4, half_lo(w.v[q].z));
+      add(8 * q + 5, half_hi(w.v[q].z));
+      add(8 * q + 6, half_lo(w.v[q].w));
+      add(8 * q + 7, half_hi(w.v[q].w));
     }
   }
 }
 
-// render.wesl:75-130 for one instance, in pixel space. `world` is
-// model_transform_mat * (pos, 1); `c` the cov3d.
+// The splat record of one instance (render.wesl:75-130), in pixel space, in
+// the functions below. `world` is model_transform_mat * (pos, 1); `c` the
+// cov3d.
 //
 // Unlike the cull and depth-key path (math.hpp, bit-exact with the f32
 // oracle), nothing here feeds a bit-exact output: the record only shapes
@@ -298,35 +357,51 @@ __device__ __forceinline__ void sh_color(const GaussianStreams& g, uint32_t i, f
 // sparsity of the Jacobian J (utils.wesl:38-42), whose third row and two
 // entries are zero: T = J W SR reduces to two rows, cov2d to three dot
 // products.
-template <int POD>
-__device__ __forceinline__ void compute_splat(const FrameConsts& fc, const GaussianStreams& g, uint32_t i,
-                                              const float pos[3], uint32_t color_bits, const float world[4],
-                                              const float c[6], Splat2D* out, float* depth_out,
-                                              uint2* rect_out = nullptr, const TileGrid* rect_grid = nullptr) {
-#pragma clang fp contract(fast)
-  float view[4], proj[4];
-  m4v(fc.view, world, view);
-  m4v(fc.proj, view, proj);
 
-  // color(): render.wesl:58-73
+// color(): render.wesl:58-73 — the view direction in model space and the SH
+// colour, clamped at 0 (`sw` holds the Gaussian's loaded SH words).
+template <int POD>
+__device__ __forceinline__ void splat_colour(const FrameConsts& fc, const ShWords<POD>& sw, const float world[4],
+                                             uint32_t color_bits, float rgb[3]) {
+#pragma clang fp contract(fast)
   const float wvd[3] = {fc.campos[0] - world[0], fc.campos[1] - world[1], fc.campos[2] - world[2]};
   float mvd[3];
-  m3v(fc.inv_sr, wvd, mvd);
+  mv3_fast(fc.inv_sr, wvd, mvd);
   const float il = __builtin_amdgcn_rsqf(mvd[0] * mvd[0] + mvd[1] * mvd[1] + mvd[2] * mvd[2]);
   const float x = -mvd[0] * il, y = -mvd[1] * il, z = -mvd[2] * il;
-  float rgb[3];
 #pragma unroll
   for (int ch = 0; ch < 3; ++ch)
     rgb[ch] = fc.no_sh0 ? 0.5f : float((color_bits >> (8 * ch)) & 0xFFu) * (1.0f / 255.0f);
-  sh_color<POD>(g, i, x, y, z, fc.sh_deg, rgb);
+  sh_fold<POD>(sw, x, y, z, fc.sh_deg, rgb);
 #pragma unroll
   for (int ch = 0; ch < 3; ++ch) rgb[ch] = fmaxf(rgb[ch], 0.0f);
+}
+
+// The rest of the record: centre, axes, bounds and tile rectangle, with the
+// colour from splat_colour.
+template <int POD>
+__device__ __forceinline__ void splat_geometry(const FrameConsts& fc, const float world[4], const float c[6],
+                                               uint32_t color_bits, const float rgb[3], Splat2D* out,
+                                               float* depth_out, uint2* rect_out, const TileGrid* rect_grid) {
+#pragma clang fp contract(fast)
+  // the view-space centre (rows 0-2) and the clip-space centre
+  // (proj * view) * world
+  float view[3], proj[4];
+  mv4_fast<3>(fc.view, world, view);
+  mv4_fast<4>(fc.pv, world, proj);
   float alpha = float(color_bits >> 24) * (1.0f / 255.0f);
 
   const float iw = __builtin_amdgcn_rcpf(proj[3]);
-  // fragment depth: every corner of the quad keeps the centre's clip z and w
-  // (render.wesl:119-125), so the rasterised depth is z / w (IEEE divide)
-  if (depth_out) *depth_out = proj[2] / proj[3];
+  if (depth_out) {
+    // fragment depth: every corner of the quad keeps the centre's clip z and
+    // w (render.wesl:119-125), so the rasterised depth is z / w, here in the
+    // vertex stage's uncontracted association proj * (view * world) and an
+    // IEEE divide (depth tests compare it with other draws' depths)
+    float v4[4], c4[4];
+    m4v(fc.view, world, v4);
+    m4v(fc.proj, v4, c4);
+    *depth_out = c4[2] / c4[3];
+  }
   Splat2D s;
   s.cx = (proj[0] * iw + 1.0f) * 0.5f * fc.size[0];
   s.cy = (1.0f - proj[1] * iw) * 0.5f * fc.size[1];
@@ -398,7 +473,7 @@ __device__ __forceinline__ void compute_splat(const FrameConsts& fc, const Gauss
         const float im = 2.0f * __builtin_amdgcn_rcpf(maj);
         s.ux = dx * im;
         s.uy = -dy * im;
-        s.rho = maj / mnr;
+        s.rho = maj * __builtin_amdgcn_rcpf(mnr);
         hx = 0.5f * sd * __builtin_amdgcn_sqrtf(maj * maj * dx * dx + mnr * mnr * dy * dy);
         hy = 0.5f * sd * __builtin_amdgcn_sqrtf(maj * maj * dy * dy + mnr * mnr * dx * dx);
       }
@@ -410,6 +485,19 @@ __device__ __forceinline__ void compute_splat(const FrameConsts& fc, const Gauss
   s.hxy = drawable ? pack_half2_up(hx, hy) : 0u;
   *out = s;
   if (rect_out) *rect_out = tile_rect(s, fc.max_std_dev * fc.max_std_dev, *rect_grid);
+}
+
+// render.wesl:75-130 for one instance: the SH words are loaded here.
+template <int POD>
+__device__ __forceinline__ void compute_splat(const FrameConsts& fc, const GaussianStreams& g, uint32_t i,
+                                              uint32_t color_bits, const float world[4], const float c[6],
+                                              Splat2D* out, float* depth_out, uint2* rect_out = nullptr,
+                                              const TileGrid* rect_grid = nullptr) {
+  ShWords<POD> sw;
+  sh_load<POD>(g, i, fc.sh_deg, sw);
+  float rgb[3];
+  splat_colour<POD>(fc, sw, world, color_bits, rgb);
+  splat_geometry<POD>(fc, world, c, color_bits, rgb, out, depth_out, rect_out, rect_grid);
 }
 
 }  // namespace gs3d

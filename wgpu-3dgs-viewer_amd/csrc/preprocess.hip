@@ -28,10 +28,10 @@ constexpr int kPreWaves = kPreThreads / kWave;
 #ifndef GS3D_PRE_WAVES_PER_EU
 #define GS3D_PRE_WAVES_PER_EU 4
 #endif
-// 1: every item's covariance words are loaded up front with its position;
-// 0: they are loaded where the splat record needs them (fewer registers)
-#ifndef GS3D_PRE_PREFETCH_COV
-#define GS3D_PRE_PREFETCH_COV 0
+// 1: the next visible item's covariance and SH words are requested before
+// the current record's geometry and stores; 0: after them
+#ifndef GS3D_PRE_PIPELINE
+#define GS3D_PRE_PIPELINE 1
 #endif
 constexpr int kPreItems = GS3D_PRE_ITEMS;
 constexpr uint32_t kPreTile = kPreThreads * kPreItems;
@@ -136,11 +136,8 @@ __global__ __launch_bounds__(kPreThreads) __attribute__((amdgpu_waves_per_eu(GS3
   float key[kPreItems];
   uint64_t masks[kPreItems];
 
-  // issue every item's position and covariance loads first: both are needed
-  // by every Gaussian in range (the covariance for the bound re-test of
-  // off-screen centres or for the splat of visible ones)
+  // issue every item's position load first
   float4 pcs[kPreItems];
-  CovWords cws[kPreItems];
   bool act[kPreItems];
 #pragma unroll
   for (int k = 0; k < kPreItems; ++k) {
@@ -151,10 +148,7 @@ __global__ __launch_bounds__(kPreThreads) __attribute__((amdgpu_waves_per_eu(GS3
       const bool bit = ((a.selection[i >> 5] >> (i & 31u)) & 1u) != 0u;
       act[k] = bit != (a.invert_selection != 0u);
     }
-    if (act[k]) {
-      pcs[k] = a.g.pc[i];
-      if (GS3D_PRE_PREFETCH_COV) cws[k] = load_cov_words<POD>(a.g, i);
-    }
+    if (act[k]) pcs[k] = a.g.pc[i];
   }
 
   // A
@@ -214,38 +208,59 @@ __global__ __launch_bounds__(kPreThreads) __attribute__((amdgpu_waves_per_eu(GS3
     status_store(a.status + tile, (tile == 0 ? kFlagInclusive : kFlagAggregate) | run);
   }
 
-  // C: the splat records of the visible items
+  // C: the splat records of the visible items. An item's covariance and SH
+  // words are requested as soon as the previous item's SH has been folded
+  // into its colour, so that their latency overlaps the rest of that record
+  // and its stores (requested after the stores, they would wait behind them).
+  ShWords<POD> sw;
+  CovWords cw;
+  if (vis[0]) {
+    cw = load_cov_words<POD>(a.g, base + tid);
+    sh_load<POD>(a.g, base + tid, fc.sh_deg, sw);
+  }
 #pragma unroll
   for (int k = 0; k < kPreItems; ++k) {
+    const uint32_t i = base + uint32_t(k) * kPreThreads + tid;
+    const uint32_t cbits = __float_as_uint(pcs[k].w);
+    float world[4], c[6], rgb[3];
     if (vis[k]) {
-      const uint32_t i = base + uint32_t(k) * kPreThreads + tid;
-      const float4 pc = pcs[k];
-      const float pos[3] = {pc.x, pc.y, pc.z};
-      const float p4[4] = {pc.x, pc.y, pc.z, 1.0f};
-      float world[4], c[6];
-      m4v(fc.model, p4, world);
-      cov3d_of<POD>(GS3D_PRE_PREFETCH_COV ? cws[k] : load_cov_words<POD>(a.g, i), c);
-      compute_splat<POD>(fc, a.g, i, pos, __float_as_uint(pc.w), world, c, a.splats + i,
-                         a.splat_depth ? a.splat_depth + i : nullptr, a.rects ? a.rects + i : nullptr,
-                         &a.rect_grid);
+      const float p4[4] = {pcs[k].x, pcs[k].y, pcs[k].z, 1.0f};
+      m4v(fc.model, p4, world);  // uncontracted: the fragment depth derives from it
+      cov3d_of<POD, true>(cw, c);
+      splat_colour<POD>(fc, sw, world, cbits, rgb);
+    }
+    if (GS3D_PRE_PIPELINE && k + 1 < kPreItems && vis[k + 1]) {
+      cw = load_cov_words<POD>(a.g, i + kPreThreads);
+      sh_load<POD>(a.g, i + kPreThreads, fc.sh_deg, sw);
+    }
+    if (vis[k])
+      splat_geometry<POD>(fc, world, c, cbits, rgb, a.splats + i, a.splat_depth ? a.splat_depth + i : nullptr,
+                          a.rects ? a.rects + i : nullptr, &a.rect_grid);
+    if (!GS3D_PRE_PIPELINE && k + 1 < kPreItems && vis[k + 1]) {
+      cw = load_cov_words<POD>(a.g, i + kPreThreads);
+      sh_load<POD>(a.g, i + kPreThreads, fc.sh_deg, sw);
     }
   }
 
-  if (tid == 0) {
+  if (w == 0) {
+    // the first wave looks back (run is meaningful in thread 0)
+    run = __shfl(run, 0, kWave);
     uint32_t excl = 0;
     if (tile > 0) {
-      excl = lookback_exclusive(a.status, tile, a.err);
-      status_store(a.status + tile, kFlagInclusive | (excl + run));
+      excl = lookback_exclusive_wave(a.status, tile, a.err);
+      if (lane == 0) status_store(a.status + tile, kFlagInclusive | (excl + run));
     }
-    s_base = excl;
-    s_total = excl + run;
-    if (tile == a.num_tiles - 1) {
-      // preprocess.wesl:108-116 (post): visible count and sort dispatch size
-      const uint32_t v = excl + run;
-      a.indirect_args[1] = v;
-      a.dispatch_args[0] = (v + kHistoBlockKvs - 1u) / kHistoBlockKvs;
-      a.dispatch_args[1] = 1u;
-      a.dispatch_args[2] = 1u;
+    if (lane == 0) {
+      s_base = excl;
+      s_total = excl + run;
+      if (tile == a.num_tiles - 1) {
+        // preprocess.wesl:108-116 (post): visible count and sort dispatch size
+        const uint32_t v = excl + run;
+        a.indirect_args[1] = v;
+        a.dispatch_args[0] = (v + kHistoBlockKvs - 1u) / kHistoBlockKvs;
+        a.dispatch_args[1] = 1u;
+        a.dispatch_args[2] = 1u;
+      }
     }
   }
   __syncthreads();
@@ -296,12 +311,11 @@ __global__ __launch_bounds__(256) void k_vertex(const FrameConsts* __restrict__ 
     const uint32_t i = indices[s];
     if (i >= g.n) continue;
     const float4 pc = g.pc[i];
-    const float pos[3] = {pc.x, pc.y, pc.z};
     const float p4[4] = {pc.x, pc.y, pc.z, 1.0f};
     float world[4], c[6];
-    m4v(fc.model, p4, world);
-    cov3d_of<POD>(load_cov_words<POD>(g, i), c);
-    compute_splat<POD>(fc, g, i, pos, __float_as_uint(pc.w), world, c, splats + i,
+    m4v(fc.model, p4, world);  // uncontracted: the fragment depth derives from it
+    cov3d_of<POD, true>(load_cov_words<POD>(g, i), c);
+    compute_splat<POD>(fc, g, i, __float_as_uint(pc.w), world, c, splats + i,
                        splat_depth ? splat_depth + i : nullptr);
   }
 }

@@ -34,6 +34,11 @@ constexpr int kRsItems = GS3D_SORT_ITEMS;
 #define GS3D_SORT_WG_PER_CU 4
 #endif
 constexpr uint32_t kRsTile = kRsThreads * kRsItems;
+// predecessors' look-back words requested per step of a digit's look-back
+#ifndef GS3D_SORT_LOOK_WINDOW
+#define GS3D_SORT_LOOK_WINDOW 4
+#endif
+constexpr int kRsLookWindow = GS3D_SORT_LOOK_WINDOW;
 
 __device__ __forceinline__ uint32_t sort_n(const SortCount& c) {
   const uint64_t v = uint64_t(*c.count) * uint64_t(c.count_mul);
@@ -204,24 +209,37 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_onesweep(SortCount cnt, const
     }
 
     // global offset of this tile's run of digit d
+    // (each step requests the words of the next kRsLookWindow predecessors
+    // at once and consumes them in order up to an inclusive prefix or a
+    // predecessor that has not published yet)
     uint32_t excl = 0;
     if (t > 0) {
       int64_t j = int64_t(t) - 1;
       uint32_t spins = 0;
-      while (j >= 0) {
-        const uint32_t s = status_load(lookback + size_t(j) * 256u + d);
-        const uint32_t flag = s & kFlagMask;
-        if (flag == 0u) {
+      bool done = false;
+      while (!done && j >= 0) {
+        uint32_t sw[kRsLookWindow];
+#pragma unroll
+        for (int q = 0; q < kRsLookWindow; ++q)
+          sw[q] = j - q >= 0 ? status_load(lookback + size_t(j - q) * 256u + d) : kFlagInclusive;
+        int adv = 0;
+#pragma unroll
+        for (int q = 0; q < kRsLookWindow; ++q) {
+          if (done || adv < q) continue;  // stopped at an earlier word
+          const uint32_t flag = sw[q] & kFlagMask;
+          if (flag == 0u) continue;  // not published: resume here
+          excl += sw[q] & kValueMask;
+          ++adv;
+          done = flag == kFlagInclusive;
+        }
+        j -= adv;
+        if (adv == 0) {
           __builtin_amdgcn_s_sleep(1);
           if (++spins > kSpinLimit) {
             atomicOr(err, kErrLookbackSpin);
             break;
           }
-          continue;
         }
-        excl += s & kValueMask;
-        if (flag == kFlagInclusive) break;
-        --j;
       }
       status_store(lookback + size_t(t) * 256u + d, kFlagInclusive | (excl + pub));
     }

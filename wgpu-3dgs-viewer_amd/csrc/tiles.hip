@@ -101,19 +101,22 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(BinArgs a) {
     __syncthreads();
     run += s_ktot[k];
   }
-  if (tid == 0) {
+  if (tid < kWave) {
+    // the first wave looks back (run is the same in every thread)
     uint32_t e = 0;
     if (t == 0) {
-      status_store(a.status, kFlagInclusive | run);
+      if (tid == 0) status_store(a.status, kFlagInclusive | run);
     } else {
-      status_store(a.status + t, kFlagAggregate | run);
-      e = lookback_exclusive(a.status, t, a.err);
-      status_store(a.status + t, kFlagInclusive | (e + run));
+      if (tid == 0) status_store(a.status + t, kFlagAggregate | run);
+      e = lookback_exclusive_wave(a.status, t, a.err);
+      if (tid == 0) status_store(a.status + t, kFlagInclusive | (e + run));
     }
-    s_base = e;
-    if (t == num_tiles - 1) {
-      *a.pair_count = e + run;
-      if (e + run > a.capacity) atomicOr(a.err, kErrTileOverflow);
+    if (tid == 0) {
+      s_base = e;
+      if (t == num_tiles - 1) {
+        *a.pair_count = e + run;
+        if (e + run > a.capacity) atomicOr(a.err, kErrTileOverflow);
+      }
     }
   }
   __syncthreads();
