@@ -25,7 +25,7 @@ Sections (all run by default; --sections picks a subset):
 Every frame runs the whole path: preprocess (cull + projection + depth keys)
 -> depth radix sort -> tile binning -> tile sort -> compositing. Gaussians are
 resident in HBM before any timed region. One JSON line on stdout; progress
-on stderr. --gpus N > 1 exits: this build runs on one GPU.
+on stderr. --gpus N > 1 exits: the viewer runs on one GPU.
 """
 from __future__ import annotations
 
@@ -294,7 +294,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); run with --gpus 1",
+        print(f"bench.py: the viewer runs on one GPU only (asked for {args.gpus}); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
     sections = [s for s in args.sections.split(",") if s]

@@ -1,7 +1,7 @@
 /*
  * CPU oracle for the per-frame splat path of wgpu-3dgs-viewer 0.6.1.
  *
- * TEST INFRASTRUCTURE ONLY. This library is the checker: only tests/,
+ * TEST INFRASTRUCTURE ONLY. Tests compare the product against it: only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it. The
  * product (libgs3d.so) never links, loads or calls it.
  *
