@@ -53,9 +53,11 @@ R_OUT = 48
 # Depth sort: 4 passes x (read key+value 8, write key+value 8) per key, plus
 # the one histogram read of the keys (4).
 SORT_BYTES_PER_KEY = 68
-# Compositing: floating-point operations per blended fragment (r^2 from the
-# two axis terms 9, exp2 1, alpha 1, weight 1, colour 6, transmittance 2).
-FLOPS_PER_FRAGMENT = 20
+# Compositing: floating-point operations per blended fragment, a fused
+# multiply-add counted as 2 (each axis term p.x u'.x + p.y u'.y - k: 2 FMAs,
+# 4; r^2 from the two terms: 3; alpha exponent 1; exp2 1; weight 1; colour
+# 3 FMAs, 6; transmittance 1 FMA, 2).
+FLOPS_PER_FRAGMENT = 22
 # HBM bytes per launch on this workload: rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE, each scaled by what the same counter reads for a 1 GiB copy
 # (x2.000 and x1.000); profiles/traffic_clustered-6M-1440p-f16_plain.md.
