@@ -29,9 +29,10 @@ constexpr int kRsWaves = kRsThreads / kWave;
 #define GS3D_SORT_ITEMS 16
 #endif
 constexpr int kRsItems = GS3D_SORT_ITEMS;
-// resident onesweep workgroups per CU (the persistent grid)
+// onesweep workgroups per CU (the persistent grid): 3 is what is resident
+// at the kernel's ~150 VGPRs (3 waves per SIMD); more only queue behind them
 #ifndef GS3D_SORT_WG_PER_CU
-#define GS3D_SORT_WG_PER_CU 4
+#define GS3D_SORT_WG_PER_CU 3
 #endif
 constexpr uint32_t kRsTile = kRsThreads * kRsItems;
 // predecessors' look-back words requested per step of a digit's look-back
