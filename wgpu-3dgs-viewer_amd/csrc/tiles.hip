@@ -156,14 +156,38 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ pai
     if (host_total) *host_total = m > total ? m : total;
   }
   const uint32_t p = total < capacity ? total : capacity;
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < p; j += gridDim.x * blockDim.x) {
-    const uint32_t k = keys[j];
-    if (k >= num_tiles) {  // never written by k_bin: report, do not index
-      atomicOr(err, kErrTileKey);
-      continue;
+  // four consecutive keys per thread and step (one 16-byte load, plus the
+  // neighbours either side), so few dependent steps cover the pairs
+  constexpr uint32_t kNone = 0xFFFFFFFFu;  // before the first / after the last key
+  const uint32_t quads = (p + 3u) / 4u;
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < quads; q += gridDim.x * blockDim.x) {
+    const uint32_t j0 = q * 4u;
+    uint32_t k[4];
+    if (j0 + 4u <= p) {
+      const uint4 v = *reinterpret_cast<const uint4*>(keys + j0);
+      k[0] = v.x;
+      k[1] = v.y;
+      k[2] = v.z;
+      k[3] = v.w;
+    } else {
+#pragma unroll
+      for (uint32_t t = 0; t < 4u; ++t) k[t] = j0 + t < p ? keys[j0 + t] : kNone;
     }
-    if (j == 0 || keys[j - 1] != k) ranges[2 * k] = j;
-    if (j == p - 1 || keys[j + 1] != k) ranges[2 * k + 1] = j + 1;
+    const uint32_t prev = j0 > 0u ? keys[j0 - 1u] : kNone;
+    const uint32_t next = j0 + 4u < p ? keys[j0 + 4u] : kNone;
+#pragma unroll
+    for (uint32_t t = 0; t < 4u; ++t) {
+      const uint32_t j = j0 + t;
+      if (j >= p) break;
+      if (k[t] >= num_tiles) {  // never written by k_bin: report, do not index
+        atomicOr(err, kErrTileKey);
+        continue;
+      }
+      const uint32_t before = t == 0u ? prev : k[t - 1u];
+      const uint32_t after = j + 1u < p ? (t < 3u ? k[t + 1u] : next) : kNone;
+      if (before != k[t]) ranges[2u * k[t]] = j;
+      if (after != k[t]) ranges[2u * k[t] + 1u] = j + 1u;
+    }
   }
 }
 
@@ -186,7 +210,7 @@ void launch_bin(hipStream_t st, const BinArgs& a) {
 void launch_ranges(hipStream_t st, const uint32_t* pair_count, uint32_t capacity, const uint32_t* keys,
                    uint32_t* ranges, uint32_t num_tiles, int num_cus, uint32_t* pair_max, uint32_t* host_total,
                    uint32_t* err) {
-  uint32_t blocks = (capacity + 255u) / 256u;
+  uint32_t blocks = (capacity / 4u + 255u) / 256u;
   const uint32_t cap = uint32_t(num_cus) * 8u;
   if (blocks > cap) blocks = cap;
   if (blocks < 1u) blocks = 1u;
