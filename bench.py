@@ -64,8 +64,8 @@ FLOPS_PER_FRAGMENT = 22
 # k_preprocess: the Gaussian streams plus the sector overhead of the scattered
 # per-Gaussian splat/rectangle writes; k_composite_fast: mostly the 32-byte
 # splat records gathered per (tile, splat) pair, one 128-byte line each.
-PREPROCESS_TRAFFIC_BYTES = 1101345792
-COMPOSITE_TRAFFIC_BYTES = 942096384
+PREPROCESS_TRAFFIC_BYTES = 1103224992
+COMPOSITE_TRAFFIC_BYTES = 942950304
 
 MIX_RES = [(640, 360), (1280, 720), (1920, 1080), (3840, 2160)]
 
