@@ -40,11 +40,27 @@
  *                           from scipy.special.sph_harm
  *   orc_stable_sort         pinned against numpy.argsort(kind="stable")
  *   orc_render_*            pinned by the closed-form single-splat image and
- *                           the two-splat over-operator
- * The wgpu-3dgs-core 0.6.0 helpers (Gaussian packing, model transform
- * matrices, flag decoding) are absent from the reference tree: their
- * restatement here is "parity unpinned" against the reference itself and is
- * pinned only by the closed forms above.
+ *                           the two-splat over-operator; the f64 twin by the
+ *                           same closed form
+ *   orc_model_mats          pinned by model (p,1) = R(q) diag(s) p + t with the
+ *                           textbook rotation matrix in float64, a known
+ *                           90-degree rotation, and inv_sr sr = I
+ *   orc_splats_* (colour)   the view direction fed to view_color pinned by
+ *                           axis-aligned cameras (one degree-1 term each) and
+ *                           by numpy's inv(R S) (campos - world) under random
+ *                           model transforms
+ *   orc_splats_* (geometry) the pixel-space footprint pinned against an EWA
+ *                           footprint built from the projection's numerical
+ *                           Jacobian (pixel y down) for tilted 10:1 needles
+ *   display modes           ellipse ring and interior and the point quad's
+ *                           half extent 0.01 size H / (2 |view|) pinned by
+ *                           closed forms
+ *   orc_frame_f32           equal to the composition of the pinned stages
+ * Each of these checks was shown to fail on planted slips (a swapped or
+ * transposed factor, a dropped sign or y flip, a wrong constant).
+ * The wgpu-3dgs-core 0.6.0 helpers (Gaussian packing, flag decoding) are
+ * absent from the reference tree: their restatement here is "parity unpinned"
+ * against the reference itself and is pinned only by the closed forms above.
  */
 #ifndef GS3D_ORACLE_H
 #define GS3D_ORACLE_H

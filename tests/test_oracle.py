@@ -420,3 +420,302 @@ def test_depth_write_follows_draw_order(orc):
     d3 = np.ones((32, 32), np.float32)
     img3 = orc.render(frame, 32, 32, sp, orc.TARGET_RGBA32F, depth=d3, compare=2, depth_write=True)
     assert np.array_equal(img3, orc.render(frame, 32, 32, sp, orc.TARGET_RGBA32F))
+
+
+# --------------------------------------------------------------------------
+# model transform (core's model_transform_mat / model_scale_rot_mat /
+# model_transform_inv_sr_mat, called at utils.wesl:37, render.wesl:65,85;
+# fed by Viewer::update_model_transform, src/lib.rs:217-225)
+# --------------------------------------------------------------------------
+
+def _rand_unit_quat(rng):
+    q = rng.normal(size=4)
+    return q / np.linalg.norm(q)
+
+
+def test_model_mats_are_translate_rotate_scale(orc):
+    # model (p, 1) = R(q) diag(s) p + t, built from the textbook rotation
+    # matrix in float64; inv_sr is the inverse of sr. Non-uniform scales make
+    # R S and S R differ, so a swapped or transposed factor fails.
+    rng = np.random.default_rng(7)
+    for _ in range(25):
+        q = _rand_unit_quat(rng)
+        s = rng.uniform(0.2, 3.0, 3)
+        t = rng.uniform(-5, 5, 3)
+        model, sr, inv_sr = (m.astype(np.float64) for m in orc.model_mats(t, q, s))
+        r = _quat_to_mat(q)
+        want_sr = r @ np.diag(s)
+        assert np.allclose(sr, want_sr, atol=2e-6 * s.max())
+        for _k in range(4):
+            p = rng.uniform(-3, 3, 3)
+            assert np.allclose(model @ np.append(p, 1.0), np.append(want_sr @ p + t, 1.0), atol=2e-5 * (1 + s.max()))
+        assert np.allclose(inv_sr @ sr, np.eye(3), atol=5e-6 * s.max() / s.min())
+        assert np.allclose(inv_sr, np.linalg.inv(want_sr), rtol=0, atol=5e-6 / s.min())
+
+
+def test_model_mats_known_rotation(orc):
+    # 90 degrees about +Z (q = (0, 0, sin 45, cos 45)): x -> y, y -> -x
+    h = math.sqrt(0.5)
+    model, sr, inv_sr = orc.model_mats((1.0, 2.0, 3.0), (0.0, 0.0, h, h), (2.0, 3.0, 4.0))
+    assert np.allclose(model @ [1, 0, 0, 1], [1.0, 4.0, 3.0, 1.0], atol=1e-6)   # 2 x -> (0, 2, 0), + t
+    assert np.allclose(model @ [0, 1, 0, 1], [-2.0, 2.0, 3.0, 1.0], atol=1e-6)  # 3 y -> (-3, 0, 0), + t
+    assert np.allclose(model @ [0, 0, 1, 1], [1.0, 2.0, 7.0, 1.0], atol=1e-6)   # 4 z
+    assert np.allclose(inv_sr @ [0.0, 2.0, 0.0], [1.0, 0.0, 0.0], atol=1e-6)
+
+
+def _model_frame(orc, view, proj, size, t, q, s, **kw):
+    model, sr, inv_sr = orc.model_mats(t, q, s)
+    return orc.make_frame(view, proj, size, model=model, sr=sr, inv_sr=inv_sr, **kw)
+
+
+def test_preprocess_applies_model_transform(orc):
+    # model_to_world (preprocess.wesl:82): a point translated and rotated by
+    # the model transform has the depth key of its world position
+    view, proj, _ = _camera(orc, eye=(0, 0, 0), target=(0, 0, -1))
+    h = math.sqrt(0.5)
+    frame = _model_frame(orc, view, proj, (1280, 720), (0.0, 0.0, -4.0), (0.0, h, 0.0, h), (1.0, 1.0, 2.0))
+    # model-space x = 1 rotates (90 deg about +Y) to world -z, then t
+    pos = np.array([[1.0, 0.0, 0.0], [-1.0, 0.0, 0.0], [0.0, 0.0, 3.0]], np.float32)
+    cov = np.tile(np.array([1e-6, 0, 0, 1e-6, 0, 1e-6], np.float32), (3, 1))
+    idx, key = orc.preprocess(frame, pos, cov)
+    # world: (0, 0, -5), (0, 0, -3), (2 * 3 = 6 along +x ... ) -> (6, 0, -4): off screen
+    assert idx.tolist() == [0, 1]
+    n, f = 0.1, 1e4
+    for k, d in zip(idx, (5.0, 3.0)):
+        assert key[k] == pytest.approx(1.0 - f / (f - n) * (1.0 - n / d), rel=1e-5)
+
+
+# --------------------------------------------------------------------------
+# view direction fed to view_color (render.wesl:58-73)
+# --------------------------------------------------------------------------
+
+def _one_splat_color(orc, frame, pos, coeff_index, value, color=(128, 128, 128, 255)):
+    sh = np.zeros((1, 15, 3), np.float32)
+    sh[0, coeff_index, :] = value
+    cov = np.array([[1e-4, 0, 0, 1e-4, 0, 1e-4]], np.float32)
+    sp = orc.splats(frame, np.array([0], np.uint32), np.array([pos], np.float32), np.array([color], np.uint8),
+                    sh.reshape(1, 45), cov, f64=False)
+    return np.array([sp["r"][0], sp["g"][0], sp["b"][0]], np.float64)
+
+
+def test_view_direction_straight_ahead(orc):
+    # The direction view_color sees is normalize(world - camera) in model
+    # space. Looking down a world axis at a splat straight ahead, only the
+    # degree-1 term of that axis survives: c1 (-y sh0 + z sh1 - x sh2)
+    # (utils.wesl:100-102), with c1 = 0.4886025.
+    c1, base, val = 0.4886025, 128 / 255, 0.2
+    cases = [  # (eye, target, splat, coefficient, expected sign of its term)
+        ((0, 0, 0), (0, 0, -1), (0, 0, -2), 1, -1.0),   # dir -z: + z sh1 -> -sh1
+        ((0, 0, 0), (0, 0, 1), (0, 0, 2), 1, +1.0),     # dir +z
+        ((0, 0, 0), (1, 0, 0), (2, 0, 0), 2, -1.0),     # dir +x: - x sh2
+        ((0, 0, 0), (-1, 0, 0), (-2, 0, 0), 2, +1.0),   # dir -x
+        ((0, 5, 0), (0, 0, 0.001), (0, 0, 0), 0, +1.0),  # dir -y: - y sh0 -> +sh0
+    ]
+    for eye, target, p, k, sign in cases:
+        view, proj, frame = _camera(orc, eye=eye, target=target, sh_deg=1)
+        got = _one_splat_color(orc, frame, p, k, val)
+        assert np.allclose(got, base + sign * c1 * val, atol=2e-6), (eye, target, k, got)
+        # the other two degree-1 coefficients contribute nothing on this axis
+        for other in {0, 1, 2} - {k}:
+            assert np.allclose(_one_splat_color(orc, frame, p, other, val), base, atol=2e-6)
+
+
+def test_view_direction_uses_inverse_model_scale_rotation(orc):
+    # render.wesl:64-65: dir = -normalize(inv(R S) (campos - world)), against
+    # numpy's inverse in float64, for random model transforms and cameras,
+    # read through a degree-1-only SH (colour = 0.5 + c1 (-y s0 + z s1 - x s2)).
+    rng = np.random.default_rng(5)
+    c1 = 0.4886025
+    for _ in range(20):
+        q, s, t = _rand_unit_quat(rng), rng.uniform(0.5, 2.0, 3), rng.uniform(-1, 1, 3)
+        eye = rng.uniform(-6, 6, 3)
+        view, proj, _ = _camera(orc, eye=tuple(eye), target=(0, 0, 0))
+        frame = _model_frame(orc, view, proj, (1280, 720), t, q, s, sh_deg=1, no_sh0=True)
+        p = rng.uniform(-1, 1, 3)
+        sr = _quat_to_mat(q) @ np.diag(s)
+        world = sr @ p + t
+        d = np.linalg.inv(sr) @ (world - eye)
+        d /= np.linalg.norm(d)
+        sh = np.zeros((1, 15, 3), np.float32)
+        sh[0, 0, 0], sh[0, 1, 1], sh[0, 2, 2] = 0.1, 0.1, 0.1  # r reads y, g reads z, b reads x
+        cov = np.array([[1e-4, 0, 0, 1e-4, 0, 1e-4]], np.float32)
+        sp = orc.splats(frame, np.array([0], np.uint32), np.array([p], np.float32),
+                        np.array([[0, 0, 0, 255]], np.uint8), sh.reshape(1, 45), cov)
+        got = np.array([sp["r"][0], sp["g"][0], sp["b"][0]], np.float64)
+        want = np.maximum(0.5 + 0.1 * c1 * np.array([-d[1], d[2], -d[0]]), 0.0)
+        assert np.allclose(got, want, atol=5e-6), (got, want)
+
+
+# --------------------------------------------------------------------------
+# footprint of a splat in pixel space (render.wesl:106-151, camera.wesl:18-20)
+# --------------------------------------------------------------------------
+
+def _pixel_of(view, proj, size, world):
+    clip = proj.astype(np.float64) @ (view.astype(np.float64) @ np.append(world, 1.0))
+    ndc = clip[:2] / clip[3]
+    return np.array([(ndc[0] + 1) * 0.5 * size[0], (1 - ndc[1]) * 0.5 * size[1]])
+
+
+def _expected_footprint(view, proj, size, world, cov3, sd, opacity):
+    """EWA footprint restated from first principles in float64: the pixel
+    Jacobian of the projection at the centre by central differences (pixel y
+    pointing down), Sigma_px = J Sigma J^T, and the shader's falloff: the quad
+    corner +-sd reaches sd sqrt(lambda) * sd / 2 px along each axis, so
+    |qo|^2 = (4 / sd^2) d^T Sigma_px^-1 d, alpha = a exp(-|qo|^2), discarded
+    past |qo|^2 = sd^2."""
+    eps = 1e-4
+    jac = np.zeros((2, 3))
+    for k in range(3):
+        e = np.zeros(3)
+        e[k] = eps
+        jac[:, k] = (_pixel_of(view, proj, size, world + e) - _pixel_of(view, proj, size, world - e)) / (2 * eps)
+    sig = jac @ cov3 @ jac.T
+    c = _pixel_of(view, proj, size, world)
+    w, h = size
+    ys, xs = np.mgrid[0:h, 0:w]
+    d = np.stack([xs + 0.5 - c[0], ys + 0.5 - c[1]], -1)
+    m = np.einsum("...i,ij,...j->...", d, np.linalg.inv(sig), d)
+    r2 = 4.0 / (sd * sd) * m
+    return np.where(r2 <= sd * sd, opacity * np.exp(-r2), 0.0), r2, c, sig
+
+
+@pytest.mark.parametrize("tilt", [45.0, -30.0, 120.0])
+def test_anisotropic_footprint_orientation(orc, tilt):
+    # A 10:1 needle tilted in the image plane, straight ahead and off axis:
+    # every pixel matches the first-principles footprint, so a mirrored
+    # (un-flipped y), transposed or rotated ellipse fails.
+    w = h = 96
+    view, proj, frame = _camera(orc, eye=(0, 0, 0), target=(0, 0, -1), w=w, h=h)
+    a = math.radians(tilt)
+    q = (0.0, 0.0, math.sin(a / 2), math.cos(a / 2))  # rotation about the view axis
+    scale = (0.2, 0.02, 0.02)
+    cov = orc.cov3d_f32(np.array([scale]), np.array([q]))
+    for p in ((0.0, 0.0, -2.0), (0.3, -0.2, -2.5)):
+        sp = orc.splats(frame, np.array([0], np.uint32), np.array([p], np.float32),
+                        np.array([[255, 255, 255, 255]], np.uint8), np.zeros((1, 45), np.float32), cov)
+        img = orc.render(frame, w, h, sp, orc.TARGET_RGBA32F)
+        cov3 = cov[0].astype(np.float64)
+        cov3m = np.array([[cov3[0], cov3[1], cov3[2]], [cov3[1], cov3[3], cov3[4]], [cov3[2], cov3[4], cov3[5]]])
+        want, r2, c, sig = _expected_footprint(view, proj, (w, h), np.array(p), cov3m, 3.0, 1.0)
+        assert np.allclose([sp["cx"][0], sp["cy"][0]], c, atol=1e-3)
+        # pixels whose |qo|^2 lies at the discard edge may go either way in f32
+        edge = np.abs(r2 - 9.0) < 1e-3
+        assert np.allclose(img[..., 0][~edge], want[~edge], atol=2e-4)
+        # the major axis runs along the direction the tilt fixes on screen:
+        # world (cos a, sin a) appears at pixel (cos a, -sin a)
+        evals, evecs = np.linalg.eigh(sig)
+        major = evecs[:, 1]
+        assert abs(abs(major @ [math.cos(a), -math.sin(a)]) - 1.0) < 2e-3
+        lit = img[..., 0] > 0.05
+        assert lit.sum() > 10
+        ys, xs = np.nonzero(lit)
+        dd = np.stack([xs + 0.5 - c[0], ys + 0.5 - c[1]], 1)
+        assert np.abs(dd @ [math.sin(a), math.cos(a)]).max() < 3.0  # a thin band across the minor axis
+        assert np.abs(dd @ [math.cos(a), -math.sin(a)]).max() > 15.0  # and long along the major one
+
+
+# --------------------------------------------------------------------------
+# ellipse and point display modes (render.wesl:92-104, 153-166)
+# --------------------------------------------------------------------------
+
+def test_ellipse_mode_outline_ring(orc):
+    # alpha = a inside (sd - 0.1)^2, 1 on the ring (sd - 0.1)^2 < |qo|^2 <= sd^2,
+    # discarded outside; with a = 0.2 the three regions give c 0.2, c, 0
+    w = h = 128
+    view, proj, _ = _camera(orc, eye=(0, 0, 0), target=(0, 0, -1), w=w, h=h)
+    sd = 3.0
+    frame = orc.make_frame(view, proj, (w, h), display_mode=orc.MODE_ELLIPSE, max_std_dev=sd)
+    s, z = 0.15, 2.0
+    cov3 = np.array([[s * s, 0, 0, s * s, 0, 2.0 * s * s]], np.float32)
+    sp = orc.splats(frame, np.array([0], np.uint32), np.array([[0, 0, -z]], np.float32),
+                    np.array([[255, 255, 255, 51]], np.uint8), np.zeros((1, 45), np.float32), cov3)
+    img = orc.render(frame, w, h, sp, orc.TARGET_RGBA32F)
+    cm = np.diag([s * s, s * s, 2.0 * s * s]).astype(np.float64)
+    _, r2, _, _ = _expected_footprint(view, proj, (w, h), np.array([0.0, 0.0, -z]), cm, sd, 1.0)
+    e2 = (sd - 0.1) ** 2
+    firm = (np.abs(r2 - sd * sd) > 1e-3) & (np.abs(r2 - e2) > 1e-3)
+    want = np.where(r2 > sd * sd, 0.0, np.where(r2 > e2, 1.0, 0.2))
+    assert np.allclose(img[..., 0][firm], want[firm], atol=1e-6)
+    assert (want == 1.0).sum() > 20 and (want == 0.2).sum() > 100  # both regions are present
+
+
+def test_point_mode_quad_extent(orc):
+    # quad offset +-0.01 size, scaled by (aspect, 1) / |view| in clip space
+    # (render.wesl:92-96, aspect = H / W, camera.wesl:8-10): a square of
+    # half side 0.01 size H / (2 |view|) pixels, colour at alpha 1
+    w, h = 200, 100
+    view, proj, _ = _camera(orc, eye=(0, 0, 0), target=(0, 0, -1), w=w, h=h)
+    size, z = 40.0, 2.0
+    frame = orc.make_frame(view, proj, (w, h), display_mode=orc.MODE_POINT, gsize=size)
+    sp = orc.splats(frame, np.array([0], np.uint32), np.array([[0, 0, -z]], np.float32),
+                    np.array([[255, 0, 0, 10]], np.uint8), np.zeros((1, 45), np.float32),
+                    np.array([[1e-4, 0, 0, 1e-4, 0, 1e-4]], np.float32))
+    half = 0.01 * size * h / (2.0 * z)  # 10 px
+    assert sp["hx"][0] == pytest.approx(half, rel=1e-6) and sp["hy"][0] == pytest.approx(half, rel=1e-6)
+    img = orc.render(frame, w, h, sp, orc.TARGET_RGBA32F)
+    lit = img[..., 0] > 0
+    ys, xs = np.nonzero(lit)
+    assert lit.sum() == int(2 * half) ** 2
+    assert xs.min() == w // 2 - 10 and xs.max() == w // 2 + 9 and ys.min() == h // 2 - 10 and ys.max() == h // 2 + 9
+    assert np.all(img[..., 0][lit] == 1.0)  # alpha 1 whatever the opacity
+    # farther away the square shrinks as 1 / |view|
+    sp2 = orc.splats(frame, np.array([0], np.uint32), np.array([[0, 0, -2 * z]], np.float32),
+                     np.array([[255, 0, 0, 10]], np.uint8), np.zeros((1, 45), np.float32),
+                     np.array([[1e-4, 0, 0, 1e-4, 0, 1e-4]], np.float32))
+    assert sp2["hx"][0] == pytest.approx(half / 2, rel=1e-6)
+
+
+# --------------------------------------------------------------------------
+# remaining oracle entry points: cov2d, the f64 twins, whole frame
+# --------------------------------------------------------------------------
+
+def test_cov2d_matches_float64(gs, orc):
+    view, proj, frame = _camera(orc, eye=(0.5, 0.3, 3.0), target=(0, 0, 0))
+    g = gs.synth_gaussians(0, 100, seed=4)
+    cov = orc.cov3d_f32(g["scale"], g["rot"]) * np.float32(50.0)
+    for i in range(len(g)):
+        c2 = _cov2d_f64(view, proj, (1280, 720), g["pos"][i], cov[i])
+        got = orc.cov2d(frame, g["pos"][i], cov[i]).astype(np.float64)
+        scale = np.abs(np.diag(c2)).max()
+        assert np.allclose(got, [c2[0, 0], c2[0, 1], c2[1, 1]], rtol=0, atol=2e-5 * scale)
+
+
+def test_f64_twins_agree_with_f32(gs, orc):
+    view, proj, frame = _camera(orc, eye=(0.5, 0.3, 3.0), target=(0, 0, 0))
+    g = gs.synth_gaussians(0, 200, seed=9)
+    c32 = orc.cov3d_f32(g["scale"], g["rot"])
+    c64 = orc.cov3d_f64(g["scale"].astype(np.float64), g["rot"].astype(np.float64))
+    dscale = np.abs(c64[:, [0, 3, 5]]).max(1, keepdims=True)  # off-diagonals cancel: error relative to the diagonal
+    assert np.all(np.abs(c32 - c64) <= 2e-6 * dscale)
+    for i in range(0, 200, 20):
+        a32 = orc.cov2d_axes(frame, g["pos"][i], c32[i] * 50, 3.0)
+        a64 = orc.cov2d_axes(frame, g["pos"][i], c64[i] * 50, 3.0, f64=True)
+        assert np.allclose(a32, a64, rtol=1e-3, atol=1e-3 * np.abs(a64).max())
+    # the f64 render equals the closed-form single-splat image as the f32 one does
+    s, z, w = 0.01, 1.0, 48
+    view, proj, frame = _camera(orc, eye=(0, 0, 0), target=(0, 0, -1), w=w, h=w)
+    cov = np.array([[s * s, 0, 0, s * s, 0, s * s]], np.float64)
+    sp = orc.splats(frame, np.array([0], np.uint32), np.array([[0, 0, -z]], np.float64),
+                    np.array([[255, 255, 255, 255]], np.uint8), np.zeros((1, 45)), cov, f64=True)
+    img = orc.render(frame, w, w, sp, f64=True)
+    want, _, _, _ = _expected_footprint(view, proj, (w, w), np.array([0, 0, -z], np.float64), np.diag([s * s] * 3),
+                                        3.0, 1.0)
+    assert np.allclose(img[..., 0], want, atol=1e-6)
+
+
+def test_full_frame_is_the_composition_of_its_stages(gs, orc):
+    # Viewer::render (src/lib.rs:266-275): preprocess -> stable sort by key
+    # bits -> vertex stage -> blend, composed by hand from the pinned stages
+    n, w, h = 3000, 160, 90
+    g = gs.synth_gaussians(0, n, seed=12)
+    view, proj, frame = _camera(orc, w=w, h=h)
+    pos = np.ascontiguousarray(g["pos"], np.float32)
+    sh = np.ascontiguousarray(g["sh"].reshape(n, 45), np.float32)
+    cov = orc.cov3d_f32(g["scale"], g["rot"])
+    idx, key = orc.preprocess(frame, pos, cov)
+    order = np.argsort(key.view(np.uint32), kind="stable")
+    sp = orc.splats(frame, idx[order], pos, g["color"], sh, cov)
+    img = orc.render(frame, w, h, sp, orc.TARGET_RGBA8_STEP)
+    fidx, fkey, fimg = orc.full_frame(frame, pos, g["color"], sh, cov, w, h, orc.TARGET_RGBA8_STEP, threads=2)
+    assert np.array_equal(fidx, idx[order]) and np.array_equal(fkey.view(np.uint32), key[order].view(np.uint32))
+    assert np.array_equal(fimg, img)
