@@ -167,16 +167,29 @@ def run_batched(gs, dev, model, w, h, views, steps, warmup, queues):
     return {"fps": views * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps, "tiles": tiles}
 
 
-def run_mix(gs, dev, steps, warmup, queues, seed=3):
+# The mixed-model batched workload: 4 resident models (clustered scene, f16),
+# model i synthesised with seed MIX_SEED + i.
+MIX_SIZES = [500_000, 1_000_000, 3_000_000, 8_000_000]
+MIX_SEED = 100
+
+
+def mix_workload(gs, seed=3, sets=4):
+    """32 views (model, width, height) at resolutions of MIX_RES and `sets`
+    camera sets on spheres of radius 5-30, drawn from one seeded stream."""
     rng = np.random.default_rng(seed)
-    sizes = [500_000, 1_000_000, 3_000_000, 8_000_000]
+    views = [(int(rng.integers(0, 4)), *MIX_RES[int(rng.integers(0, 4))]) for _ in range(32)]
+    pod_sets = [gs.pods_array([sphere_pod(gs, rng, v[1], v[2]) for v in views]) for _ in range(sets)]
+    return views, pod_sets
+
+
+def run_mix(gs, dev, steps, warmup, queues, seed=3):
+    sizes = MIX_SIZES
     models = []
     for i, n in enumerate(sizes):
-        models.append(gs.Model(dev, gs.synth_gaussians(2, n, 100 + i), gs.GaussianPod.HALF))
-    views = [(int(rng.integers(0, 4)), *MIX_RES[int(rng.integers(0, 4))]) for _ in range(32)]
+        models.append(gs.Model(dev, gs.synth_gaussians(2, n, MIX_SEED + i), gs.GaussianPod.HALF))
+    views, pod_sets = mix_workload(gs, seed)
     batch = gs.Batch(dev, models, queues=queues)
     batch.set_layout(views)
-    pod_sets = [gs.pods_array([sphere_pod(gs, rng, v[1], v[2]) for v in views]) for _ in range(4)]
     batch.autotune([gs.CameraPod.from_bytes(bytes(p)) for p in pod_sets[0]])
     for _ in range(2):
         for k in range(4):
@@ -226,13 +239,22 @@ def serve_requests(gs, rng, sizes, rate, seconds):
     return req
 
 
+# The serving workload's models: model i synthesised with seed SERVE_SEED + i.
+SERVE_SEED = 200
+
+
+def serve_model_sizes(rng):
+    """8 model sizes log-uniform in 1M-12M, scaled to total 40M Gaussians."""
+    raw = np.exp(rng.uniform(math.log(1e6), math.log(12e6), 8))
+    return [int(x) for x in np.round(raw * (40e6 / raw.sum()))]
+
+
 def run_serve(gs, dev, seconds, rate, queues, seed=4):
     rng = np.random.default_rng(seed)
-    raw = np.exp(rng.uniform(math.log(1e6), math.log(12e6), 8))
-    sizes = [int(x) for x in np.round(raw * (40e6 / raw.sum()))]
+    sizes = serve_model_sizes(rng)
     models = []
     for i, n in enumerate(sizes):
-        models.append(gs.Model(dev, gs.synth_gaussians(2, n, 200 + i), gs.GaussianPod.HALF))
+        models.append(gs.Model(dev, gs.synth_gaussians(2, n, SERVE_SEED + i), gs.GaussianPod.HALF))
     log(f"serve: models resident ({sum(sizes) / 1e6:.1f}M Gaussians)")
     req = serve_requests(gs, rng, sizes, rate, seconds)
     server = gs.Server(dev, models, queues=queues)

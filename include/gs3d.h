@@ -223,6 +223,14 @@ gs3d_status gs3d_viewer_draw(gs3d_viewer* v, gs3d_texture* target);
 gs3d_status gs3d_viewer_draw_standalone(gs3d_viewer* v, gs3d_texture* target);
 
 gs3d_status gs3d_viewer_set_blend_mode(gs3d_viewer* v, gs3d_blend_mode mode);
+/* No counterpart in the reference (its rasterizer needs no lists): the
+ * renderer's (tile, splat) pair lists start at max(2^20, 2N) pairs and grow
+ * when a frame needs more; a draw that overflows them draws nothing, and
+ * gs3d_device_poll grows the lists and records every draw since the last
+ * poll again (an error when one loaded its target's contents or wrote its
+ * depth attachment). This replaces the lists with ones of `pairs` pairs
+ * (1 .. 2^30 - 1); it polls the device first. */
+gs3d_status gs3d_viewer_set_pair_capacity(gs3d_viewer* v, uint32_t pairs);
 /* Compositing tile size: 8x8, 16x16 (default) or 32x8 pixels. */
 gs3d_status gs3d_viewer_set_tile_size(gs3d_viewer* v, uint32_t tile_w, uint32_t tile_h);
 /* Record HIP events around each stage of every render (off by default). */

@@ -437,3 +437,100 @@ def test_handles_outlive_their_device_handle(gs):
     assert img.shape == (h, w, 4)
     v.close()
     t.close()
+
+
+@pytest.mark.parametrize("compare", ["GREATER", "GREATER_EQUAL", "NOT_EQUAL"])
+def test_depth_writes_with_order_dependent_compare(gs, orc, device, compare):
+    """With depth writes, Greater / GreaterEqual / NotEqual depend on the
+    depths that farther splats drawn before wrote (src/renderer.rs:300-305,
+    back-to-front draw order): only the farthest passing splat of a pixel
+    draws under Greater. The default (fast) blend sends such draws through
+    the in-order path; image and attachment match the oracle's sequential
+    render with writes."""
+    _case(f"depth writes compare={compare}")
+    g, pod, w, h = _small_scene(gs)
+    cmp = getattr(gs.CompareFunction, compare)
+    v = gs.Viewer(device, gs.TextureFormat.RGBA8_UNORM, g)
+    v.update_camera_with_pod(pod)
+    v.set_depth_stencil(True, cmp, True)
+    frame = orc.frame_from_pod(pod.to_bytes())
+    pos, color, sh, cov = oracle_inputs(orc, g)
+    oidx, _, _ = orc.full_frame(frame, pos, color, sh, cov, w, h, orc.TARGET_RGBA8_F32)
+    sp = orc.splats(frame, oidx, pos, color, sh, cov)
+    d0 = _depth_plane(sp, w, h)
+    depth = gs.Texture(device, w, h, gs.TextureFormat.DEPTH32_FLOAT)
+    depth.write(d0)
+    t = gs.Texture(device, w, h, gs.TextureFormat.RGBA8_UNORM)
+    v.render_with_depth(t, depth)
+    img, gd = t.read(), depth.read()
+    od = d0.copy()
+    oimg = orc.render(frame, w, h, sp, orc.TARGET_RGBA8_STEP, depth=od, compare=int(cmp), depth_write=True)
+    no_write = orc.render(frame, w, h, sp, orc.TARGET_RGBA8_STEP, depth=d0.copy(), compare=int(cmp))
+    d, close = max_abs_diff(img, oimg), np.isclose(gd, od, rtol=2e-6, atol=0).mean()
+    print(f"depth writes {compare}: max diff {d}/255, depth as the oracle's at {close * 100:.3f}% of pixels, "
+          f"{np.mean(np.any(oimg != no_write, axis=2)) * 100:.1f}% of pixels changed by the writes", flush=True)
+    if compare != "NOT_EQUAL":
+        assert np.any(oimg != no_write)  # the writes matter for this compare
+    assert d <= 2
+    assert close >= 0.999
+    v.close()
+
+
+# ---- tile-list overflow: grown and recorded again at poll ------------------------------
+
+def test_overflowed_draws_are_recorded_again(gs, orc, device):
+    """Lists far too small for the frame: two frames of one viewer into two
+    targets (different cameras) before one poll; both draws overflow, the
+    poll grows the lists and records both frames again with the cameras they
+    were recorded with. Both match the oracle."""
+    _case("overflow: two frames before poll")
+    n, w, h = 20_000, 320, 180
+    g = gs.synth_gaussians(0, n, 5)
+    v = gs.Viewer(device, gs.TextureFormat.RGBA8_UNORM, g)
+    pods = [gs.CameraPod.look_at((0.3, 0.2, 3.0), (0, 0, 0), (w, h)),
+            gs.CameraPod.look_at((-2.5, 0.4, 1.5), (0, 0, 0), (w, h))]
+    targets = [gs.Texture(device, w, h) for _ in pods]
+    v.set_pair_capacity(64)
+    for pod, t in zip(pods, targets):
+        v.update_camera_with_pod(pod)
+        v.render(t)
+    device.poll()
+    pos, color, sh, cov = oracle_inputs(orc, g)
+    for pod, t in zip(pods, targets):
+        frame = orc.frame_from_pod(pod.to_bytes())
+        _, _, oimg = orc.full_frame(frame, pos, color, sh, cov, w, h, orc.TARGET_RGBA8_F32)
+        img = t.read()
+        assert img[..., :3].max() > 0
+        assert max_abs_diff(img, oimg) <= 2
+    # the viewer's buffers hold the last recorded frame, as without overflow
+    assert np.array_equal(v.indirect_indices(int(v.indirect_args()[1])),
+                          orc.full_frame(orc.frame_from_pod(pods[1].to_bytes()), pos, color, sh, cov, w, h)[0])
+    v.close()
+
+
+def test_overflow_of_a_loading_draw_is_an_error(gs, device):
+    """A draw that blends over the target's contents (LoadOp::Load) cannot be
+    recorded again once it drew nothing: the poll reports it. A texture
+    destroyed before the poll drops its draws."""
+    _case("overflow: loading draw")
+    g, pod, w, h = _small_scene(gs)
+    v = gs.Viewer(device, gs.TextureFormat.RGBA8_UNORM, g)
+    v.update_camera_with_pod(pod)
+    v.set_depth_stencil(True, gs.CompareFunction.ALWAYS, False)
+    depth = gs.Texture(device, w, h, gs.TextureFormat.DEPTH32_FLOAT)
+    t = gs.Texture(device, w, h)
+    v.set_pair_capacity(16)
+    v.render_with_depth(t, depth, load=True)
+    with pytest.raises(gs.Gs3dError, match="cannot be recorded again"):
+        device.poll()
+    # destroyed before the poll: nothing is recorded again, nothing fails
+    v.set_pair_capacity(16)
+    t2 = gs.Texture(device, w, h)
+    v.render_with_depth(t2, depth, load=True)
+    t2.close()
+    device.poll()
+    # a clearing draw that overflows is recorded again by the next poll
+    v.render_with_depth(t, depth)
+    device.poll()
+    assert t.read()[..., :3].max() > 0
+    v.close()

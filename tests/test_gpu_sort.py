@@ -23,7 +23,7 @@ def _check(gs, device, keys, vals=None):
     s.close()
 
 
-@pytest.mark.parametrize("n", [0, 1, 2, 255, 4095, 4096, 4097, 65537, 1_000_003])
+@pytest.mark.parametrize("n", [0, 1, 2, 255, 4095, 4096, 4097, 65537, 1_000_003, 8_000_000, 12_000_000])
 def test_sort_random_sizes(gs, device, n):
     print(f"[case] sort random n={n}", flush=True)
     rng = np.random.default_rng(n)

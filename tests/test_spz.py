@@ -121,3 +121,21 @@ def test_spz_errors(gs):
     with pytest.raises(gs.Gs3dError, match="unsupported version"):
         gs.read_spz(gzip.compress(raw[:4] + struct.pack("<I", 9) + raw[8:]))
     assert gs.read_spz(_pack_spz(pos[:0], alpha[:0], dc[:0], log_scale[:0], q[:0], sh[:0])).shape == (0,)
+
+
+def test_spz_rejects_fractional_bits_and_bounds_inflation(gs):
+    pos, alpha, dc, log_scale, q, sh = _records(5, 9)
+    raw = gzip.decompress(_pack_spz(pos, alpha, dc, log_scale, q, sh))
+    for bits in (25, 32, 255):
+        bad = raw[:13] + bytes([bits]) + raw[14:]
+        with pytest.raises(gs.Gs3dError, match="fractional bits"):
+            gs.read_spz(gzip.compress(bad))
+    # a stream far longer than its point count needs (a gzip bomb behind a
+    # small header) is read up to what the header's count needs
+    bomb = gzip.compress(raw + bytes(64 << 20), compresslevel=1)
+    assert len(bomb) < (1 << 20)
+    g = gs.read_spz(bomb)
+    assert g.tobytes() == gs.read_spz(gzip.compress(raw)).tobytes()
+    # and a truncated stream is an error, not a partial read
+    with pytest.raises(gs.Gs3dError):
+        gs.read_spz(gzip.compress(raw)[:-12])

@@ -191,6 +191,7 @@ def _declare(L):
         "gs3d_viewer_draw_standalone": ([vp, vp], i32),
         "gs3d_viewer_set_blend_mode": ([vp, i32], i32),
         "gs3d_viewer_set_tile_size": ([vp, u32, u32], i32),
+        "gs3d_viewer_set_pair_capacity": ([vp, u32], i32),
         "gs3d_viewer_set_profiling": ([vp, i32], i32),
         "gs3d_viewer_stage_times": ([vp, vp], i32),
         "gs3d_viewer_update_selection": ([vp, vp, u32], i32),
@@ -258,7 +259,7 @@ EXPORTED_SYMBOLS = (
     "gs3d_texture_read", "gs3d_viewer_new", "gs3d_viewer_destroy", "gs3d_viewer_update_camera",
     "gs3d_viewer_update_camera_with_pod", "gs3d_viewer_update_model_transform",
     "gs3d_viewer_update_gaussian_transform", "gs3d_viewer_render", "gs3d_viewer_preprocess", "gs3d_viewer_sort",
-    "gs3d_viewer_draw", "gs3d_viewer_draw_standalone", "gs3d_viewer_set_blend_mode", "gs3d_viewer_set_tile_size",
+    "gs3d_viewer_draw", "gs3d_viewer_draw_standalone", "gs3d_viewer_set_blend_mode", "gs3d_viewer_set_tile_size", "gs3d_viewer_set_pair_capacity",
     "gs3d_viewer_set_profiling", "gs3d_viewer_stage_times", "gs3d_viewer_count_fragments",
     "gs3d_viewer_update_selection", "gs3d_viewer_update_invert_selection",
     "gs3d_viewer_set_depth_stencil", "gs3d_viewer_render_with_depth", "gs3d_texture_write",
@@ -532,6 +533,11 @@ class Viewer:
 
     def set_tile_size(self, tw: int, th: int) -> None:
         _check(lib().gs3d_viewer_set_tile_size(self._h, int(tw), int(th)))
+
+    def set_pair_capacity(self, pairs: int) -> None:
+        """Replaces the (tile, splat) pair lists with ones of `pairs` pairs
+        (they grow again when a frame needs more)."""
+        _check(lib().gs3d_viewer_set_pair_capacity(self._h, int(pairs)))
 
     def set_profiling(self, on: bool) -> None:
         _check(lib().gs3d_viewer_set_profiling(self._h, int(bool(on))))

@@ -93,6 +93,21 @@ void MultiModelViewer::render(Texture& target, const std::vector<std::string>& k
   }
   // ... then drawn in the caller's order into one pass (:505-527)
   for (size_t i = 0; i < vs.size(); ++i) vs[i]->renderer.render(target, vs[i]->indirect_args_buffer, false, i == 0);
+
+  // kept for Device::poll, with every listed model's state
+  RecordedUnit u;
+  std::vector<Viewer::State> states;
+  for (Viewer* v : vs) {
+    states.push_back(v->state());
+    u.renderers.push_back(&v->renderer);
+  }
+  Texture* t = &target;
+  u.replay = [this, t, keys, vs, states] {
+    for (size_t i = 0; i < vs.size(); ++i) vs[i]->restore(states[i]);
+    render(*t, keys);
+  };
+  u.textures = {&target};
+  dev_.record_unit(std::move(u));
 }
 
 // ---- BatchRenderer ----------------------------------------------------------------
@@ -179,7 +194,7 @@ void BatchRenderer::render(const CameraPod* pods, bool use_graph) {
     hip_check(hipEventRecord(fork_[0], s0), "hipEventRecord");
     for (size_t q = 1; q < queues_.size(); ++q)
       hip_check(hipStreamWaitEvent(dev_.stream(queues_[q]), fork_[0], 0), "hipStreamWaitEvent");
-    for (size_t i = 0; i < specs_.size(); ++i) viewers_[i]->render(*targets_[i]);
+    for (size_t i = 0; i < specs_.size(); ++i) viewers_[i]->render(*targets_[i], false);
     for (size_t q = 1; q < queues_.size(); ++q) {
       hip_check(hipEventRecord(join_[q], dev_.stream(queues_[q])), "hipEventRecord");
       hip_check(hipStreamWaitEvent(s0, join_[q], 0), "hipStreamWaitEvent");
@@ -206,16 +221,22 @@ void BatchRenderer::finish() {
   dev_.wait();
   const uint32_t e = dev_.take_errors();
   throw_kernel_errors(e);
+  // every view's viewer drew once since the last finish: a view whose lists
+  // overflowed is drawn again with larger lists until it fits
   bool regrown = false;
-  for (auto& v : viewers_)
-    if (v->renderer.grow_if_overflowed()) {
-      v->renderer.rerender();
-      regrown = true;
-    }
-  if (regrown) {
+  for (int attempt = 0;; ++attempt) {
+    bool any = false;
+    for (auto& v : viewers_)
+      if (v->renderer.grow_if_overflowed()) {
+        v->renderer.rerender();
+        any = true;
+      }
+    if (!any) break;
+    regrown = true;
+    if (attempt >= 3) throw Error(GS3D_ERROR_KERNEL, "tile lists still overflowed after growing them three times");
     dev_.wait();
-    drop_graph();  // the lists moved: record again on the next graph render
   }
+  if (regrown) drop_graph();  // the lists moved: record again on the next graph render
 }
 
 void BatchRenderer::autotune(const CameraPod* pods, int reps) {
@@ -232,12 +253,9 @@ void BatchRenderer::autotune(const CameraPod* pods, int reps) {
       v.renderer.set_tile_size(sizes[k][0], sizes[k][1]);
       // warm: sizes the buffers (and the tile lists) for this tile size, so
       // that the timed draws below are complete frames
-      v.render(*targets_[i]);
+      v.render(*targets_[i], false);
       dev_.wait();
-      for (int g = 0; g < 4 && v.renderer.grow_if_overflowed(); ++g) {
-        v.render(*targets_[i]);
-        dev_.wait();
-      }
+      v.renderer.settle();
       hipEvent_t a, b;
       hip_check(hipEventCreate(&a), "hipEventCreate");
       hip_check(hipEventCreate(&b), "hipEventCreate");
@@ -353,11 +371,8 @@ ServeStats Server::run(const std::vector<ServeRequest>& reqs, const std::vector<
       }
       hip_check(q, "hipEventQuery");
       hipStream_t sq = dev_.stream(s->viewer->queue);
-      if (s->viewer->renderer.grow_if_overflowed()) {
-        // the frame's tile lists overflowed: render it again with larger lists
-        s->viewer->renderer.rerender();
-        hip_check(hipStreamSynchronize(sq), "hipStreamSynchronize");
-      }
+      // a frame whose tile lists overflowed is drawn again with larger lists
+      s->viewer->renderer.settle();
       const double t = now_s();
       lat[s->request] = t - reqs[s->request].arrival_s;
       last_done = std::max(last_done, t);
@@ -383,7 +398,7 @@ ServeStats Server::run(const std::vector<ServeRequest>& reqs, const std::vector<
       s->request = pending.front();
       pending.pop_front();
       s->viewer->update_camera_with_pod(r.pod);
-      s->viewer->render(*s->target);
+      s->viewer->render(*s->target, false);
       hip_check(hipEventRecord(s->done, dev_.stream(s->viewer->queue)), "hipEventRecord");
       in_flight.push_back(s);
       st.peak_hbm_bytes = std::max<uint64_t>(st.peak_hbm_bytes, used_);
@@ -427,13 +442,9 @@ void Server::render_alone(const ServeRequest& r, uint8_t* out) {
   Slot* s = acquire(r);
   if (!s) throw Error(GS3D_ERROR_OUT_OF_MEMORY, "the request does not fit the HBM budget");
   s->viewer->update_camera_with_pod(r.pod);
-  s->viewer->render(*s->target);
-  hipStream_t sq = dev_.stream(s->viewer->queue);
-  hip_check(hipStreamSynchronize(sq), "hipStreamSynchronize");
-  if (s->viewer->renderer.grow_if_overflowed()) {
-    s->viewer->renderer.rerender();
-    hip_check(hipStreamSynchronize(sq), "hipStreamSynchronize");
-  }
+  s->viewer->render(*s->target, false);
+  hip_check(hipStreamSynchronize(dev_.stream(s->viewer->queue)), "hipStreamSynchronize");
+  s->viewer->renderer.settle();
   hip_check(hipMemcpy(out, s->target->dev(), s->target->bytes(), hipMemcpyDeviceToHost), "hipMemcpy(image)");
 }
 

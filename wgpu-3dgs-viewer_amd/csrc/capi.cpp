@@ -288,14 +288,14 @@ gs3d_status gs3d_viewer_render_with_depth(gs3d_viewer* v, gs3d_texture* target, 
 gs3d_status gs3d_viewer_draw(gs3d_viewer* v, gs3d_texture* target) {
   return guard([&] {
     need(v != nullptr && target != nullptr, "null argument");
-    v->v.renderer.render(target->tex, v->v.indirect_args_buffer, false);
+    v->v.draw(target->tex, false);
   });
 }
 
 gs3d_status gs3d_viewer_draw_standalone(gs3d_viewer* v, gs3d_texture* target) {
   return guard([&] {
     need(v != nullptr && target != nullptr, "null argument");
-    v->v.renderer.render(target->tex, v->v.indirect_args_buffer, true);
+    v->v.draw(target->tex, true);
   });
 }
 
@@ -304,6 +304,13 @@ gs3d_status gs3d_viewer_set_blend_mode(gs3d_viewer* v, gs3d_blend_mode mode) {
     need(v != nullptr, "viewer is null");
     need(mode == GS3D_BLEND_FAST || mode == GS3D_BLEND_EXACT, "unknown blend mode");
     v->v.renderer.set_blend(mode == GS3D_BLEND_FAST ? gs3d::kBlendFast : gs3d::kBlendExactUnorm8);
+  });
+}
+
+gs3d_status gs3d_viewer_set_pair_capacity(gs3d_viewer* v, uint32_t pairs) {
+  return guard([&] {
+    need(v != nullptr, "viewer is null");
+    v->v.renderer.set_capacity(pairs);
   });
 }
 

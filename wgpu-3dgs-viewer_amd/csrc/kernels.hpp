@@ -59,7 +59,8 @@ void launch_vertex(hipStream_t st, int pod, const FrameConsts* fc, const Gaussia
 
 // ---- radix_sort.hip --------------------------------------------------------
 
-// Element count read on the device: n = min(*count * count_mul, count_max).
+// Element count read on the device: n = min(*count * count_mul, count_max),
+// or count_max when count is null (a count fixed at launch).
 struct SortCount {
   const uint32_t* count;
   uint32_t count_mul;

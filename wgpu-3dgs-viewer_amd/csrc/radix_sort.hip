@@ -16,6 +16,8 @@
 // dispatch): n = min(*count * count_mul, count_max).
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "device_util.hpp"
 #include "kernels.hpp"
 #include "types.hpp"
@@ -42,6 +44,7 @@ constexpr uint32_t kRsTile = kRsThreads * kRsItems;
 constexpr int kRsLookWindow = GS3D_SORT_LOOK_WINDOW;
 
 __device__ __forceinline__ uint32_t sort_n(const SortCount& c) {
+  if (c.count == nullptr) return c.count_max;  // a count fixed at launch
   const uint64_t v = uint64_t(*c.count) * uint64_t(c.count_mul);
   return v < uint64_t(c.count_max) ? uint32_t(v) : c.count_max;
 }
@@ -114,20 +117,109 @@ __global__ __launch_bounds__(256) void k_rs_upsweep(SortCount cnt, const uint32_
   if (c) atomicAdd(&hist[threadIdx.x], c);
 }
 
+// ---- one tile of a pass: load, rank, reorder in LDS ------------------------------
+
+// LDS of a pass over one tile.
+struct TileLds {
+  uint32_t keys[kRsTile];
+  uint32_t vals[kRsTile];
+  uint32_t wcnt[kRsWaves][256];  // per wave and digit: count, then the wave's offset
+  uint32_t dbase[256];           // the digit's first slot in the tile
+  uint32_t gbase[256];           // global position of slot 0 of the digit's run
+  uint32_t wsum[kRsWaves];
+};
+
+// Loads tile keys/values [base, base + valid) (key 0xFFFFFFFF past the end:
+// digit 255, ranked after every valid key) in the (wave, item, lane) order
+// the ranking assumes.
+__device__ __forceinline__ void load_tile(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                          uint32_t base, uint32_t valid, uint32_t (&key)[kRsItems],
+                                          uint32_t (&val)[kRsItems]) {
+  const uint32_t lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < kRsItems; ++k) {
+    const uint32_t li = w * (kWave * kRsItems) + uint32_t(k) * kWave + lane;
+    const bool ok = li < valid;
+    key[k] = ok ? kin[base + li] : 0xFFFFFFFFu;
+    val[k] = ok ? vin[base + li] : 0u;
+  }
+}
+
+// Stable rank of every key within its wave by wave64 match-by-ballot (8
+// ballots give the lanes sharing the key's digit); lds.wcnt[w][d] ends as the
+// wave's count of digit d. lds.wcnt must be zero on entry.
+__device__ __forceinline__ void rank_tile(TileLds& lds, const uint32_t (&key)[kRsItems], int shift,
+                                          uint32_t (&lrank)[kRsItems]) {
+  const uint32_t w = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < kRsItems; ++k) {
+    const uint32_t d = (key[k] >> shift) & 0xFFu;
+    uint64_t m = ~0ull;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bb = __ballot(bit);
+      m &= bit ? bb : ~bb;
+    }
+    const uint32_t r = mask_rank(m);
+    const uint32_t prev = lds.wcnt[w][d];
+    if (r == 0u) lds.wcnt[w][d] = prev + uint32_t(__popcll(m));
+    lrank[k] = prev + r;
+  }
+}
+
+// Thread d: turns the wave counts of digit d into wave offsets and returns
+// the tile's count of digit d. Call after a barrier that follows rank_tile.
+__device__ __forceinline__ uint32_t digit_wave_offsets(TileLds& lds) {
+  const uint32_t d = threadIdx.x;
+  uint32_t tot = 0;
+#pragma unroll
+  for (int q = 0; q < kRsWaves; ++q) {
+    const uint32_t c = lds.wcnt[q][d];
+    lds.wcnt[q][d] = tot;
+    tot += c;
+  }
+  return tot;
+}
+
+// Writes every key and value to its slot in the tile sorted by the digit
+// (stable: ranks follow input order). lds.dbase must be set.
+__device__ __forceinline__ void reorder_tile(TileLds& lds, const uint32_t (&key)[kRsItems],
+                                             const uint32_t (&val)[kRsItems], const uint32_t (&lrank)[kRsItems],
+                                             int shift) {
+  const uint32_t w = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < kRsItems; ++k) {
+    const uint32_t dk = (key[k] >> shift) & 0xFFu;
+    const uint32_t slot = lds.dbase[dk] + lds.wcnt[w][dk] + lrank[k];
+    lds.keys[slot] = key[k];
+    lds.vals[slot] = val[k];
+  }
+}
+
+// Writes the tile's valid slots to gbase[digit] + slot: runs of equal digits
+// land contiguously. lds.gbase must be set and visible.
+__device__ __forceinline__ void scatter_tile(const TileLds& lds, uint32_t valid, uint32_t n, int shift,
+                                             uint32_t* __restrict__ kout, uint32_t* __restrict__ vout) {
+  for (uint32_t j = threadIdx.x; j < valid; j += kRsThreads) {
+    const uint32_t k = lds.keys[j];
+    const uint32_t pos = lds.gbase[(k >> shift) & 0xFFu] + j;
+    if (pos < n) {  // always true for consistent counts; guards the write
+      kout[pos] = k;
+      vout[pos] = lds.vals[j];
+    }
+  }
+}
+
 __global__ __launch_bounds__(kRsThreads) void k_rs_onesweep(SortCount cnt, const uint32_t* __restrict__ kin,
                                                            const uint32_t* __restrict__ vin,
                                                            uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                            const uint32_t* __restrict__ hist, uint32_t* lookback,
                                                            uint32_t* ticket, int shift, uint32_t* err,
                                                            uint32_t* hist_next) {
-  __shared__ uint32_t s_keys[kRsTile];
+  __shared__ TileLds lds;
   __shared__ uint32_t s_next[256];  // next pass's digit counts of the tile
-  __shared__ uint32_t s_vals[kRsTile];
-  __shared__ uint32_t s_wcnt[kRsWaves][256];
-  __shared__ uint32_t s_dbase[256];
-  __shared__ uint32_t s_gbase[256];
   __shared__ uint32_t s_hscan[256];
-  __shared__ uint32_t s_wsum[kRsWaves];
   __shared__ uint32_t s_tile;
 
   const uint32_t n = sort_n(cnt);
@@ -138,7 +230,7 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_onesweep(SortCount cnt, const
 
   {
     const uint32_t h = hist[tid];
-    s_hscan[tid] = block_inclusive_scan_256(h, s_wsum) - h;
+    s_hscan[tid] = block_inclusive_scan_256(h, lds.wsum) - h;
   }
 
   for (;;) {
@@ -148,18 +240,12 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_onesweep(SortCount cnt, const
     const uint32_t valid = (n - base) < kRsTile ? (n - base) : kRsTile;
 
 #pragma unroll
-    for (int q = 0; q < kRsWaves; ++q) s_wcnt[q][tid] = 0u;
+    for (int q = 0; q < kRsWaves; ++q) lds.wcnt[q][tid] = 0u;
     s_next[tid] = 0u;
     __syncthreads();
 
     uint32_t key[kRsItems], val[kRsItems], lrank[kRsItems];
-#pragma unroll
-    for (int k = 0; k < kRsItems; ++k) {
-      const uint32_t li = w * (kWave * kRsItems) + uint32_t(k) * kWave + lane;
-      const bool ok = li < valid;
-      key[k] = ok ? kin[base + li] : 0xFFFFFFFFu;
-      val[k] = ok ? vin[base + li] : 0u;
-    }
+    load_tile(kin, vin, base, valid, key, val);
     if (hist_next) {
       // the next pass's histogram: the same keys, whatever their order
 #pragma unroll
@@ -168,46 +254,19 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_onesweep(SortCount cnt, const
         if (li < valid) atomicAdd(&s_next[(key[k] >> (shift + 8)) & 0xFFu], 1u);
       }
     }
-#pragma unroll
-    for (int k = 0; k < kRsItems; ++k) {
-      const uint32_t d = (key[k] >> shift) & 0xFFu;
-      uint64_t m = ~0ull;
-#pragma unroll
-      for (int b = 0; b < 8; ++b) {
-        const bool bit = (d >> b) & 1u;
-        const uint64_t bb = __ballot(bit);
-        m &= bit ? bb : ~bb;
-      }
-      const uint32_t r = mask_rank(m);
-      const uint32_t prev = s_wcnt[w][d];
-      if (r == 0u) s_wcnt[w][d] = prev + uint32_t(__popcll(m));
-      lrank[k] = prev + r;
-    }
+    rank_tile(lds, key, shift, lrank);
     __syncthreads();
 
     // per digit (thread = digit): wave offsets, tile total, publish
     const uint32_t d = tid;
-    uint32_t tot = 0;
-#pragma unroll
-    for (int q = 0; q < kRsWaves; ++q) {
-      const uint32_t c = s_wcnt[q][d];
-      s_wcnt[q][d] = tot;
-      tot += c;
-    }
+    const uint32_t tot = digit_wave_offsets(lds);
     const uint32_t pub = tot - ((d == 255u) ? (kRsTile - valid) : 0u);
     status_store(lookback + size_t(t) * 256u + d, (t == 0 ? kFlagInclusive : kFlagAggregate) | pub);
-    const uint32_t dbase = block_inclusive_scan_256(tot, s_wsum) - tot;
-    s_dbase[d] = dbase;
+    const uint32_t dbase = block_inclusive_scan_256(tot, lds.wsum) - tot;
+    lds.dbase[d] = dbase;
     __syncthreads();
 
-    // reorder the tile in LDS (stable: ranks follow input order)
-#pragma unroll
-    for (int k = 0; k < kRsItems; ++k) {
-      const uint32_t dk = (key[k] >> shift) & 0xFFu;
-      const uint32_t slot = s_dbase[dk] + s_wcnt[w][dk] + lrank[k];
-      s_keys[slot] = key[k];
-      s_vals[slot] = val[k];
-    }
+    reorder_tile(lds, key, val, lrank, shift);
 
     // global offset of this tile's run of digit d
     // (each step requests the words of the next kRsLookWindow predecessors
@@ -244,20 +303,116 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_onesweep(SortCount cnt, const
       }
       status_store(lookback + size_t(t) * 256u + d, kFlagInclusive | (excl + pub));
     }
-    s_gbase[d] = s_hscan[d] + excl - dbase;
+    lds.gbase[d] = s_hscan[d] + excl - dbase;
     __syncthreads();
     if (hist_next && s_next[tid]) atomicAdd(&hist_next[tid], s_next[tid]);
 
-    for (uint32_t j = tid; j < valid; j += kRsThreads) {
-      const uint32_t k = s_keys[j];
-      const uint32_t pos = s_gbase[(k >> shift) & 0xFFu] + j;
-      if (pos < n) {  // always true for a consistent histogram; guards the write
-        kout[pos] = k;
-        vout[pos] = s_vals[j];
-      }
-    }
+    scatter_tile(lds, valid, n, shift, kout, vout);
     __syncthreads();
   }
+}
+
+// ---- reduce-then-scan passes -------------------------------------------------------
+//
+// The same pass without any wait between workgroups, in three launches:
+//   count    per tile, the count of each digit: counts[d][t] (digit-major, so
+//            that each digit's column over the tiles is contiguous)
+//   colscan  per digit (one workgroup each), the exclusive prefix of its
+//            column, in place, and the digit's total
+//   scatter  per tile: the digit's global base (exclusive scan of the
+//            totals) plus the tile's prefix, then rank, reorder and scatter
+//            as above.
+// It reads the keys once more per pass (4 bytes per key), but no tile waits
+// on another: a onesweep tile's look-back round trips to its predecessors'
+// status words grow with the number of tiles that publish at once.
+
+__global__ __launch_bounds__(kRsThreads) void k_rs_count(SortCount cnt, const uint32_t* __restrict__ keys,
+                                                        int shift, uint32_t* __restrict__ counts) {
+  __shared__ uint32_t s_h[kRsWaves][256];
+  const uint32_t n = sort_n(cnt);
+  const uint32_t num_tiles = (n + kRsTile - 1) / kRsTile;
+  const uint32_t t = blockIdx.x;
+  if (t >= num_tiles) return;
+  const uint32_t tid = threadIdx.x, lane = tid % kWave, w = tid / kWave;
+#pragma unroll
+  for (int q = 0; q < kRsWaves; ++q) s_h[q][tid] = 0u;
+  __syncthreads();
+  const uint32_t base = t * kRsTile;
+  const uint32_t valid = (n - base) < kRsTile ? (n - base) : kRsTile;
+  uint32_t key[kRsItems];
+#pragma unroll
+  for (int k = 0; k < kRsItems; ++k) {
+    const uint32_t li = w * (kWave * kRsItems) + uint32_t(k) * kWave + lane;
+    key[k] = li < valid ? keys[base + li] : 0u;
+  }
+#pragma unroll
+  for (int k = 0; k < kRsItems; ++k) {
+    const uint32_t li = w * (kWave * kRsItems) + uint32_t(k) * kWave + lane;
+    if (li < valid) atomicAdd(&s_h[w][(key[k] >> shift) & 0xFFu], 1u);
+  }
+  __syncthreads();
+  uint32_t c = 0;
+#pragma unroll
+  for (int q = 0; q < kRsWaves; ++q) c += s_h[q][tid];
+  counts[size_t(tid) * num_tiles + t] = c;
+}
+
+__global__ __launch_bounds__(kRsThreads) void k_rs_colscan(SortCount cnt, uint32_t* __restrict__ counts,
+                                                          uint32_t* __restrict__ totals) {
+  __shared__ uint32_t s_wsum[kRsWaves];
+  __shared__ uint32_t s_last;
+  const uint32_t n = sort_n(cnt);
+  const uint32_t num_tiles = (n + kRsTile - 1) / kRsTile;
+  const uint32_t d = blockIdx.x, tid = threadIdx.x;
+  uint32_t* col = counts + size_t(d) * num_tiles;
+  uint32_t carry = 0;
+  for (uint32_t c0 = 0; c0 < num_tiles; c0 += kRsThreads) {
+    const uint32_t i = c0 + tid;
+    const uint32_t x = i < num_tiles ? col[i] : 0u;
+    const uint32_t inc = block_inclusive_scan_256(x, s_wsum);
+    if (i < num_tiles) col[i] = carry + inc - x;
+    if (tid == kRsThreads - 1) s_last = inc;
+    __syncthreads();
+    carry += s_last;
+    __syncthreads();
+  }
+  if (tid == 0) totals[d] = carry;
+}
+
+__global__ __launch_bounds__(kRsThreads) void k_rs_scatter(SortCount cnt, const uint32_t* __restrict__ kin,
+                                                          const uint32_t* __restrict__ vin,
+                                                          uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                          const uint32_t* __restrict__ counts,
+                                                          const uint32_t* __restrict__ totals, int shift) {
+  __shared__ TileLds lds;
+  const uint32_t n = sort_n(cnt);
+  const uint32_t num_tiles = (n + kRsTile - 1) / kRsTile;
+  const uint32_t t = blockIdx.x;
+  if (t >= num_tiles) return;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t base = t * kRsTile;
+  const uint32_t valid = (n - base) < kRsTile ? (n - base) : kRsTile;
+
+  // requested first: the loads overlap the scans below
+  uint32_t key[kRsItems], val[kRsItems], lrank[kRsItems];
+  load_tile(kin, vin, base, valid, key, val);
+  const uint32_t tot_all = totals[tid];
+  const uint32_t col = counts[size_t(tid) * num_tiles + t];
+#pragma unroll
+  for (int q = 0; q < kRsWaves; ++q) lds.wcnt[q][tid] = 0u;
+  // global base of digit tid's run of this tile
+  const uint32_t gb = block_inclusive_scan_256(tot_all, lds.wsum) - tot_all + col;  // has barriers
+
+  rank_tile(lds, key, shift, lrank);
+  __syncthreads();
+  const uint32_t tot = digit_wave_offsets(lds);
+  const uint32_t dbase = block_inclusive_scan_256(tot, lds.wsum) - tot;
+  lds.dbase[tid] = dbase;
+  lds.gbase[tid] = gb - dbase;
+  __syncthreads();
+  reorder_tile(lds, key, val, lrank, shift);
+  __syncthreads();
+  scatter_tile(lds, valid, n, shift, kout, vout);
 }
 
 uint32_t sort_tile_items() { return kRsTile; }
@@ -278,10 +433,37 @@ static void launch_zero(hipStream_t st, int num_cus, SortCount cnt, const SortSc
                      zero_hist ? 1 : 0);
 }
 
+static int sort_algo() {
+  static const int a = [] {
+    const char* e = std::getenv("GS3D_SORT_ALGO");
+    return (e && e[0] == 'r') ? 1 : 0;
+  }();
+  return a;
+}
+
 int launch_radix_sort(hipStream_t st, int num_cus, SortCount cnt, uint32_t* k0, uint32_t* v0, uint32_t* k1,
                       uint32_t* v1, int passes, const SortScratch& sc, uint32_t* pad_keys, uint32_t pad_begin,
                       uint32_t pad_end, bool hist_ready) {
   if (sc.max_tiles == 0 || passes <= 0) return 0;
+  if (sort_algo() == 1) {
+    // reduce-then-scan: counts [256][tiles] in the look-back words, digit
+    // totals in the histogram words; nothing to clear but the pad keys
+    if (pad_keys) launch_zero(st, num_cus, cnt, sc, 0, pad_keys, pad_begin, pad_end, false);
+    uint32_t *ks = k0, *vs = v0, *kd = k1, *vd = v1;
+    for (int p = 0; p < passes; ++p) {
+      hipLaunchKernelGGL(k_rs_count, dim3(sc.max_tiles), dim3(kRsThreads), 0, st, cnt, ks, 8 * p, sc.lookback);
+      hipLaunchKernelGGL(k_rs_colscan, dim3(256), dim3(kRsThreads), 0, st, cnt, sc.lookback, sc.hist);
+      hipLaunchKernelGGL(k_rs_scatter, dim3(sc.max_tiles), dim3(kRsThreads), 0, st, cnt, ks, vs, kd, vd,
+                         sc.lookback, sc.hist, 8 * p);
+      uint32_t* t = ks;
+      ks = kd;
+      kd = t;
+      t = vs;
+      vs = vd;
+      vd = t;
+    }
+    return passes & 1;
+  }
   launch_zero(st, num_cus, cnt, sc, passes, pad_keys, pad_begin, pad_end, !hist_ready);
   if (!hist_ready) {
     uint32_t up_blocks = (sc.max_tiles * kRsTile + 4095u) / 4096u;

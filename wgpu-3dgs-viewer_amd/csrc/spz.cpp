@@ -15,6 +15,7 @@
 // innermost). No coordinate-system conversion is applied.
 #include <zlib.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -34,27 +35,42 @@ constexpr uint32_t kMagic = 0x5053474eu;
 constexpr float kColorScale = 0.15f;
 constexpr double kC0 = 0.28209479177387814;
 
-std::vector<uint8_t> gunzip(const uint8_t* data, size_t size) {
-  z_stream zs{};
-  if (inflateInit2(&zs, 16 + MAX_WBITS) != Z_OK) bad("cannot initialise zlib");
-  zs.next_in = const_cast<Bytef*>(data);
-  zs.avail_in = uInt(size);
-  std::vector<uint8_t> out;
-  uint8_t chunk[1 << 16];
-  int rc = Z_OK;
-  do {
-    zs.next_out = chunk;
-    zs.avail_out = sizeof(chunk);
-    rc = inflate(&zs, Z_NO_FLUSH);
-    if (rc != Z_OK && rc != Z_STREAM_END) {
-      inflateEnd(&zs);
-      bad("corrupt gzip stream");
+// Inflates a gzip stream into `out` until it holds `want` bytes or the stream
+// ends (no more than `want` bytes are kept, so a header cannot make the
+// reader inflate more than its point count needs). The caller may call again
+// with a larger `want` on the same stream.
+class Gunzip {
+ public:
+  Gunzip(const uint8_t* data, size_t size) {
+    if (inflateInit2(&zs_, 16 + MAX_WBITS) != Z_OK) bad("cannot initialise zlib");
+    zs_.next_in = const_cast<Bytef*>(data);
+    zs_.avail_in = uInt(size);
+    if (size > size_t(uInt(-1))) bad("compressed stream above 4 GiB");
+  }
+  ~Gunzip() { inflateEnd(&zs_); }
+  Gunzip(const Gunzip&) = delete;
+  Gunzip& operator=(const Gunzip&) = delete;
+
+  const std::vector<uint8_t>& fill(size_t want) {
+    uint8_t chunk[1 << 16];
+    while (!ended_ && out_.size() < want) {
+      zs_.next_out = chunk;
+      zs_.avail_out = uInt(std::min(sizeof(chunk), want - out_.size()));
+      const int rc = inflate(&zs_, Z_NO_FLUSH);
+      if (rc != Z_OK && rc != Z_STREAM_END) bad("corrupt gzip stream");
+      const size_t got = size_t(zs_.next_out - chunk);
+      if (rc == Z_OK && got == 0 && zs_.avail_in == 0) bad("truncated gzip stream");
+      out_.insert(out_.end(), chunk, chunk + got);
+      ended_ = rc == Z_STREAM_END;
     }
-    out.insert(out.end(), chunk, chunk + (sizeof(chunk) - zs.avail_out));
-  } while (rc != Z_STREAM_END);
-  inflateEnd(&zs);
-  return out;
-}
+    return out_;
+  }
+
+ private:
+  z_stream zs_{};
+  std::vector<uint8_t> out_;
+  bool ended_ = false;
+};
 
 struct Header {
   uint32_t magic, version, points;
@@ -74,19 +90,24 @@ uint8_t unit_to_u8(double x) {
 
 uint64_t spz_read(const uint8_t* data, size_t size, gs3d_gaussian* out, uint64_t capacity) {
   if (size < 2 || data[0] != 0x1f || data[1] != 0x8b) bad("not a gzip stream");
-  const std::vector<uint8_t> raw = gunzip(data, size);
-  if (raw.size() < sizeof(Header)) bad("file is shorter than its header");
+  Gunzip gz(data, size);
+  if (gz.fill(sizeof(Header)).size() < sizeof(Header)) bad("file is shorter than its header");
   Header h;
-  std::memcpy(&h, raw.data(), sizeof(h));
+  std::memcpy(&h, gz.fill(sizeof(Header)).data(), sizeof(h));
   if (h.magic != kMagic) bad("bad magic");
   if (h.version != 2 && h.version != 3) bad("unsupported version " + std::to_string(h.version));
   if (h.sh_degree > 3) bad("SH degree above 3");
+  // positions are 24-bit fixed point: more fractional bits than that is not
+  // a position format (and a shift by 32 or more is undefined)
+  if (h.fractional_bits > 24) bad("fractional bits above 24");
   const uint64_t n = h.points;
   if (out == nullptr) return n;
   if (capacity < n) bad("output capacity " + std::to_string(capacity) + " < " + std::to_string(n));
   const int dim = sh_dim(h.sh_degree);
   const size_t rot_bytes = h.version == 2 ? 3 : 4;
   const size_t need = sizeof(Header) + n * (9 + 1 + 3 + 3 + rot_bytes + size_t(dim) * 3);
+  // inflate exactly what the point count needs, no more
+  const std::vector<uint8_t>& raw = gz.fill(need);
   if (raw.size() < need) bad("data is shorter than its point count");
   const uint8_t* pos = raw.data() + sizeof(Header);
   const uint8_t* alpha = pos + n * 9;

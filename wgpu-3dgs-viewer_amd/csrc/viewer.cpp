@@ -60,11 +60,23 @@ uint32_t Device::take_errors() {
   return e;
 }
 
-void Device::note_render(Renderer* r) {
-  if (std::find(pending_.begin(), pending_.end(), r) == pending_.end()) pending_.push_back(r);
+void Device::record_unit(RecordedUnit u) { units_.push_back(std::move(u)); }
+
+void Device::forget_renderer(const Renderer* r) {
+  units_.erase(std::remove_if(units_.begin(), units_.end(),
+                              [r](const RecordedUnit& u) {
+                                return std::find(u.renderers.begin(), u.renderers.end(), r) != u.renderers.end();
+                              }),
+               units_.end());
 }
 
-void Device::forget_renderer(Renderer* r) { pending_.erase(std::remove(pending_.begin(), pending_.end(), r), pending_.end()); }
+void Device::forget_texture(const Texture* t) {
+  units_.erase(std::remove_if(units_.begin(), units_.end(),
+                              [t](const RecordedUnit& u) {
+                                return std::find(u.textures.begin(), u.textures.end(), t) != u.textures.end();
+                              }),
+               units_.end());
+}
 
 void throw_kernel_errors(uint32_t e) {
   if (e & kErrLookbackSpin) throw Error(GS3D_ERROR_KERNEL, "a kernel exceeded its look-back wait limit");
@@ -75,28 +87,47 @@ void Device::poll() {
   make_current();
   for (int attempt = 0;; ++attempt) {
     wait();
-    uint32_t e = 0;
-    hip_check(hipMemcpy(&e, err_, 4, hipMemcpyDeviceToHost), "read error word");
-    if (e == 0) break;
-    hip_check(hipMemset(err_, 0, 4), "clear error word");
+    // the overflow bit is not needed here: every renderer keeps its own
+    // largest pair total since it was last asked
+    const uint32_t e = take_errors();
     if (e & (kErrLookbackSpin | kErrTileKey)) {
-      pending_.clear();
+      units_.clear();
       throw_kernel_errors(e);
     }
-    if ((e & kErrTileOverflow) && attempt < 4) {
-      // an overflowed frame drew nothing; later draws of a shared pass
-      // (LoadOp::Load) blended over what was there, so every pending draw
-      // is recorded again, in its order
-      bool any = false;
-      for (Renderer* r : pending_) any = r->grow_if_overflowed() || any;
-      if (any) {
-        for (Renderer* r : pending_) r->rerender();
-        continue;
+    std::vector<Renderer*> asked;
+    bool grown = false;
+    try {
+      for (const RecordedUnit& u : units_)
+        for (Renderer* r : u.renderers)
+          if (std::find(asked.begin(), asked.end(), r) == asked.end()) {
+            asked.push_back(r);
+            grown = r->grow_if_overflowed() || grown;
+          }
+    } catch (...) {
+      units_.clear();
+      throw;
+    }
+    if (!grown) break;
+    // A draw that overflowed drew nothing, and later draws into its target
+    // blended over what was there: every unit since the last poll is
+    // recorded again, in its order, with the state it was recorded with.
+    if (attempt >= 3) {
+      units_.clear();
+      throw Error(GS3D_ERROR_KERNEL, "tile lists still overflowed after growing them three times");
+    }
+    for (const RecordedUnit& u : units_) {
+      const char* why = u.blocker ? u.blocker() : nullptr;
+      if (why) {
+        units_.clear();
+        throw Error(GS3D_ERROR_KERNEL, std::string("a draw overflowed its tile lists and cannot be recorded again: ") +
+                                           why + " (render it again, or draw it alone)");
       }
     }
-    break;
+    std::vector<RecordedUnit> again;
+    again.swap(units_);
+    for (RecordedUnit& u : again) u.replay();  // records each unit again
   }
-  pending_.clear();
+  units_.clear();
 }
 
 // ---- DeviceBuffer -------------------------------------------------------------
@@ -257,7 +288,12 @@ void ModelTransformBuffer::update(const float pos[3], const float rot[4], const 
     t.scale[i] = scale[i];
   }
   for (int i = 0; i < 4; ++i) t.rot[i] = rot[i];
-  buf_.update(&t);
+  update_with_pod(t);
+}
+
+void ModelTransformBuffer::update_with_pod(const ModelTransformPod& pod) {
+  pod_ = pod;
+  buf_.update(&pod_);
 }
 
 GaussianTransformBuffer::GaussianTransformBuffer(Device& dev, int queue)
@@ -273,7 +309,12 @@ void GaussianTransformBuffer::update(float size, int32_t mode, uint32_t sh_deg, 
   t.size = size;
   t.flags = pack_gaussian_flags(mode, sh_deg, no_sh0);
   t.max_std_dev = max_std_dev;
-  buf_.update(&t);
+  update_with_pod(t);
+}
+
+void GaussianTransformBuffer::update_with_pod(const GaussianTransformPod& pod) {
+  pod_ = pod;
+  buf_.update(&pod_);
 }
 
 uint16_t float_to_half(float f) {
@@ -414,10 +455,12 @@ Texture::Texture(Device& dev, uint32_t w, uint32_t h, TextureFormat fmt)
   buf_ = DeviceBuffer(dev, size_t(w) * h * (fmt == kFormatRgba32Float ? 16 : 4));
 }
 
+Texture::~Texture() { device_->forget_texture(this); }
+
 void Texture::write(const void* src, size_t bytes) {
   if (bytes != buf_.size()) throw Error(GS3D_ERROR_INVALID_ARGUMENT, "texture write size mismatch");
-  // draws in flight may still use the texture
-  device_->wait();
+  // draws recorded before may still use the texture, or be recorded again
+  device_->poll();
   hip_check(hipMemcpy(buf_.get(), src, bytes, hipMemcpyHostToDevice), "hipMemcpy(texture)");
 }
 
@@ -492,6 +535,7 @@ Preprocessor::Preprocessor(Device& dev, const CameraBuffer& camera, const ModelT
 }
 
 void Preprocessor::frame_pre(uint32_t gaussian_count) {
+  ++generation_;
   const uint32_t n = std::min(gaussian_count, gaussians_.len());
   launch_frame_pre(st_, camera_.dev(), model_.dev(), gt_.dev(), consts_.as<FrameConsts>(),
                    indirect_args_.dev(), rs_args_.dev(), status_.as<uint32_t>(), preprocess_num_tiles(n),
@@ -608,7 +652,7 @@ Renderer::Renderer(Device& dev, TextureFormat format, const CameraBuffer& camera
   // src/renderer.rs:130-135
   if (gaussians.size_bytes() > dev.buffer_limit())
     throw Error(GS3D_ERROR_MODEL_SIZE_EXCEEDS_DEVICE_LIMIT, "model size exceeds the device limit");
-  ensure_capacity(std::max<uint32_t>(1u << 20, std::min<uint64_t>(uint64_t(indices.len()) * 2, 0x3FFFFFFFu)));
+  ensure_capacity(std::max<uint32_t>(1u << 20, std::min<uint64_t>(uint64_t(indices.len()) * 2, kMaxPairs)));
 }
 
 Renderer::~Renderer() {
@@ -630,8 +674,15 @@ void Renderer::ensure_capacity(uint32_t pairs) {
     pk_[b] = DeviceBuffer(dev_, size_t(pairs) * 4);
     pv_[b] = DeviceBuffer(dev_, size_t(pairs) * 4);
   }
-  scratch_ = std::make_unique<SortScratchBuffers>(dev_, pairs, 3);
+  scratch_ = std::make_unique<SortScratchBuffers>(dev_, pairs, kTileSortPasses);
   capacity_ = pairs;
+}
+
+void Renderer::set_capacity(uint32_t pairs) {
+  if (pairs == 0 || pairs > kMaxPairs) throw Error(GS3D_ERROR_INVALID_ARGUMENT, "pair capacity must be 1..2^30-1");
+  dev_.poll();  // nothing recorded before may be recorded again with other lists
+  capacity_ = 0;
+  ensure_capacity(pairs);
 }
 
 bool Renderer::grow_if_overflowed() {
@@ -642,19 +693,38 @@ bool Renderer::grow_if_overflowed() {
   const uint32_t zero = 0u;
   hip_check(hipMemcpy(pair_count_.as<uint32_t>() + 1, &zero, 4, hipMemcpyHostToDevice), "reset pair maximum");
   if (total <= capacity_) return false;
-  const uint64_t want = std::min<uint64_t>(uint64_t(total) + total / 4 + 1024, 0x3FFFFFFFu);
+  // status words hold 30-bit values: no list may hold more pairs
+  if (total > kMaxPairs)
+    throw Error(GS3D_ERROR_OUT_OF_MEMORY, "a frame needs " + std::to_string(total) + " (tile, splat) pairs, above " +
+                                              std::to_string(kMaxPairs));
+  const uint64_t want = std::min<uint64_t>(uint64_t(total) + total / 4 + 1024, kMaxPairs);
   ensure_capacity(uint32_t(want));
   return true;
 }
 
+void Renderer::settle() {
+  for (int attempt = 0; grow_if_overflowed(); ++attempt) {
+    if (attempt >= 3) throw Error(GS3D_ERROR_KERNEL, "tile lists still overflowed after growing them three times");
+    rerender();
+    hip_check(hipStreamSynchronize(st_), "hipStreamSynchronize");
+  }
+}
+
 uint64_t Renderer::count_fragments(Texture& target, const IndirectArgsBuffer& indirect_args) {
   if (frag_count_.size() == 0) frag_count_ = DeviceBuffer(dev_, 64);
-  hip_check(hipMemsetAsync(frag_count_.get(), 0, 4, st_), "hipMemsetAsync");
   const BlendMode old = blend_;
   blend_ = kBlendExactUnorm8;
   counting_ = true;
   try {
-    render(target, indirect_args, false);
+    dev_.poll();  // nothing recorded before may be recorded again after this
+    // (the count restarts with each attempt of the draw)
+    for (int attempt = 0;; ++attempt) {
+      hip_check(hipMemsetAsync(frag_count_.get(), 0, 4, st_), "hipMemsetAsync");
+      render(target, indirect_args, false);
+      dev_.wait();
+      if (!grow_if_overflowed()) break;
+      if (attempt >= 3) throw Error(GS3D_ERROR_KERNEL, "tile lists still overflowed after growing them three times");
+    }
   } catch (...) {
     blend_ = old;
     counting_ = false;
@@ -662,7 +732,6 @@ uint64_t Renderer::count_fragments(Texture& target, const IndirectArgsBuffer& in
   }
   blend_ = old;
   counting_ = false;
-  dev_.poll();
   uint32_t n = 0;
   hip_check(hipMemcpyAsync(&n, frag_count_.get(), 4, hipMemcpyDeviceToHost, st_), "hipMemcpyAsync");
   hip_check(hipStreamSynchronize(st_), "hipStreamSynchronize");
@@ -723,6 +792,8 @@ void Renderer::render(Texture& target, const IndirectArgsBuffer& indirect_args, 
                   indices_.len(), depth_enabled_ ? splats_.depth() : nullptr);
 
   const TileGrid g = grid_for(target);
+  // tile_rect packs tile coordinates into 12 bits
+  if (g.tiles_x >= 4096u || g.tiles_y >= 4096u) throw Error(GS3D_ERROR_INVALID_ARGUMENT, "tile grid above 4095 tiles");
   const uint32_t num_tiles = g.tiles_x * g.tiles_y;
   if (ranges_words_ < num_tiles * 2) {
     dev_.wait();
@@ -733,6 +804,9 @@ void Renderer::render(Texture& target, const IndirectArgsBuffer& indirect_args, 
   tm.mark(GS3D_STAGE_BIN, false);
   const uint32_t bin_tiles = bin_max_tiles(indices_.len());
   const int passes = int((ceil_log2(num_tiles) + 7) / 8);
+  // the tile sort's scratch holds kTileSortPasses passes' histograms and
+  // look-back words (16384-pixel targets in 8x8 tiles: 2^22 tiles, 3 passes)
+  if (passes > kTileSortPasses) throw Error(GS3D_ERROR_INVALID_ARGUMENT, "tile grid needs more sort passes than allocated");
   const SortScratch tile_scratch = scratch_->get();
   launch_bin_reset(st, ranges_.as<uint32_t>(), num_tiles, bin_status_.as<uint32_t>(), bin_tiles,
                    bin_ticket_.as<uint32_t>(), pair_count_.as<uint32_t>(), tile_scratch.hist,
@@ -790,7 +864,14 @@ void Renderer::render(Texture& target, const IndirectArgsBuffer& indirect_args, 
   ca.splats = splats_.dev();
   ca.grid = g;
   ca.format = format_;
-  ca.blend = blend_;
+  // The fast blend tests every fragment against the attachment as it was
+  // before the draw, which is what the rasterizer's back-to-front order gives
+  // for Less / LessEqual / Equal / Always even with depth writes. With writes,
+  // Greater / GreaterEqual / NotEqual depend on what the farther splats drawn
+  // before wrote: those draws take the exact (in-order) path.
+  const bool order_dependent = depth && depth_write_ &&
+                               (depth_cmp_ == kCmpGreater || depth_cmp_ == kCmpGreaterEqual || depth_cmp_ == kCmpNotEqual);
+  ca.blend = order_dependent ? kBlendExactUnorm8 : blend_;
   ca.load = clear ? 0 : 1;
   ca.frag_count = counting_ ? frag_count_.as<uint32_t>() : nullptr;
   ca.out = target.dev();
@@ -819,7 +900,6 @@ void Renderer::render(Texture& target, const IndirectArgsBuffer& indirect_args, 
   last_standalone_ = standalone;
   last_clear_ = clear;
   last_depth_ = depth;
-  dev_.note_render(this);
 }
 
 // ---- Viewer --------------------------------------------------------------------------
@@ -856,8 +936,8 @@ void Viewer::update_selection(const uint32_t* words, uint32_t count) {
     throw Error(GS3D_ERROR_INVALID_ARGUMENT,
                 "selection needs " + std::to_string(need) + " words, got " + std::to_string(count));
   if (need == 0) return;
-  // the previous frame may still read the bitset
-  device.wait();
+  // the previous frames may still read the bitset, or be recorded again
+  device.poll();
   if (selection_buffer.size() == 0) selection_buffer = DeviceBuffer(device, size_t(need) * 4);
   hip_check(hipMemcpy(selection_buffer.get(), words, size_t(need) * 4, hipMemcpyHostToDevice), "hipMemcpy(selection)");
   preprocessor.set_selection(selection_buffer.as<uint32_t>());
@@ -868,7 +948,63 @@ void Viewer::set_depth_stencil(bool enabled, CompareFn cmp, bool write) {
   preprocessor.set_splat_depth(enabled ? splats_buffer.depth() : nullptr);
 }
 
+Viewer::State Viewer::state() const {
+  State s{};
+  s.camera = camera_buffer.host();
+  s.model = model_transform_buffer.host();
+  s.gaussian = gaussian_transform_buffer.host();
+  s.blend = renderer.blend();
+  s.tile_w = renderer.tile_w();
+  s.tile_h = renderer.tile_h();
+  return s;
+}
+
+void Viewer::restore(const State& s) {
+  camera_buffer.update_with_pod(s.camera);
+  model_transform_buffer.update_with_pod(s.model);
+  gaussian_transform_buffer.update_with_pod(s.gaussian);
+  renderer.set_blend(s.blend);
+  renderer.set_tile_size(s.tile_w, s.tile_h);
+}
+
+void Viewer::draw(Texture& target, bool standalone) {
+  renderer.render(target, indirect_args_buffer, standalone);
+  // recorded again only over the splats it drew: none of a later frame
+  RecordedUnit u;
+  const uint64_t gen = preprocessor.generation();
+  const State st = state();
+  Texture* t = &target;
+  u.replay = [this, t, standalone, st] {
+    restore(st);
+    draw(*t, standalone);
+  };
+  u.renderers = {&renderer};
+  u.textures = {&target};
+  u.blocker = [this, gen]() -> const char* {
+    return preprocessor.generation() == gen ? nullptr : "the viewer preprocessed another frame after the draw";
+  };
+  device.record_unit(std::move(u));
+}
+
 void Viewer::render_with_depth(Texture& target, Texture& depth, bool load) {
+  {
+    RecordedUnit u;
+    const State st = state();
+    Texture *t = &target, *d = &depth;
+    u.replay = [this, t, d, load, st] {
+      restore(st);
+      render_with_depth(*t, *d, load);
+    };
+    u.renderers = {&renderer};
+    u.textures = {&target, &depth};
+    const bool writes = renderer.depth_write();
+    u.blocker = [load, writes]() -> const char* {
+      if (load) return "the draw loads the target's contents (LoadOp::Load)";
+      if (writes) return "the draw writes its depth attachment";
+      return nullptr;
+    };
+    device.record_unit(std::move(u));
+  }
   splats_buffer.set_rect_grid(renderer.grid_for(target));
   // the projection also writes the depth sort's first histogram
   preprocessor.set_sort_hist(radix_sorter.hist(), radix_sorter.len());
@@ -884,7 +1020,19 @@ void Viewer::render_with_depth(Texture& target, Texture& depth, bool load) {
   renderer.render(target, indirect_args_buffer, false, !load, &depth);
 }
 
-void Viewer::render(Texture& target) {
+void Viewer::render(Texture& target, bool record) {
+  if (record) {
+    RecordedUnit u;
+    const State st = state();
+    Texture* t = &target;
+    u.replay = [this, t, st] {
+      restore(st);
+      render(*t);
+    };
+    u.renderers = {&renderer};
+    u.textures = {&target};
+    device.record_unit(std::move(u));
+  }
   splats_buffer.set_rect_grid(renderer.grid_for(target));
   // the projection also writes the depth sort's first histogram
   preprocessor.set_sort_hist(radix_sorter.hist(), radix_sorter.len());

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -24,6 +25,11 @@
 
 namespace gs3d {
 
+// Radix passes of the (tile, splat) pair sort the renderer's scratch holds:
+// tile ids of the largest grid (16384 x 16384 pixels in 8x8 tiles, 2^22
+// tiles) take 22 bits.
+constexpr int kTileSortPasses = 3;
+
 // Error carrying a gs3d_status (the reference's error enums, src/error.rs).
 struct Error : std::runtime_error {
   gs3d_status code;
@@ -33,6 +39,21 @@ struct Error : std::runtime_error {
 void hip_check(hipError_t e, const char* what);
 
 class Renderer;
+class Texture;
+
+// A unit of recorded work that Device::poll records again when one of its
+// draws overflowed its tile lists (such a draw draws nothing): a whole
+// Viewer::render with the uniforms and settings it was recorded with, one
+// MultiModelViewer pass, one stage-level draw. It is dropped when a texture
+// or renderer it uses is destroyed.
+struct RecordedUnit {
+  std::function<void()> replay;
+  std::vector<Renderer*> renderers;
+  std::vector<const Texture*> textures;
+  // why the unit cannot be recorded again as it was (null: it can), asked
+  // when a replay is needed
+  std::function<const char*()> blocker;
+};
 
 // wgpu::Device + wgpu::Queue: one HIP device and one stream.
 // Raises the error a kernel reported through the device's error word (the
@@ -59,14 +80,19 @@ class Device {
   uint32_t* err() const { return err_; }
   void make_current() const;
 
-  // queue.submit + device.poll(wait). Re-renders frames whose tile lists
-  // overflowed; throws on kernel-reported errors.
+  // queue.submit + device.poll(wait). When a draw since the last poll
+  // overflowed its tile lists, grows them and records every unit since the
+  // last poll again, in order; throws on kernel-reported errors, and when a
+  // unit that needs recording again cannot be.
   void poll();
   void wait();
   // Reads and clears the kernels' error bits; the caller has waited.
   uint32_t take_errors();
-  void note_render(Renderer* r);
-  void forget_renderer(Renderer* r);
+  // Keeps a unit of work until the next poll.
+  void record_unit(RecordedUnit u);
+  // Drops the recorded units that use a renderer / texture being destroyed.
+  void forget_renderer(const Renderer* r);
+  void forget_texture(const Texture* t);
   // Bytes held by DeviceBuffers on this device.
   uint64_t allocated() const { return allocated_; }
   void track(int64_t delta) { allocated_ = uint64_t(int64_t(allocated_) + delta); }
@@ -78,7 +104,7 @@ class Device {
   std::vector<hipStream_t> streams_;
   uint32_t* err_ = nullptr;
   uint64_t allocated_ = 0;
-  std::vector<Renderer*> pending_;
+  std::vector<RecordedUnit> units_;
 };
 
 // Owning device allocation.
@@ -164,20 +190,26 @@ class ModelTransformBuffer {
  public:
   explicit ModelTransformBuffer(Device& dev, int queue = 0);
   void update(const float pos[3], const float rot[4], const float scale[3]);
+  void update_with_pod(const ModelTransformPod& pod);
   const ModelTransformPod* dev() const { return static_cast<const ModelTransformPod*>(buf_.dev()); }
+  const ModelTransformPod& host() const { return pod_; }
 
  private:
   UniformBuffer buf_;
+  ModelTransformPod pod_{};
 };
 
 class GaussianTransformBuffer {
  public:
   explicit GaussianTransformBuffer(Device& dev, int queue = 0);
   void update(float size, int32_t mode, uint32_t sh_deg, bool no_sh0, float max_std_dev);
+  void update_with_pod(const GaussianTransformPod& pod);
   const GaussianTransformPod* dev() const { return static_cast<const GaussianTransformPod*>(buf_.dev()); }
+  const GaussianTransformPod& host() const { return pod_; }
 
  private:
   UniformBuffer buf_;
+  GaussianTransformPod pod_{};
 };
 
 // GaussiansBuffer<G>: the packed model in HBM (structure of streams).
@@ -271,6 +303,9 @@ class SplatsBuffer {
 class Texture {
  public:
   Texture(Device& dev, uint32_t w, uint32_t h, TextureFormat fmt);
+  ~Texture();
+  Texture(const Texture&) = delete;
+  Texture& operator=(const Texture&) = delete;
   uint32_t width() const { return w_; }
   uint32_t height() const { return h_; }
   TextureFormat format() const { return fmt_; }
@@ -338,6 +373,9 @@ class Preprocessor {
     sort_max_ = sort_max;
   }
   void set_invert_selection(bool invert) { invert_selection_ = invert; }
+  // Counts the frames this preprocessor has recorded (a draw recorded after
+  // frame k draws frame k's splats).
+  uint64_t generation() const { return generation_; }
 
  private:
   Device& dev_;
@@ -358,6 +396,7 @@ class Preprocessor {
   uint32_t* sort_hist_ = nullptr;
   uint32_t sort_max_ = 0;
   bool invert_selection_ = true;  // PreprocessorInvertSelectionBuffer::new: initial value true
+  uint64_t generation_ = 0;
 };
 
 // Reusable sort scratch for up to `capacity` pairs and `passes` passes.
@@ -366,6 +405,9 @@ class SortScratchBuffers {
   SortScratchBuffers(Device& dev, uint32_t capacity, int passes);
   SortScratch get() const;
   uint32_t capacity() const { return capacity_; }
+  // Replaces the lists with ones of exactly `pairs` pairs (they still grow
+  // when a frame needs more); waits for the device.
+  void set_capacity(uint32_t pairs);
 
  private:
   Device& dev_;
@@ -415,6 +457,9 @@ class Renderer {
   // depth compare function and depth write of the pipeline, or none.
   void set_depth_stencil(bool enabled, CompareFn cmp, bool write);
   bool depth_enabled() const { return depth_enabled_; }
+  bool depth_write() const { return depth_enabled_ && depth_write_; }
+  // Most (tile, splat) pairs a frame's lists may hold (30-bit status values).
+  static constexpr uint32_t kMaxPairs = 0x3FFFFFFFu;
   // The tile grid of a frame drawn into `target`.
   TileGrid grid_for(const Texture& target) const;
   void set_seg_len(uint32_t s) { seg_len_ = s; }
@@ -424,16 +469,26 @@ class Renderer {
   uint32_t tile_w() const { return tile_w_; }
   uint32_t tile_h() const { return tile_h_; }
   void set_blend(BlendMode b) { blend_ = b; }
+  BlendMode blend() const { return blend_; }
   void set_tile_size(uint32_t w, uint32_t h);
   StageTimer& timer() { return timer_; }
   void set_timer(StageTimer* t) { ext_timer_ = t; }
-  // Called by Device::poll: true if the last frame overflowed its tile
-  // lists (capacity is grown, the frame must be rendered again).
+  // True if a draw since the last call overflowed the tile lists: they are
+  // grown to hold it, and the draw must be recorded again. Throws when no
+  // capacity can hold it. The draws have completed.
   bool grow_if_overflowed();
+  // Records the last draw again (for owners whose every renderer draws once
+  // between waits: the batched and served views).
   void rerender();
+  // rerender() until the last draw fits its lists (at most 3 times more),
+  // waiting for each; the draws so far have completed.
+  void settle();
   // (tile, splat) pairs of the last completed frame, and the list capacity
   uint32_t last_pair_total() const { return *static_cast<volatile uint32_t*>(pair_total_host_); }
   uint32_t capacity() const { return capacity_; }
+  // Replaces the lists with ones of exactly `pairs` pairs (they still grow
+  // when a frame needs more); waits for the device.
+  void set_capacity(uint32_t pairs);
 
  private:
   void ensure_capacity(uint32_t pairs);
@@ -480,6 +535,16 @@ class Renderer {
 // Viewer (src/lib.rs:65-276).
 class Viewer {
  public:
+  // Everything a recorded frame depends on besides the model: the uniforms
+  // and the renderer's settings.
+  struct State {
+    CameraPod camera;
+    ModelTransformPod model;
+    GaussianTransformPod gaussian;
+    BlendMode blend;
+    uint32_t tile_w, tile_h;
+  };
+
   Viewer(Device& dev, TextureFormat format, const gs3d_gaussian* g, uint32_t n, PodKind pod);
   // A viewer over a resident model shared with other viewers, recording on
   // the device's queue `queue`.
@@ -492,7 +557,12 @@ class Viewer {
   void update_gaussian_transform(float size, int32_t mode, uint32_t sh_deg, bool no_sh0, float max_std_dev) {
     gaussian_transform_buffer.update(size, mode, sh_deg, no_sh0, max_std_dev);
   }
-  void render(Texture& target);
+  // record: keep the frame as a unit Device::poll may record again (owners
+  // that handle overflow themselves pass false).
+  void render(Texture& target, bool record = true);
+  State state() const;
+  // Writes the uniforms (in stream order) and settings of a recorded state.
+  void restore(const State& s);
   // SelectionBuffer contents (one bit per Gaussian, ceil(N / 32) words) and
   // PreprocessorInvertSelectionBuffer::update (src/selection/buffer.rs:
   // 151-172). The bitset starts all zero and invert true: nothing is hidden.
@@ -504,6 +574,9 @@ class Viewer {
   // 186-195); load keeps the target's colour (LoadOp::Load).
   void set_depth_stencil(bool enabled, CompareFn cmp, bool write);
   void render_with_depth(Texture& target, Texture& depth, bool load);
+  // The stage-level draw (Renderer::render after separate preprocess and
+  // sort calls), kept as a unit like render().
+  void draw(Texture& target, bool standalone);
 
   Device& device;
   int queue;
