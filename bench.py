@@ -42,7 +42,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-FP32_PEAK_TFLOPS = 157.3
+# MI355X vector fp32 without packed (two-wide) fp32 instructions, which this
+# build disables: half of the 157.3 TFLOP/s packed rate.
+VALU_FP32_PEAK_TFLOPS = 78.6
 
 # Algorithmic bytes of the projection kernel per Gaussian read (f16 pod:
 # position+colour 16, packed f16 rotation/scale 16, f16 SH 96) and per visible
@@ -58,14 +60,26 @@ SORT_BYTES_PER_KEY = 68
 # 4; r^2 from the two terms: 3; alpha exponent 1; exp2 1; weight 1; colour
 # 3 FMAs, 6; transmittance 1 FMA, 2).
 FLOPS_PER_FRAGMENT = 22
-# HBM bytes per launch on this workload: rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE, each scaled by what the same counter reads for a 1 GiB copy
-# (x2.000 and x1.000); profiles/traffic_clustered-6M-1440p-f16_plain.md.
-# k_preprocess: the Gaussian streams plus the sector overhead of the scattered
-# per-Gaussian splat/rectangle writes; k_composite_fast: mostly the 32-byte
-# splat records gathered per (tile, splat) pair, one 128-byte line each.
-PREPROCESS_TRAFFIC_BYTES = 1103224992
-COMPOSITE_TRAFFIC_BYTES = 942950304
+# HBM bytes per launch on this workload come from the committed rocprofv3
+# summary (FETCH_SIZE and WRITE_SIZE runs, each scaled by what the counter
+# reads for a copy of known size); the profile names the kernels it measured.
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "traffic_clustered-6M-1440p-f16_plain.md")
+
+
+def profile_traffic(kernel):
+    """HBM bytes per launch of `kernel` (a substring of its name) from the
+    traffic profile's table, and the profile's path; (None, path) when the
+    profile has no row for it."""
+    rel = os.path.relpath(TRAFFIC_PROFILE, ROOT)
+    try:
+        lines = open(TRAFFIC_PROFILE).read().splitlines()
+    except OSError:
+        return None, rel
+    for ln in lines:
+        cells = [c.strip() for c in ln.strip().strip("|").split("|")]
+        if len(cells) >= 5 and kernel in cells[0] and cells[-1].endswith("MB"):
+            return int(round(float(cells[-1].split()[0].replace(",", "")) * 1e6)), rel
+    return None, rel
 
 MIX_RES = [(640, 360), (1280, 720), (1920, 1080), (3840, 2160)]
 
@@ -276,8 +290,9 @@ def run_serve(gs, dev, seconds, rate, queues, seed=4):
 
 
 def cpu_baseline(gs, g, w, h, threads, budget_s):
-    """The oracle (a CPU restatement of the reference's path) on frames of the
-    same workload, until budget_s of wall time has been spent."""
+    """The oracle (a CPU restatement of the reference's path, not the
+    reference, which cannot be built here) on frames of the benchmark's own
+    workload, until budget_s of wall time has been spent."""
     import oracle as orc
 
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -293,9 +308,15 @@ def cpu_baseline(gs, g, w, h, threads, budget_s):
         el = time.perf_counter() - t0
         if el >= budget_s or frames >= 64:
             break
-    return {"value": round(frames / el, 4), "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": f"{frames} full frame(s) of the 6M-Gaussian 2560x1440 f16 workload (orbit cameras) "
-                      f"through the C oracle (oracle/oracle.c, OpenMP, {threads} threads) in {el:.1f} s"}
+    try:
+        host_cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        host_cores = os.cpu_count()
+    return {"value": round(frames / el, 4), "unit": "frames/s", "cores": threads, "host_cores": host_cores,
+            "kind": "port", "label": "CPU restatement, not the reference",
+            "sample": f"{frames} full frame(s) of the benchmark's own 6M-Gaussian 2560x1440 f16 workload (orbit "
+                      f"cameras; not the 65,536-Gaussian reference case) through the C oracle (oracle/oracle.c, "
+                      f"OpenMP, {threads} threads on a host with {host_cores} available cores) in {el:.1f} s"}
 
 
 # ---------------------------------------------------------------------------
@@ -360,10 +381,13 @@ def main():
     comp_ms = float(stage_ms[5])
     comp_flops = plain["fragments_mean"] * FLOPS_PER_FRAGMENT
     comp_tflops = comp_flops / (comp_ms * 1e-3) / 1e12
-    composite = {"bound": "mfma", "achieved": round(comp_tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                 "frac": round(comp_tflops / FP32_PEAK_TFLOPS, 4), "traffic": COMPOSITE_TRAFFIC_BYTES,
+    comp_traffic, traffic_src = profile_traffic("k_composite_fast")
+    composite = {"bound": "valu", "achieved": round(comp_tflops, 3), "peak": VALU_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                 "frac": round(comp_tflops / VALU_FP32_PEAK_TFLOPS, 4), "traffic": comp_traffic,
+                 "traffic_source": traffic_src,
                  "kernel": "k_composite_fast",
-                 "note": "vector fp32 (exp2 + blend per fragment); no MFMA, compositing is not a contraction",
+                 "note": "vector fp32 (exp2 + blend per fragment) against the non-packed VALU peak; no MFMA, "
+                         "compositing is not a contraction",
                  "avg_launch_ms": round(comp_ms, 4),
                  "fragments_per_launch": int(plain["fragments_mean"]),
                  "flops_per_fragment": FLOPS_PER_FRAGMENT}
@@ -372,8 +396,10 @@ def main():
     pre_ms = float(stage_ms[6])
     pre_bytes = n * R_IN["half"] + v_mean * R_OUT
     pre_gbs = pre_bytes / (pre_ms * 1e-3) / 1e9
+    pre_traffic, _ = profile_traffic("k_preprocess")
     projection = {"bound": "hbm", "achieved": round(pre_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                  "frac": round(pre_gbs / HBM_PEAK_GBS, 4), "traffic": PREPROCESS_TRAFFIC_BYTES,
+                  "frac": round(pre_gbs / HBM_PEAK_GBS, 4), "traffic": pre_traffic,
+                  "traffic_source": traffic_src,
                   "kernel": "k_preprocess<1>",
                   "avg_launch_ms": round(pre_ms, 4),
                   "bytes_per_launch": int(pre_bytes),
