@@ -270,6 +270,25 @@ gs3d_status gs3d_radix_sorter_write(gs3d_radix_sorter* s, const uint32_t* keys, 
 gs3d_status gs3d_radix_sorter_sort(gs3d_radix_sorter* s);
 gs3d_status gs3d_radix_sorter_read(gs3d_radix_sorter* s, uint32_t* keys, uint32_t* vals, uint32_t n);
 
+/* Most pairs gs3d_radix_sort_noalloc sorts in one call. */
+#define GS3D_RADIX_SORT_NOALLOC_MAX_KEYS 4194304u
+/* No counterpart in the reference. An allocation-free variant of
+ * gs3d_radix_sorter_sort for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): sorts the n (key, value) pairs at device addresses keys_in /
+ * vals_in into keys_out / vals_out, stable and ascending by key (the same
+ * kernels and result as gs3d_radix_sorter_sort), on the device's queue 0;
+ * asynchronous, like every launch (gs3d_device_poll waits and reports kernel
+ * errors). The inputs are left unchanged; keys_in may equal keys_out and
+ * vals_in vals_out (an in-place sort). It allocates nothing: its ping-pong
+ * pairs, histograms and look-back words live in one fixed device array per GPU
+ * (37.8 MB, held until the process ends), so n above
+ * GS3D_RADIX_SORT_NOALLOC_MAX_KEYS is GS3D_ERROR_INVALID_ARGUMENT. Every call
+ * on one GPU shares that array: calls that overlap on one GPU (from several
+ * threads, or on streams the caller orders differently) race. */
+gs3d_status gs3d_radix_sort_noalloc(gs3d_device* dev, const uint32_t* keys_in, const uint32_t* vals_in,
+                                    uint32_t* keys_out, uint32_t* vals_out, uint32_t n);
+
 /* ---- resident models ------------------------------------------------------ */
 
 /* A GaussiansBuffer (core's GaussiansBuffer<G>) kept in HBM and shared by

@@ -213,6 +213,7 @@ def _declare(L):
         "gs3d_radix_sorter_write": ([vp, vp, vp, u32], i32),
         "gs3d_radix_sorter_sort": ([vp], i32),
         "gs3d_radix_sorter_read": ([vp, vp, vp, u32], i32),
+        "gs3d_radix_sort_noalloc": ([vp, vp, vp, vp, vp, u32], i32),
         "gs3d_camera_pod_new": ([vp, u32, u32, vp], i32),
         "gs3d_camera_pod_look_at": ([vp, vp, f32, f32, f32, u32, u32, vp], i32),
         "gs3d_gaussian_cov3d": ([vp, vp], i32),
@@ -268,7 +269,7 @@ EXPORTED_SYMBOLS = (
     "gs3d_viewer_read_indirect_args", "gs3d_viewer_read_radix_sort_indirect_args",
     "gs3d_viewer_read_indirect_indices", "gs3d_viewer_read_gaussians_depth", "gs3d_depth_buffer_len",
     "gs3d_radix_sorter_new", "gs3d_radix_sorter_destroy", "gs3d_radix_sorter_write", "gs3d_radix_sorter_sort",
-    "gs3d_radix_sorter_read", "gs3d_camera_pod_new", "gs3d_camera_pod_look_at", "gs3d_gaussian_cov3d",
+    "gs3d_radix_sorter_read", "gs3d_radix_sort_noalloc", "gs3d_camera_pod_new", "gs3d_camera_pod_look_at", "gs3d_gaussian_cov3d",
     "gs3d_pack_half", "gs3d_synth_gaussians", "gs3d_ply_read_file", "gs3d_ply_read_memory",
     "gs3d_spz_read_file", "gs3d_spz_read_memory", "gs3d_model_new", "gs3d_model_destroy",
     "gs3d_multi_model_viewer_new", "gs3d_multi_model_viewer_destroy", "gs3d_multi_model_viewer_insert_model",
@@ -641,6 +642,18 @@ class RadixSorter:
             self.close()
         except Exception:
             pass
+
+
+RADIX_SORT_NOALLOC_MAX_KEYS = 4194304
+
+
+def radix_sort_noalloc(device: Device, keys_in: int, vals_in: int, keys_out: int, vals_out: int, n: int) -> None:
+    """The allocation-free sort (no counterpart in the reference): sorts n
+    (key, value) pairs at device addresses keys_in/vals_in into
+    keys_out/vals_out on the device's queue 0, using a fixed per-GPU scratch
+    array instead of allocating. Asynchronous: Device.poll() waits."""
+    _check(lib().gs3d_radix_sort_noalloc(device.handle, C.c_void_p(keys_in), C.c_void_p(vals_in),
+                                          C.c_void_p(keys_out), C.c_void_p(vals_out), int(n)))
 
 
 def pods_array(pods) -> np.ndarray:

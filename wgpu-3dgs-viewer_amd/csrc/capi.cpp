@@ -10,6 +10,7 @@
 
 #include "../../include/gs3d.h"
 #include "batch.hpp"
+#include "kernels.hpp"
 #include "math.hpp"
 #include "viewer.hpp"
 
@@ -432,6 +433,26 @@ gs3d_status gs3d_radix_sorter_read(gs3d_radix_sorter* s, uint32_t* keys, uint32_
     s->dev.poll();
     gs3d::hip_check(hipMemcpy(keys, s->keys.dev(), size_t(n) * 4, hipMemcpyDeviceToHost), "download keys");
     gs3d::hip_check(hipMemcpy(vals, s->vals.dev(), size_t(n) * 4, hipMemcpyDeviceToHost), "download values");
+  });
+}
+
+gs3d_status gs3d_radix_sort_noalloc(gs3d_device* dev, const uint32_t* keys_in, const uint32_t* vals_in,
+                                    uint32_t* keys_out, uint32_t* vals_out, uint32_t n) {
+  return guard([&] {
+    need(dev != nullptr, "device is null");
+    if (n > GS3D_RADIX_SORT_NOALLOC_MAX_KEYS)
+      throw gs3d::Error(GS3D_ERROR_INVALID_ARGUMENT,
+                        "n exceeds GS3D_RADIX_SORT_NOALLOC_MAX_KEYS (4194304 pairs), the size of the "
+                        "allocation-free sort's fixed scratch");
+    if (n == 0) return;
+    need(keys_in && vals_in && keys_out && vals_out, "null buffer");
+    need(keys_out != vals_out && keys_in != vals_in, "keys and values share a buffer");
+    gs3d::Device& d = *dev->dev;
+    d.make_current();
+    const int r = gs3d::launch_radix_sort_noalloc(d.stream(0), d.num_cus(), d.ordinal(), keys_in, vals_in,
+                                                  keys_out, vals_out, n, d.err());
+    if (r == 2) throw gs3d::Error(GS3D_ERROR_DEVICE, "allocation-free sort scratch not initialised on this device");
+    gs3d::hip_check(hipGetLastError(), "launch allocation-free sort");
   });
 }
 

@@ -86,9 +86,25 @@ size_t sort_lookback_words(uint32_t count_max, int passes);
 // histogram comes from an upsweep over the keys, or, with hist_ready, from
 // whoever wrote the keys: then sc.hist[0..255] already holds the digit-0
 // counts of the n keys and sc.hist[256..] is zero.
+// k_first/v_first (when non-null): pass 0 reads them instead of (k0, v0),
+// which it leaves unread; the passes after it alternate as above.
 int launch_radix_sort(hipStream_t st, int num_cus, SortCount cnt, uint32_t* k0, uint32_t* v0, uint32_t* k1,
                       uint32_t* v1, int passes, const SortScratch& sc, uint32_t* pad_keys = nullptr,
-                      uint32_t pad_begin = 0, uint32_t pad_end = 0, bool hist_ready = false);
+                      uint32_t pad_begin = 0, uint32_t pad_end = 0, bool hist_ready = false,
+                      const uint32_t* k_first = nullptr, const uint32_t* v_first = nullptr);
+
+// The allocation-free sort: a fixed device array per GPU as its scratch.
+// noalloc_sort_init (with the device current, once per device) looks up the
+// array's address on that device. launch_radix_sort_noalloc sorts n pairs from
+// (keys_in, vals_in) into (keys_out, vals_out), stable and ascending, on
+// stream st, allocating nothing; returns 1 when n exceeds
+// GS3D_RADIX_SORT_NOALLOC_MAX_KEYS and 2 when the device was never
+// initialised (nothing launched), 0 otherwise.
+hipError_t noalloc_sort_init(int ordinal);
+size_t noalloc_sort_scratch_bytes();
+int launch_radix_sort_noalloc(hipStream_t st, int num_cus, int ordinal, const uint32_t* keys_in,
+                              const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out, uint32_t n,
+                              uint32_t* err);
 
 // ---- tiles.hip ---------------------------------------------------------------
 

@@ -17,6 +17,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
+#include <mutex>
+#include <unordered_map>
+
+#include "gs3d.h"
 
 #include "device_util.hpp"
 #include "kernels.hpp"
@@ -443,24 +447,24 @@ static int sort_algo() {
 
 int launch_radix_sort(hipStream_t st, int num_cus, SortCount cnt, uint32_t* k0, uint32_t* v0, uint32_t* k1,
                       uint32_t* v1, int passes, const SortScratch& sc, uint32_t* pad_keys, uint32_t pad_begin,
-                      uint32_t pad_end, bool hist_ready) {
+                      uint32_t pad_end, bool hist_ready, const uint32_t* k_first, const uint32_t* v_first) {
   if (sc.max_tiles == 0 || passes <= 0) return 0;
+  // pass 0 reads (k_first, v_first) when given, (k0, v0) otherwise
+  const uint32_t* ks0 = k_first ? k_first : k0;
+  const uint32_t* vs0 = k_first ? v_first : v0;
   if (sort_algo() == 1) {
     // reduce-then-scan: counts [256][tiles] in the look-back words, digit
     // totals in the histogram words; nothing to clear but the pad keys
     if (pad_keys) launch_zero(st, num_cus, cnt, sc, 0, pad_keys, pad_begin, pad_end, false);
-    uint32_t *ks = k0, *vs = v0, *kd = k1, *vd = v1;
     for (int p = 0; p < passes; ++p) {
+      const uint32_t* ks = p == 0 ? ks0 : ((p & 1) ? k1 : k0);
+      const uint32_t* vs = p == 0 ? vs0 : ((p & 1) ? v1 : v0);
+      uint32_t* kd = (p & 1) ? k0 : k1;
+      uint32_t* vd = (p & 1) ? v0 : v1;
       hipLaunchKernelGGL(k_rs_count, dim3(sc.max_tiles), dim3(kRsThreads), 0, st, cnt, ks, 8 * p, sc.lookback);
       hipLaunchKernelGGL(k_rs_colscan, dim3(256), dim3(kRsThreads), 0, st, cnt, sc.lookback, sc.hist);
       hipLaunchKernelGGL(k_rs_scatter, dim3(sc.max_tiles), dim3(kRsThreads), 0, st, cnt, ks, vs, kd, vd,
                          sc.lookback, sc.hist, 8 * p);
-      uint32_t* t = ks;
-      ks = kd;
-      kd = t;
-      t = vs;
-      vs = vd;
-      vd = t;
     }
     return passes & 1;
   }
@@ -469,23 +473,87 @@ int launch_radix_sort(hipStream_t st, int num_cus, SortCount cnt, uint32_t* k0, 
     uint32_t up_blocks = (sc.max_tiles * kRsTile + 4095u) / 4096u;
     if (up_blocks > uint32_t(num_cus) * 4u) up_blocks = uint32_t(num_cus) * 4u;
     if (up_blocks < 1) up_blocks = 1;
-    hipLaunchKernelGGL(k_rs_upsweep, dim3(up_blocks), dim3(256), 0, st, cnt, k0, sc.hist);
+    hipLaunchKernelGGL(k_rs_upsweep, dim3(up_blocks), dim3(256), 0, st, cnt, ks0, sc.hist);
   }
   uint32_t grid = sc.max_tiles;
   if (grid > uint32_t(num_cus) * GS3D_SORT_WG_PER_CU) grid = uint32_t(num_cus) * GS3D_SORT_WG_PER_CU;
-  uint32_t *ks = k0, *vs = v0, *kd = k1, *vd = v1;
   for (int p = 0; p < passes; ++p) {
+    const uint32_t* ks = p == 0 ? ks0 : ((p & 1) ? k1 : k0);
+    const uint32_t* vs = p == 0 ? vs0 : ((p & 1) ? v1 : v0);
+    uint32_t* kd = (p & 1) ? k0 : k1;
+    uint32_t* vd = (p & 1) ? v0 : v1;
     uint32_t* next = (p + 1 < passes) ? sc.hist + (p + 1) * 256 : nullptr;
     hipLaunchKernelGGL(k_rs_onesweep, dim3(grid), dim3(kRsThreads), 0, st, cnt, ks, vs, kd, vd, sc.hist + p * 256,
                        sc.lookback + size_t(p) * sc.max_tiles * 256u, sc.tickets + p, 8 * p, sc.err, next);
-    uint32_t* t = ks;
-    ks = kd;
-    kd = t;
-    t = vs;
-    vs = vd;
-    vd = t;
   }
   return passes & 1;
+}
+
+
+// ---- the allocation-free sort ----------------------------------------------------
+//
+// One fixed array per GPU holds everything a sort of at most
+// GS3D_RADIX_SORT_NOALLOC_MAX_KEYS pairs needs beyond the caller's input and
+// output: the ping-pong keys and values, the look-back words, the four
+// histograms and the tickets. It is zero-filled when the code object loads and
+// lives until the process ends. Host code never passes the array's name to a
+// kernel (on the host it is only a stand-in for the symbol): its device address
+// comes from hipGetSymbolAddress, once per device, at device set-up.
+constexpr uint32_t kNoAllocMaxKeys = GS3D_RADIX_SORT_NOALLOC_MAX_KEYS;
+constexpr uint32_t kNoAllocMaxTiles = (kNoAllocMaxKeys + kRsTile - 1) / kRsTile;
+constexpr int kNoAllocPasses = 4;
+constexpr size_t kNoAllocKeysOff = 0;
+constexpr size_t kNoAllocValsOff = kNoAllocKeysOff + kNoAllocMaxKeys;
+constexpr size_t kNoAllocLookOff = kNoAllocValsOff + kNoAllocMaxKeys;
+constexpr size_t kNoAllocHistOff = kNoAllocLookOff + size_t(kNoAllocMaxTiles) * 256u * kNoAllocPasses;
+constexpr size_t kNoAllocTicketOff = kNoAllocHistOff + 256u * kNoAllocPasses;
+constexpr size_t kNoAllocWords = kNoAllocTicketOff + 64u;  // tickets, padded to 256 bytes
+
+static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
+
+namespace {
+std::mutex g_noalloc_mu;
+std::unordered_map<int, uint32_t*> g_noalloc_ptr;  // device ordinal -> address of `scratch`
+}  // namespace
+
+hipError_t noalloc_sort_init(int ordinal) {
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (g_noalloc_ptr.count(ordinal)) return hipSuccess;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if (p == nullptr) return hipErrorInvalidSymbol;
+  g_noalloc_ptr[ordinal] = static_cast<uint32_t*>(p);
+  return hipSuccess;
+}
+
+size_t noalloc_sort_scratch_bytes() { return kNoAllocWords * sizeof(uint32_t); }
+
+int launch_radix_sort_noalloc(hipStream_t st, int num_cus, int ordinal, const uint32_t* keys_in,
+                              const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out, uint32_t n,
+                              uint32_t* err) {
+  if (n > kNoAllocMaxKeys) return 1;
+  uint32_t* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    const auto it = g_noalloc_ptr.find(ordinal);
+    if (it == g_noalloc_ptr.end()) return 2;
+    base = it->second;
+  }
+  if (n == 0) return 0;
+  SortScratch sc{};
+  sc.hist = base + kNoAllocHistOff;
+  sc.lookback = base + kNoAllocLookOff;
+  sc.tickets = base + kNoAllocTicketOff;
+  sc.max_tiles = kNoAllocMaxTiles;
+  sc.err = err;
+  const SortCount cnt{nullptr, 1u, n};
+  // pass 0 reads the caller's input into the fixed pair, the passes then
+  // alternate between the caller's output and the fixed pair; four passes end
+  // in the output
+  launch_radix_sort(st, num_cus, cnt, keys_out, vals_out, base + kNoAllocKeysOff, base + kNoAllocValsOff,
+                    kNoAllocPasses, sc, nullptr, 0, 0, false, keys_in, vals_in);
+  return 0;
 }
 
 }  // namespace gs3d

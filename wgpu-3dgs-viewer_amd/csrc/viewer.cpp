@@ -29,6 +29,8 @@ Device::Device(int ordinal) : ordinal_(ordinal) {
   add_queue();
   hip_check(hipMalloc(&err_, 256), "hipMalloc(err)");
   hip_check(hipMemset(err_, 0, 256), "hipMemset(err)");
+  // the allocation-free sort's fixed array: its address on this device
+  hip_check(noalloc_sort_init(ordinal), "hipGetSymbolAddress(scratch of the allocation-free sort)");
 }
 
 Device::~Device() {
