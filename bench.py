@@ -55,11 +55,17 @@ R_OUT = 48
 # Depth sort: 4 passes x (read key+value 8, write key+value 8) per key, plus
 # the one histogram read of the keys (4).
 SORT_BYTES_PER_KEY = 68
+# Tile sort: 2 passes x (read key+value 8, write key+value 8) per (tile,
+# splat) pair; its first histogram comes from the binning.
+TILE_SORT_BYTES_PER_PAIR = 32
 # Compositing: floating-point operations per blended fragment, a fused
-# multiply-add counted as 2 (each axis term p.x u'.x + p.y u'.y - k: 2 FMAs,
-# 4; r^2 from the two terms: 3; alpha exponent 1; exp2 1; weight 1; colour
-# 3 FMAs, 6; transmittance 1 FMA, 2).
-FLOPS_PER_FRAGMENT = 22
+# multiply-add counted as 2. Both blends: the two axis terms
+# p.x u'.x + p.y u'.y - k (2 FMAs each, 8), r^2 from them (3), exp2 (1), times
+# the opacity (1). The reference's blend (exact, the default): 1 - a (1),
+# 255 a (1), per colour channel q (1 - a) + 255 c a (a multiply and an FMA,
+# 3) and its rounding (1), the alpha channel's FMA and rounding (3): 30. The
+# f32 blend (fast): weight T a (1), colour (3 FMAs, 6), transmittance (1): 21.
+FLOPS_PER_FRAGMENT = {"exact": 30, "fast": 21}
 # HBM bytes per launch on this workload come from the committed rocprofv3
 # summary (FETCH_SIZE and WRITE_SIZE runs, each scaled by what the counter
 # reads for a copy of known size); the profile names the kernels it measured.
@@ -109,9 +115,10 @@ def percentile(xs, q):
 # sections
 
 
-def run_plain(gs, dev, g, w, h, steps, warmup, tile):
+def run_plain(gs, dev, g, w, h, steps, warmup, tile, blend="exact"):
     viewer = gs.Viewer(dev, gs.TextureFormat.RGBA8_UNORM, g, gs.GaussianPod.HALF)
     viewer.set_tile_size(*tile)
+    viewer.set_blend_mode(gs.BlendMode.EXACT if blend == "exact" else gs.BlendMode.FAST)
     target = gs.Texture(dev, w, h, gs.TextureFormat.RGBA8_UNORM)
     pods = [orbit_pod(gs, w, h, 2.0 * math.pi * k / 64) for k in range(64)]
     for k in range(warmup):
@@ -128,7 +135,7 @@ def run_plain(gs, dev, g, w, h, steps, warmup, tile):
     # per-frame latency and per-stage / per-kernel GPU times from HIP events
     # recorded on the viewer's stream
     viewer.set_profiling(True)
-    lat, stages, vis = [], [], []
+    lat, stages, vis, pairs = [], [], [], []
     for k in range(steps):
         viewer.update_camera_with_pod(pods[k % 64])
         t1 = time.perf_counter()
@@ -137,6 +144,7 @@ def run_plain(gs, dev, g, w, h, steps, warmup, tile):
         lat.append((time.perf_counter() - t1) * 1e3)
         stages.append(viewer.stage_times())
         vis.append(int(viewer.indirect_args()[1]))
+        pairs.append(viewer.pair_count())
     viewer.set_profiling(False)
     stage_ms = np.array(stages).mean(axis=0)
 
@@ -151,7 +159,7 @@ def run_plain(gs, dev, g, w, h, steps, warmup, tile):
     return {
         "fps": steps / elapsed, "ms_per_frame": 1e3 * elapsed / steps,
         "latency_ms": {"p50": percentile(lat, 50), "p99": percentile(lat, 99)},
-        "stage_ms": stage_ms, "visible_mean": float(np.mean(vis)),
+        "stage_ms": stage_ms, "visible_mean": float(np.mean(vis)), "pairs_mean": float(np.mean(pairs)),
         "fragments_mean": float(np.mean(frags)),
     }
 
@@ -303,7 +311,7 @@ def cpu_baseline(gs, g, w, h, threads, budget_s):
     while True:
         pod = orbit_pod(gs, w, h, 2.0 * math.pi * frames / 64)
         frame = orc.frame_from_pod(pod.to_bytes())
-        orc.full_frame(frame, pos, color, sh, cov, w, h, orc.TARGET_RGBA8_F32, threads=threads)
+        orc.full_frame(frame, pos, color, sh, cov, w, h, orc.TARGET_RGBA8_STEP, threads=threads)
         frames += 1
         el = time.perf_counter() - t0
         if el >= budget_s or frames >= 64:
@@ -331,6 +339,9 @@ def main():
     ap.add_argument("--views", type=int, default=8, help="views per batch in the batched section")
     ap.add_argument("--queues", type=int, default=4)
     ap.add_argument("--tile", default="16x16", help="tile size of the plain section")
+    ap.add_argument("--blend", default="exact", choices=["fast", "exact"],
+                    help="blend of the plain section (exact: the reference's per-draw unorm8 rounding, the default "
+                         "of every form; fast: f32 front to back, also timed beside it)")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=400.0)
     ap.add_argument("--serve-queues", type=int, default=8, help="device queues of the serving loop")
@@ -359,8 +370,11 @@ def main():
     out_extra = {}
 
     # the plain section always runs: the rooflines and stage times come from it
-    plain = run_plain(gs, dev, g, w, h, args.steps, args.warmup, tile)
-    log(f"plain: {plain['fps']:.1f} frames/s")
+    plain = run_plain(gs, dev, g, w, h, args.steps, args.warmup, tile, args.blend)
+    log(f"plain ({args.blend} blend): {plain['fps']:.1f} frames/s")
+    other = "fast" if args.blend == "exact" else "exact"
+    plain_other = run_plain(gs, dev, g, w, h, args.steps, args.warmup, tile, other)
+    log(f"plain ({other} blend): {plain_other['fps']:.1f} frames/s")
     batched = None
     if "batched" in sections:
         model = gs.Model(dev, g, gs.GaussianPod.HALF)
@@ -379,18 +393,19 @@ def main():
     stage_ms = plain["stage_ms"]
     v_mean = plain["visible_mean"]
     comp_ms = float(stage_ms[5])
-    comp_flops = plain["fragments_mean"] * FLOPS_PER_FRAGMENT
+    comp_flops = plain["fragments_mean"] * FLOPS_PER_FRAGMENT[args.blend]
     comp_tflops = comp_flops / (comp_ms * 1e-3) / 1e12
-    comp_traffic, traffic_src = profile_traffic("k_composite_fast")
+    comp_kernel = "k_composite_px<%s, false, 256>" % ("true" if args.blend == "exact" else "false")
+    comp_traffic, traffic_src = profile_traffic(comp_kernel)
     composite = {"bound": "valu", "achieved": round(comp_tflops, 3), "peak": VALU_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                  "frac": round(comp_tflops / VALU_FP32_PEAK_TFLOPS, 4), "traffic": comp_traffic,
                  "traffic_source": traffic_src,
-                 "kernel": "k_composite_fast",
+                 "kernel": comp_kernel,
                  "note": "vector fp32 (exp2 + blend per fragment) against the non-packed VALU peak; no MFMA, "
                          "compositing is not a contraction",
                  "avg_launch_ms": round(comp_ms, 4),
                  "fragments_per_launch": int(plain["fragments_mean"]),
-                 "flops_per_fragment": FLOPS_PER_FRAGMENT}
+                 "flops_per_fragment": FLOPS_PER_FRAGMENT[args.blend], "blend": args.blend}
     # the projection pass (k_preprocess), priced on the bytes it must move
     # per launch
     pre_ms = float(stage_ms[6])
@@ -409,21 +424,29 @@ def main():
     # dominate it (compositing and projection); both are reported
     roofline = max((composite, projection), key=lambda r: r["avg_launch_ms"])
     sort_gbs = SORT_BYTES_PER_KEY * v_mean / (float(stage_ms[1]) * 1e-3) / 1e9
+    p_mean = plain["pairs_mean"]
+    tile_gbs = TILE_SORT_BYTES_PER_PAIR * p_mean / (float(stage_ms[3]) * 1e-3) / 1e9
     stage_pre_gbs = pre_bytes / (float(stage_ms[0]) * 1e-3) / 1e9
     hbm = {"preprocess_stage": {"achieved_gbs": round(stage_pre_gbs, 1), "frac": round(stage_pre_gbs / HBM_PEAK_GBS, 4),
                                   "bytes_per_launch": int(pre_bytes), "ms": round(float(stage_ms[0]), 4),
                                 "note": "k_frame_pre + k_preprocess"},
            "depth_sort": {"achieved_gbs": round(sort_gbs, 1), "frac": round(sort_gbs / HBM_PEAK_GBS, 4),
-                          "bytes_per_sort": int(SORT_BYTES_PER_KEY * v_mean), "ms": round(float(stage_ms[1]), 4)}}
+                          "bytes_per_sort": int(SORT_BYTES_PER_KEY * v_mean), "ms": round(float(stage_ms[1]), 4),
+                          "passes": 4, "keys": int(v_mean)},
+           "tile_sort": {"achieved_gbs": round(tile_gbs, 1), "frac": round(tile_gbs / HBM_PEAK_GBS, 4),
+                         "bytes_per_sort": int(TILE_SORT_BYTES_PER_PAIR * p_mean), "ms": round(float(stage_ms[3]), 4),
+                         "passes": 2, "pairs": int(p_mean),
+                         "note": "2 passes x 16 B per pair (tile keys of a 160x90 grid: 14 bits), plus the range scan"}}
 
     names = ["preprocess", "sort", "bin", "tile_sort", "composite", "composite_main_kernel",
              "preprocess_main_kernel"]
     if batched is not None:
         value, ms_step, steps = batched["fps"], batched["ms_per_step"], args.steps
-        form = f"batched: {args.views} views per batch, HIP graph over {args.queues} queues, tile autotuned"
+        form = (f"batched: {args.views} views per batch, HIP graph over {args.queues} queues, tile autotuned, "
+                f"exact blend")
     else:
         value, ms_step, steps = plain["fps"], plain["ms_per_frame"], args.steps
-        form = "plain single view"
+        form = f"plain single view, {args.blend} blend"
     out = {
         "metric": "frames/s at 6M Gaussians 1440p (batched vs plain); p50/p99 served frame latency",
         "value": round(value, 2),
@@ -439,10 +462,18 @@ def main():
         "data": "synthetic (clustered scene generator, seeded)",
         "config": {"workload": "clustered-6M-1440p-f16", "gaussians": n, "width": w, "height": h,
                    "storage": "f16 SH/rotation/scale, u8 opacity", "form": form,
+                   "blend": "exact: back to front, the Rgba8Unorm target rounded after every splat (the reference's "
+                            "fixed-function blend); the default of every form",
                    "visible_mean": int(v_mean)},
         "plain": {"fps": round(plain["fps"], 2), "ms_per_frame": round(plain["ms_per_frame"], 4),
                   "latency_ms": plain["latency_ms"], "tile": args.tile,
-                  "stage_ms": {nm: round(float(x), 4) for nm, x in zip(names, stage_ms)}},
+                  "stage_ms": {nm: round(float(x), 4) for nm, x in zip(names, stage_ms)}, "blend": args.blend},
+        "plain_" + other + "_blend": {
+            "fps": round(plain_other["fps"], 2), "ms_per_frame": round(plain_other["ms_per_frame"], 4),
+            "latency_ms": plain_other["latency_ms"],
+            "stage_ms": {nm: round(float(x), 4) for nm, x in zip(names, plain_other["stage_ms"])},
+            "note": "fast: front to back in f32, rounded once; measured against the reference's per-draw rounding "
+                    "(tests/test_gpu_parity.py::test_reference_blend_full_size): max 6/255, 51.8 dB on this frame"},
         "roofline": roofline,
         "kernel_rooflines": {"projection": projection, "compositing": composite},
         "hbm_rooflines": hbm,

@@ -113,9 +113,11 @@ typedef enum gs3d_display_mode {
 } gs3d_display_mode;
 
 /* How the compositor blends (the reference uses fixed-function
- * ALPHA_BLENDING, src/renderer.rs:296-300). FAST: front to back in f32 with
- * early termination. EXACT: back to front, destination rounded to the target
- * format after every splat, as the blender does. */
+ * ALPHA_BLENDING, src/renderer.rs:296-300). EXACT (the default): back to
+ * front, destination rounded to the target format after every splat, as the
+ * blender does. FAST: front to back in f32 with early termination, rounded
+ * once (on an Rgba8Unorm target it differs from the reference's image where
+ * many faint splats overlap: up to a few /255). */
 typedef enum gs3d_blend_mode {
     GS3D_BLEND_FAST = 0,
     GS3D_BLEND_EXACT = 1
@@ -244,6 +246,10 @@ gs3d_status gs3d_viewer_count_fragments(gs3d_viewer* v, gs3d_texture* target, ui
 
 /* Buffer downloads (the Viewer's pub buffers, src/lib.rs:66-73); each waits
  * for the queue first. */
+/* (tile, splat) pairs in the tile lists: the largest count of the frames the
+ * viewer drew before this call, which waits for them (no reference
+ * counterpart: the reference rasterises instead of binning). */
+gs3d_status gs3d_viewer_pair_count(gs3d_viewer* v, uint32_t* out);
 gs3d_status gs3d_viewer_gaussian_count(gs3d_viewer* v, uint32_t* out);
 /* IndirectArgsBuffer {vertex_count 6, instance_count, 0, 0}
  * (src/buffer/indirect_args.rs:12-26). */

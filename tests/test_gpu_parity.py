@@ -82,7 +82,7 @@ def test_render_config0(gs, orc, device, scene0, blend):
     viewer.render(target)
     device.poll()
     img = target.read()
-    viewer.set_blend_mode(gs.BlendMode.FAST)
+    viewer.set_blend_mode(gs.BlendMode.EXACT)  # the default
     tgt = orc.TARGET_RGBA8_F32 if blend == "fast" else orc.TARGET_RGBA8_STEP
     oidx, okey, oimg = orc.full_frame(frame, pos, color, sh, cov, w, h, tgt, threads=8)
     assert np.array_equal(viewer.indirect_indices(len(oidx)), oidx)
@@ -141,7 +141,6 @@ def test_tile_sizes_identical(gs, device, scene0, tile):
     viewer.render(other)
     device.poll()
     viewer.set_tile_size(16, 16)
-    viewer.set_blend_mode(gs.BlendMode.FAST)
     assert np.array_equal(ref.read(), other.read())
 
 
@@ -183,9 +182,15 @@ def test_needle_splats_fast_blend_drops_no_fragment(gs, orc, device):
     frame = orc.frame_from_pod(pod.to_bytes())
     pos, color, sh, cov = oracle_inputs(orc, g)
     _, _, oimg = orc.full_frame(frame, pos, color, sh, cov, w, h, orc.TARGET_RGBA32F, threads=8)
+    dmax = float(np.abs(a - oimg[..., :3]).max())
     close = (np.abs(a - oimg[..., :3]) <= 2.0 / 255.0).all(axis=2).mean()
-    print(f"needles: {close * 100:.3f}% of pixels within 2/255 of the oracle", flush=True)
-    assert close >= 0.995
+    p = psnr(a, oimg[..., :3], peak=1.0)
+    print(f"needles: max diff {dmax * 255:.3f}/255 from the oracle, {close * 100:.4f}% of pixels within 2/255, "
+          f"PSNR {p:.1f} dB", flush=True)
+    # thin splats take their axes from the exact operation order (the minor
+    # eigenvalue cancels), so the oracle's rounding decides their shape
+    assert dmax <= 2.0 / 255.0
+    assert p >= 45.0
 
 
 @pytest.mark.parametrize("case", range(12))
@@ -216,7 +221,7 @@ def test_random_cameras_bit_exact(gs, orc, device, case):
     device.poll()
     frame = orc.frame_from_pod(pod.to_bytes())
     pos, color, sh, cov = (oracle_inputs_half if half else oracle_inputs)(orc, g)
-    oidx, okey, oimg = orc.full_frame(frame, pos, color, sh, cov, w, h, orc.TARGET_RGBA8_F32, threads=8)
+    oidx, okey, oimg = orc.full_frame(frame, pos, color, sh, cov, w, h, orc.TARGET_RGBA8_STEP, threads=8)
     v = int(viewer.indirect_args()[1])
     assert v == len(oidx)
     assert np.array_equal(viewer.indirect_indices(v), oidx)
@@ -238,7 +243,7 @@ def test_half_pod_config0_bit_exact(gs, orc, device):
     viewer.render(target)
     device.poll()
     pos, color, sh, cov = oracle_inputs_half(orc, g)
-    oidx, okey, oimg = orc.full_frame(frame, pos, color, sh, cov, w, h, orc.TARGET_RGBA8_F32, threads=8)
+    oidx, okey, oimg = orc.full_frame(frame, pos, color, sh, cov, w, h, orc.TARGET_RGBA8_STEP, threads=8)
     v = int(viewer.indirect_args()[1])
     assert v == len(oidx)
     assert np.array_equal(viewer.indirect_indices(v), oidx)
@@ -257,7 +262,7 @@ def test_config1_full_frame(gs, orc, device):
     viewer.render(target)
     device.poll()
     pos, color, sh, cov = oracle_inputs(orc, g)
-    oidx, okey, oimg = orc.full_frame(frame, pos, color, sh, cov, w, h, orc.TARGET_RGBA8_F32, threads=16)
+    oidx, okey, oimg = orc.full_frame(frame, pos, color, sh, cov, w, h, orc.TARGET_RGBA8_STEP, threads=16)
     v = int(viewer.indirect_args()[1])
     assert v == len(oidx)
     assert np.array_equal(viewer.indirect_indices(v), oidx)
@@ -290,13 +295,55 @@ def test_config2_full_frame_half(gs, orc, device):
     frame = orc.frame_from_pod(pod.to_bytes())
     pos, color, sh, cov = oracle_inputs_half(orc, g)
     del g
-    oidx, okey, oimg = orc.full_frame(frame, pos, color, sh, cov, w, h, orc.TARGET_RGBA8_F32, threads=16)
+    oidx, okey, oimg = orc.full_frame(frame, pos, color, sh, cov, w, h, orc.TARGET_RGBA8_STEP, threads=16)
     assert v == len(oidx)
     assert np.array_equal(idx, oidx)
     assert np.array_equal(keys, okey.view(np.uint32))
     d, p = max_abs_diff(img, oimg), psnr(img, oimg)
     print(f"config2: V={v} max diff {d}/255 PSNR {p:.2f} dB", flush=True)
     assert d <= 2 and p >= 45.0
+
+
+@pytest.mark.parametrize("config", [CONFIG1, CONFIG2], ids=["config1", "config2"])
+def test_reference_blend_full_size(gs, orc, device, config):
+    """The image the reference renders: its blender rounds the Rgba8Unorm
+    target after every splat, back to front (src/renderer.rs:296-300,
+    tests/common/given.rs:33). The exact blend does the same and is held to
+    the bar at full size (1M f32 at 1920x1080; 6M f16 at 2560x1440, the
+    benchmark frame); the fast blend (front to back in f32, quantised once),
+    the one the benchmark times, is measured against the same per-step image
+    and held to the bound stated for it."""
+    cfg, n, seed, w, h = config
+    half = cfg == 2
+    _case(f"reference blend config{cfg} {n} {w}x{h}")
+    g = gs.synth_gaussians(cfg, n, seed)
+    pod = camera_pod(gs, cfg, w, h, azimuth=0.0)
+    viewer = gs.Viewer(device, gs.TextureFormat.RGBA8_UNORM, g, gs.GaussianPod.HALF if half else gs.GaussianPod.SINGLE)
+    viewer.update_camera_with_pod(pod)
+    imgs = {}
+    for name, mode in (("exact", gs.BlendMode.EXACT), ("fast", gs.BlendMode.FAST)):
+        viewer.set_blend_mode(mode)
+        t = gs.Texture(device, w, h, gs.TextureFormat.RGBA8_UNORM)
+        viewer.render(t)
+        device.poll()
+        imgs[name] = t.read()
+        t.close()
+    viewer.close()
+    frame = orc.frame_from_pod(pod.to_bytes())
+    pos, color, sh, cov = (oracle_inputs_half if half else oracle_inputs)(orc, g)
+    del g
+    _, _, step = orc.full_frame(frame, pos, color, sh, cov, w, h, orc.TARGET_RGBA8_STEP, threads=16)
+    stats = {}
+    for name, img in imgs.items():
+        diff = np.abs(img.astype(np.int32) - step.astype(np.int32))[..., :3]
+        stats[name] = (int(diff.max()), psnr(img, step), float((diff.max(axis=2) <= 2).mean()),
+                       float(diff.mean()))
+        print(f"config{cfg} {name} blend vs the per-step reference blend: max {stats[name][0]}/255, "
+              f"PSNR {stats[name][1]:.2f} dB, {100 * stats[name][2]:.4f}% of pixels within 2/255, "
+              f"mean {stats[name][3]:.4f}/255", flush=True)
+    d, p, _, _ = stats["exact"]
+    assert d <= 2 and p >= 45.0
+    assert stats["fast"][1] >= 30.0
 
 
 def test_fp16_storage_error_vs_fp64_oracle(gs, orc, device):

@@ -36,7 +36,7 @@ def _oracle_check(gs, orc, g, pod, img, label, viewer=None):
     frame = orc.frame_from_pod(pod.to_bytes())
     pos, color, sh, cov = oracle_inputs_half(orc, g)
     w, h = int(pod.size[0]), int(pod.size[1])
-    oidx, okey, oimg = orc.full_frame(frame, pos, color, sh, cov, w, h, orc.TARGET_RGBA8_F32,
+    oidx, okey, oimg = orc.full_frame(frame, pos, color, sh, cov, w, h, orc.TARGET_RGBA8_STEP,
                                       threads=ORACLE_THREADS)
     d, p = max_abs_diff(img, oimg), psnr(img, oimg)
     print(f"{label}: V={len(oidx)} max diff {d:.0f}/255 PSNR {p:.1f} dB", flush=True)

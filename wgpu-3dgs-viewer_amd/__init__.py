@@ -84,6 +84,9 @@ class DisplayMode(enum.IntEnum):
 
 
 class BlendMode(enum.IntEnum):
+    """EXACT (the default): back to front, the destination rounded to the
+    target format after every splat, as the reference's blender does. FAST:
+    front to back in f32, rounded once."""
     FAST = 0
     EXACT = 1
 
@@ -214,6 +217,7 @@ def _declare(L):
         "gs3d_radix_sorter_sort": ([vp], i32),
         "gs3d_radix_sorter_read": ([vp, vp, vp, u32], i32),
         "gs3d_radix_sort_noalloc": ([vp, vp, vp, vp, vp, u32], i32),
+        "gs3d_viewer_pair_count": ([vp, C.POINTER(u32)], i32),
         "gs3d_camera_pod_new": ([vp, u32, u32, vp], i32),
         "gs3d_camera_pod_look_at": ([vp, vp, f32, f32, f32, u32, u32, vp], i32),
         "gs3d_gaussian_cov3d": ([vp, vp], i32),
@@ -265,7 +269,7 @@ EXPORTED_SYMBOLS = (
     "gs3d_viewer_update_selection", "gs3d_viewer_update_invert_selection",
     "gs3d_viewer_set_depth_stencil", "gs3d_viewer_render_with_depth", "gs3d_texture_write",
     "gs3d_multi_model_viewer_update_selection", "gs3d_multi_model_viewer_update_invert_selection",
-    "gs3d_viewer_gaussian_count",
+    "gs3d_viewer_gaussian_count", "gs3d_viewer_pair_count",
     "gs3d_viewer_read_indirect_args", "gs3d_viewer_read_radix_sort_indirect_args",
     "gs3d_viewer_read_indirect_indices", "gs3d_viewer_read_gaussians_depth", "gs3d_depth_buffer_len",
     "gs3d_radix_sorter_new", "gs3d_radix_sorter_destroy", "gs3d_radix_sorter_write", "gs3d_radix_sorter_sort",
@@ -565,6 +569,13 @@ class Viewer:
     def update_invert_selection(self, invert: bool) -> None:
         """PreprocessorInvertSelectionBuffer::update (initially true)."""
         _check(lib().gs3d_viewer_update_invert_selection(self._h, int(bool(invert))))
+
+    def pair_count(self) -> int:
+        """(tile, splat) pairs in the tile lists: the largest count of the frames
+        drawn before this call (which waits for them)."""
+        n = C.c_uint32()
+        _check(lib().gs3d_viewer_pair_count(self._h, C.byref(n)))
+        return n.value
 
     def count_fragments(self, target: Texture) -> int:
         """Fragments of one frame that pass the discard test (waits)."""

@@ -53,7 +53,7 @@ class MultiModelViewer {
   CameraPod camera_{};
   bool have_camera_ = false;
   GaussianTransformPod gt_{1.0f, pack_gaussian_flags(kModeSplat, 3, false), 3.0f, 0u};
-  BlendMode blend_ = kBlendFast;
+  BlendMode blend_ = kBlendExactUnorm8;
 };
 
 struct ViewSpec {

@@ -345,6 +345,14 @@ gs3d_status gs3d_viewer_count_fragments(gs3d_viewer* v, gs3d_texture* target, ui
   });
 }
 
+gs3d_status gs3d_viewer_pair_count(gs3d_viewer* v, uint32_t* out) {
+  return guard([&] {
+    need(v != nullptr && out != nullptr, "null argument");
+    v->v.device.poll();
+    *out = v->v.renderer.checked_pair_total();
+  });
+}
+
 gs3d_status gs3d_viewer_gaussian_count(gs3d_viewer* v, uint32_t* out) {
   return guard([&] {
     need(v != nullptr && out != nullptr, "null argument");

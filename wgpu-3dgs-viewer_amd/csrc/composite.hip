@@ -5,23 +5,23 @@
 // One workgroup per work item, one lane per pixel of a screen tile (8x8,
 // 16x16 or 32x8 pixels); each wave owns an 8x8 block of the tile. The tile's
 // splats (its run of the tile-sorted pairs, in draw order) are staged through
-// LDS in batches of one splat per lane. While staging, every splat's pixel
-// bound is tested against the tile's 4x4 quadrants, and each quadrant
-// receives, by ballot compaction, the ordered list of the staged splats that
-// can touch it. Most splats of dense scenes cover a few pixels: each 16-lane
-// quadrant blends only its own list.
+// LDS in batches of one splat per lane, and the batch is described per tile
+// column and per tile row by a bitset of the slots whose pixel bound reaches
+// that column / row (ballots). A pixel's splats are the bits of
+// column & row, which its lane walks in slot order: a lane evaluates only
+// the splats whose bound holds its pixel.
 //
-// fast  : front to back in f32 with transmittance. The over operator is
-//         associative, so a tile whose list is longer than one segment is
-//         split into segments composited in parallel, each to a partial
-//         (colour, transmittance) per pixel, and merged in order afterwards:
-//         the few very dense tiles of a clustered scene no longer set the
-//         kernel's length. Multi-segment tiles are scheduled first. A wave
-//         stops when the transmittance of all its lanes is below 1e-4, a
-//         workgroup when all its waves have.
-// exact : back to front, one workgroup per tile; the destination is rounded
-//         to the target format after every splat, as the blender does for an
-//         Rgba8Unorm target (rounding is not associative: no segments).
+// exact : (the default) back to front; on an Rgba8Unorm target the
+//         destination is rounded to 8 bits after every splat, as the blender
+//         does (src/renderer.rs:296-300); rounding is not associative, so a
+//         tile is one work item.
+// fast  : front to back in f32 with transmittance, quantised once. The over
+//         operator is associative, so a tile whose list is longer than one
+//         segment is split into segments composited in parallel, each to a
+//         partial (colour, transmittance) per pixel, and merged in order
+//         afterwards. A lane stops once its transmittance is below 1e-4.
+// Either way the tiles longer than one segment are scheduled first (a work
+// list taken by ticket).
 #include <hip/hip_runtime.h>
 
 #include "device_util.hpp"
@@ -42,51 +42,24 @@ __device__ __forceinline__ uint32_t unorm8(float x) {
   return uint32_t(int(x * 255.0f + 0.5f));
 }
 
-__device__ __forceinline__ float clamp01(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
-
 __device__ __forceinline__ float h_lo(uint32_t w) { return __half2float(__ushort_as_half((unsigned short)(w & 0xFFFFu))); }
 __device__ __forceinline__ float h_hi(uint32_t w) { return __half2float(__ushort_as_half((unsigned short)(w >> 16))); }
 
-// One batch of staged splats plus, per 4x4 quadrant of the tile, the ordered
-// slots that touch it.
-// Slot NT (one past the staged splats) holds a splat of opacity 0 (blending
-// it changes nothing): lanes whose list is shorter than the wave's longest
-// blend it. List rows carry kListPad spare entries so that a step may read
-// past a list's end.
-constexpr uint32_t kListPad = 4;
+// Two f16 values of one word as a vector: converting its elements where they
+// feed an FMA lets the compiler use the mixed-precision FMA (f16 sources,
+// f32 arithmetic) instead of separate conversions.
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ half2v as_h2(uint32_t w) { return __builtin_bit_cast(half2v, w); }
 
-// LDS of a workgroup of NT lanes (one per pixel of its tile), with one list
-// per 4x4 quadrant (NT / 16 lists). Sized per tile shape, so that 8x8 tiles
-// (one wave) take a quarter of a 256-lane tile's LDS and as many workgroups
-// fit a CU.
-template <int NT, bool Z = true>
-struct StageT {
-  static constexpr int kWaves = NT / kWave;
-  static constexpr int kQuads = 4 * kWaves;
-  static constexpr uint32_t kDummy = NT;
-  // splat / ellipse: |qo|^2 log2(e) = (p.u' - k1)^2 + (p.v' - k2)^2 for the
-  // pixel centre p relative to the tile's 32 x 16 cell, u' = u sqrt(log2(e)),
-  // v' likewise, k1 = c.u', k2 = c.v' for the splat's centre c (cell-relative)
-  float4 geo[NT + 1];  // splat: u'.x, u'.y, v'.x, v'.y   point: c.x, c.y, hx, hy
-  float4 col[NT + 1];  // splat: k1, k2, opacity (fast splat: log2 opacity), r   point: 0, 0, opacity, r
-  float2 gb[NT + 1];   // g, b
-  float z[Z ? NT + 1 : 1];  // fragment depth, for depth-tested draws (Z)
-  uint16_t list[kQuads][NT + kListPad];
-  uint64_t qbal[kWaves][kQuads];  // [staging wave][quadrant] ballots
-};
-using Stage = StageT<kCompMaxThreads>;
-
-// The fast path stages log2(opacity) for splat mode (its alpha is then one
-// exp2 of log2(opacity) - |qo|^2 log2(e)): the dummy's opacity 0 is -inf.
-template <int NT, bool Z>
-__device__ __forceinline__ void init_dummy(StageT<NT, Z>& s, int mode) {
-  constexpr uint32_t kDummy = StageT<NT, Z>::kDummy;
-  if (threadIdx.x == 0) {
-    s.geo[kDummy] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    s.col[kDummy] = make_float4(0.0f, 0.0f, mode == kModeSplat ? -__builtin_inff() : 0.0f, 0.0f);
-    s.gb[kDummy] = make_float2(0.0f, 0.0f);
-    if (Z) s.z[kDummy] = 0.0f;
-  }
+// f16 bits of clamp01 of an f16 value (exact: [0, 1] of an f16 is an f16).
+__device__ __forceinline__ uint32_t clamp01_h2(uint32_t w) {
+  const half2v h = as_h2(w);
+  const _Float16 lo = h.x > _Float16(1.0f) ? _Float16(1.0f) : (h.x > _Float16(0.0f) ? h.x : _Float16(0.0f));
+  const _Float16 hi = h.y > _Float16(1.0f) ? _Float16(1.0f) : (h.y > _Float16(0.0f) ? h.y : _Float16(0.0f));
+  half2v r;
+  r.x = lo;
+  r.y = hi;
+  return __builtin_bit_cast(uint32_t, r);
 }
 
 struct Tiling {
@@ -120,109 +93,6 @@ __device__ __forceinline__ Tiling setup_tile(const TileGrid& g, uint32_t tile) {
   return t;
 }
 
-// Stage one splat record into slot tid. `log2op`: store log2(opacity) (the
-// fast path's splat mode) instead of the opacity.
-template <int NT, bool Z>
-__device__ __forceinline__ void stage_splat(StageT<NT, Z>& s, uint32_t tid, const Splat2D& r, const Tiling& t,
-                                            bool clamp_src, int mode, bool log2op) {
-  float cr = h_lo(r.rg), cg = h_hi(r.rg), cb = h_lo(r.ba);
-  const float op = h_hi(r.ba);
-  if (clamp_src) {
-    cr = clamp01(cr);
-    cg = clamp01(cg);
-    cb = clamp01(cb);
-  }
-  const float cx = r.cx - t.sx, cy = r.cy - t.sy;
-  if (mode == kModePoint) {
-    s.geo[tid] = make_float4(cx, cy, r.ux, r.uy);
-    s.col[tid] = make_float4(0.0f, 0.0f, op, cr);
-  } else {
-    const float ux = r.ux * kSqrtLog2e, uy = r.uy * kSqrtLog2e;
-    const float vx = -r.rho * uy, vy = r.rho * ux;
-    s.geo[tid] = make_float4(ux, uy, vx, vy);
-    // log2(0) = -inf: exp2 gives alpha 0
-    s.col[tid] = make_float4(cx * ux + cy * uy, cx * vx + cy * vy, log2op ? __log2f(op) : op, cr);
-  }
-  s.gb[tid] = make_float2(cg, cb);
-}
-
-// Stage splat k (or nothing) into slot threadIdx.x and build one ordered list
-// per 4x4 quadrant of the tile (row-major over the tile's quadrants; each
-// wave blends the four of its 8x8 block). (Testing each quadrant against the
-// splat's ellipse, as the binning does per tile, shortens the lists but costs
-// more in staging than it saves in blending: measured.) Contains
-// __syncthreads; every lane must call it.
-template <int NT, bool Z>
-__device__ __forceinline__ void stage_batch_quads(StageT<NT, Z>& s, const Tiling& t, const Splat2D* splats,
-                                                  const float* zsrc, bool valid, uint32_t k, bool clamp_src,
-                                                  int mode, bool log2op) {
-  const uint32_t tid = threadIdx.x, lane = tid % kWave, sw = tid / kWave;
-  uint32_t mask = 0u;
-  if (valid) {
-    const Splat2D r = splats[k];
-    stage_splat(s, tid, r, t, clamp_src, mode, log2op);
-    if (Z && zsrc) s.z[tid] = zsrc[k];
-    // Quadrant column c of the tile holds pixel centres [4c + 0.5, 4c + 3.5]
-    // (tile-relative); the bound [x0, x1] may touch it when x1 >= 4c and
-    // x0 <= 4c + 4 (half a pixel of margin). Same for rows. Quadrant (c, r)
-    // is list r ncols + c (row-major over the tile's quadrants).
-    const float hx = h_lo(r.hxy), hy = h_hi(r.hxy);
-    const float x0 = r.cx - hx - t.ox, x1 = r.cx + hx - t.ox;
-    const float y0 = r.cy - hy - t.oy, y1 = r.cy + hy - t.oy;
-    const int ncols = int(t.blocks_x) * 2, nrows = int(t.waves / t.blocks_x) * 2;
-    const int c0 = max(0, int(ceilf((x0 - 4.0f) * 0.25f))), c1 = min(ncols - 1, int(floorf(x1 * 0.25f)));
-    const int r0 = max(0, int(ceilf((y0 - 4.0f) * 0.25f))), r1 = min(nrows - 1, int(floorf(y1 * 0.25f)));
-    if (c0 <= c1 && r0 <= r1) {
-      const uint32_t colbits = (2u << c1) - (1u << c0);  // bits c0 .. c1
-      for (int rr = r0; rr <= r1; ++rr) mask |= colbits << (rr * ncols);
-    }
-  }
-  constexpr int kQuads = StageT<NT, Z>::kQuads;
-#pragma unroll
-  for (int q = 0; q < kQuads; ++q) {
-    const uint64_t bal = __ballot((mask >> q) & 1u);
-    if (lane == 0) s.qbal[sw][q] = bal;
-  }
-  __syncthreads();
-  // this wave's first entry in each list: the earlier staging waves' counts
-  // (lane q < kQuads holds list q's)
-  uint32_t first = 0;
-  if (lane < uint32_t(kQuads))
-    for (uint32_t p = 0; p < sw; ++p) first += uint32_t(__popcll(s.qbal[p][lane]));
-  // every lane appends its slot to the lists of its quadrants at first +
-  // its rank in this wave's ballot; one uniform step per list (a per-lane
-  // walk over its set bits would run as long as the wave's largest splat)
-#pragma unroll
-  for (int q = 0; q < kQuads; ++q) {
-    const uint32_t off = __builtin_amdgcn_readlane(first, q);
-    // the ballot again, rather than kept live across the barrier; quadrant
-    // lists hold the slot's byte offset in geo / col (16 per slot)
-    const uint64_t bal = __ballot((mask >> q) & 1u);
-    if ((mask >> q) & 1u) s.list[q][off + mask_rank(bal)] = uint16_t(tid * 16u);
-  }
-  __syncthreads();
-}
-
-// Fragment alpha (render.wesl:143-166) of staged slot q at this lane's pixel
-// centre; negative when the fragment is discarded. sd2l = sd^2 log2(e),
-// ell2l = (sd - 0.1)^2 log2(e).
-__device__ __forceinline__ float fragment(const Stage& s, uint32_t q, const Tiling& t, int mode, float sd2l,
-                                          float ell2l, float4& col) {
-  const float4 g = s.geo[q];
-  col = s.col[q];
-  if (mode == kModePoint) {
-    const float dx = t.fpx - g.x;
-    const float dy = t.fpy - g.y;
-    return (dx >= -g.z && dx < g.z && dy >= -g.w && dy < g.w) ? 1.0f : -1.0f;
-  }
-  // |qo|^2 log2(e)
-  const float t1 = (t.fpx * g.x + t.fpy * g.y) - col.x, t2 = (t.fpx * g.z + t.fpy * g.w) - col.y;
-  const float r2 = t1 * t1 + t2 * t2;
-  if (!(r2 <= sd2l)) return -1.0f;
-  if (mode == kModeEllipse) return col.z + (1.0f - col.z) * ((r2 > ell2l) ? 1.0f : 0.0f);
-  return col.z * __builtin_amdgcn_exp2f(-r2);
-}
-
 // Depth test of a fragment at depth z against the attachment's value d.
 __device__ __forceinline__ bool depth_pass(int cmp, float z, float d) {
   switch (cmp) {
@@ -239,25 +109,30 @@ __device__ __forceinline__ bool depth_pass(int cmp, float z, float d) {
 
 // Fragment alpha for the fast path: 0 for a discarded fragment (blending
 // alpha 0 leaves colour and transmittance unchanged), so the loop has no
-// branches. `kept` tells whether the fragment survives the discard test.
+// branches. `kept` tells whether the fragment survives the discard test. In
+// splat mode the opacity is left out (the caller folds it into the
+// transmittance factor, one mixed-precision FMA).
 template <int MODE>
 __device__ __forceinline__ float fragment_alpha(float4 g, float4 c, const Tiling& t, float sd2l, float ell2l,
                                                 bool& kept) {
 #pragma clang fp contract(fast)
+  const half2v bo = __builtin_bit_cast(half2v, c.w);  // b, opacity
   if (MODE == kModePoint) {
     const float dx = t.fpx - g.x;
     const float dy = t.fpy - g.y;
     kept = dx >= -g.z && dx < g.z && dy >= -g.w && dy < g.w;
-    return kept ? c.z : 0.0f;
+    return kept ? float(bo.y) : 0.0f;
   }
   const float t1 = __builtin_fmaf(t.fpx, g.x, __builtin_fmaf(t.fpy, g.y, -c.x));
   const float t2 = __builtin_fmaf(t.fpx, g.z, __builtin_fmaf(t.fpy, g.w, -c.y));
   const float r2 = __builtin_fmaf(t1, t1, t2 * t2);
   float al;
-  if (MODE == kModeEllipse)
-    al = c.z + (1.0f - c.z) * ((r2 > ell2l) ? 1.0f : 0.0f);
-  else
-    al = __builtin_amdgcn_exp2f(c.z - r2);  // c.z = log2(opacity)
+  if (MODE == kModeEllipse) {
+    const float op = float(bo.y);
+    al = op + (1.0f - op) * ((r2 > ell2l) ? 1.0f : 0.0f);
+  } else {
+    al = __builtin_amdgcn_exp2f(-r2);
+  }
   kept = r2 <= sd2l;
   return kept ? al : 0.0f;
 }
@@ -270,103 +145,6 @@ struct DepthLane {
   float first;
   bool found;
 };
-
-// Splats blended per step of the fast loop (1, 2 or 4): more overlaps the
-// LDS reads of a step, fewer keeps registers for occupancy.
-#ifndef GS3D_COMPOSITE_UNROLL
-#define GS3D_COMPOSITE_UNROLL 2
-#endif
-constexpr int kUnroll = GS3D_COMPOSITE_UNROLL;
-static_assert(kUnroll == 1 || kUnroll == 2 || kUnroll == 4, "unroll must be 1, 2 or 4");
-static_assert(kListPad % kUnroll == 0, "a step reads at most kListPad entries past a list");
-
-// Front-to-back blend of the lane's quadrant list, kUnroll entries per step;
-// a wave steps through its longest quadrant list and the lanes of shorter
-// lists blend the dummy slot. Lanes past the transmittance threshold keep
-// blending (their colour only gets closer to the exact sum) until the whole
-// wave is done.
-template <int MODE, bool DEPTH, int NT>
-__device__ __forceinline__ void blend_quads(StageT<NT, DEPTH>& s, const Tiling& t, uint32_t w, float sd2l, float ell2l,
-                                            float& T, float& cr, float& cg, float& cb, bool& done, DepthLane& dl) {
-#pragma clang fp contract(fast)
-  const uint32_t lane = threadIdx.x % kWave;
-  // this wave's 8x8 block holds quadrant columns 2 bx, 2 bx + 1 and rows
-  // 2 by, 2 by + 1; the lane's quadrant is (qx, qy) within it
-  const uint32_t ncols = 2u * t.blocks_x, bx = w % t.blocks_x, by = w / t.blocks_x;
-  const uint32_t q00 = 2u * by * ncols + 2u * bx;
-  const uint32_t myq = q00 + ((lane >> 5) ? ncols : 0u) + ((lane >> 2) & 1u);
-  uint32_t lens[4] = {0u, 0u, 0u, 0u};
-  for (uint32_t p = 0; p < t.waves; ++p) {
-#pragma unroll
-    for (uint32_t qd = 0; qd < 4; ++qd) lens[qd] += uint32_t(__popcll(s.qbal[p][q00 + (qd >> 1) * ncols + (qd & 1u)]));
-  }
-  const uint32_t mylen = lens[((lane >> 2) & 1u) | ((lane >> 5) << 1)];
-  // the same for the whole wave: keep the loop bound in a scalar register
-  const uint32_t wmax =
-      __builtin_amdgcn_readfirstlane(max(max(lens[0], lens[1]), max(lens[2], lens[3])));
-  constexpr uint32_t kDummyOff = StageT<NT, DEPTH>::kDummy * 16u;
-  // pad this wave's four lists with the dummy up to the longest one (whole
-  // steps), so that every step reads list entries without a bounds check;
-  // only this wave reads its lists, and its LDS accesses run in order
-  {
-    const uint32_t wend = (wmax + uint32_t(kUnroll) - 1u) & ~uint32_t(kUnroll - 1);
-    const uint32_t qlane = (lane & 3u) | (((lane >> 3) & 3u) << 2);
-    for (uint32_t e = mylen + qlane; e < wend; e += 16u) s.list[myq][e] = uint16_t(kDummyOff);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  }
-  const char* geo = reinterpret_cast<const char*>(s.geo);
-  const char* col = reinterpret_cast<const char*>(s.col);
-  const char* gbs = reinterpret_cast<const char*>(s.gb);
-  const char* zs = reinterpret_cast<const char*>(s.z);
-  for (uint32_t j = 0; j < wmax; j += kUnroll) {
-    if ((j & 31u) == 0u && __all(done)) break;
-    uint32_t q[kUnroll];  // byte offsets of the slots in geo / col
-    if (kUnroll == 1) {
-      q[0] = s.list[myq][j];
-    } else {
-#pragma unroll
-      for (int k = 0; k < kUnroll; k += 2) {
-        const uint32_t qq = *reinterpret_cast<const uint32_t*>(&s.list[myq][j + k]);
-        q[k] = qq & 0xFFFFu;
-        q[k + 1 < kUnroll ? k + 1 : 0] = qq >> 16;
-      }
-    }
-    float4 g[kUnroll], c[kUnroll];
-    float2 gb[kUnroll];
-    float zz[kUnroll];
-#pragma unroll
-    for (int k = 0; k < kUnroll; ++k) {
-      g[k] = *reinterpret_cast<const float4*>(geo + q[k]);
-      c[k] = *reinterpret_cast<const float4*>(col + q[k]);
-      gb[k] = *reinterpret_cast<const float2*>(gbs + (q[k] >> 1));
-      if (DEPTH) zz[k] = *reinterpret_cast<const float*>(zs + (q[k] >> 2));
-    }
-#pragma unroll
-    for (int k = 0; k < kUnroll; ++k) {
-      bool kept;
-      float al = fragment_alpha<MODE>(g[k], c[k], t, sd2l, ell2l, kept);
-      if (DEPTH) {
-        // the test is against the attachment as it was before the draw: for
-        // Less / LessEqual and back-to-front order, what depth writes by
-        // farther splats would change only matters at equal depths
-        const bool pass = depth_pass(dl.cmp, zz[k], dl.ref) && q[k] != kDummyOff;
-        al = pass ? al : 0.0f;
-        if (kept && pass && !dl.found) {
-          dl.first = zz[k];
-          dl.found = true;
-        }
-      }
-      const float wgt = T * al;
-      cr += wgt * c[k].w;
-      cg += wgt * gb[k].x;
-      cb += wgt * gb[k].y;
-      T = __builtin_fmaf(-T, al, T);
-    }
-    done = T < kMinTransmittance;
-  }
-}
 
 // Final pixel: the front-to-back colour plus what is left of the background
 // (black with alpha 1 after a clear; the target's contents when loading).
@@ -402,6 +180,16 @@ __device__ __forceinline__ uint32_t segments_of(uint32_t len, uint32_t seg) {
   return len <= seg ? 1u : (len + seg - 1u) / seg;
 }
 
+// Work items of a tile: its segments (fast blend); one for the exact blend,
+// which cannot split a tile. Either way a tile longer than one segment is
+// "heavy" and scheduled first.
+__device__ __forceinline__ void tile_items(const CompositeArgs& a, uint32_t len, uint32_t& heavy, uint32_t& light) {
+  const uint32_t ns = segments_of(len, a.seg_len);
+  const uint32_t n = a.blend != kBlendFast ? 1u : ns;
+  heavy = ns > 1 ? n : 0u;
+  light = ns > 1 ? 0u : 1u;
+}
+
 // Work list: one item per (tile, segment); the segments of multi-segment
 // tiles come first (heavy-first), then one item per other tile. Built in two
 // launches over blocks of kPlanTiles tiles: per-block counts, then each block
@@ -434,11 +222,7 @@ __global__ __launch_bounds__(kPlanTiles) void k_plan_count(CompositeArgs a) {
   const uint32_t nt = a.grid.tiles_x * a.grid.tiles_y;
   const uint32_t t = blockIdx.x * kPlanTiles + threadIdx.x;
   uint32_t heavy = 0, light = 0;
-  if (t < nt) {
-    const uint32_t ns = segments_of(a.ranges[2 * t + 1] - a.ranges[2 * t], a.seg_len);
-    heavy = ns > 1 ? ns : 0u;
-    light = ns > 1 ? 0u : 1u;
-  }
+  if (t < nt) tile_items(a, a.ranges[2 * t + 1] - a.ranges[2 * t], heavy, light);
   uint32_t th, tl;
   block_exclusive_scan(heavy, s_wave[0], th);
   block_exclusive_scan(light, s_wave[1], tl);
@@ -489,20 +273,16 @@ __global__ __launch_bounds__(kPlanTiles) void k_plan(CompositeArgs a) {
     la += s_red[3][q];
   }
   const uint32_t t = blockIdx.x * kPlanTiles + threadIdx.x;
-  uint32_t ns = 0, heavy = 0, light = 0;
-  if (t < nt) {
-    ns = segments_of(a.ranges[2 * t + 1] - a.ranges[2 * t], a.seg_len);
-    heavy = ns > 1 ? ns : 0u;
-    light = ns > 1 ? 0u : 1u;
-  }
+  uint32_t heavy = 0, light = 0;
+  if (t < nt) tile_items(a, a.ranges[2 * t + 1] - a.ranges[2 * t], heavy, light);
   uint32_t th, tl;
   const uint32_t hx = block_exclusive_scan(heavy, s_wave[0], th);
   const uint32_t lx = block_exclusive_scan(light, s_wave[1], tl);
   if (t < nt) {
-    if (ns > 1) {
+    if (heavy > 0) {
       uint32_t item = hb + hx;
       a.seg_base[t] = item;
-      for (uint32_t sg = 0; sg < ns; ++sg, ++item) {
+      for (uint32_t sg = 0; sg < heavy; ++sg, ++item) {
         if (item < max_items) {
           a.item_tile[item] = t;
           a.item_seg[item] = sg;
@@ -524,66 +304,6 @@ __global__ __launch_bounds__(kPlanTiles) void k_plan(CompositeArgs a) {
     a.plan[1] = ha;
     // the work ticket, in a cache line of its own
     a.plan[kPlanTicketWord] = 0u;
-  }
-}
-
-template <bool DEPTH, int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_composite_fast(CompositeArgs a) {
-#pragma clang fp contract(fast)
-  if (*a.pair_count > a.capacity) return;  // truncated lists: the frame is drawn again
-  __shared__ StageT<NT, DEPTH> s;
-  __shared__ uint32_t s_item;
-  const FrameConsts& fc = *a.fc;
-  const int mode = fc.display_mode;
-  const float sd2l = fc.max_std_dev * fc.max_std_dev * kLog2e;
-  const float e = fc.max_std_dev - 0.1f;
-  const float ell2l = e * e * kLog2e;
-  const bool q8 = a.format == kFormatRgba8Unorm;
-  const uint32_t nt = blockDim.x, tid = threadIdx.x, w = tid / kWave;
-  // read once: loads from the ticket's cache line while every workgroup
-  // takes tickets from it wait behind the atomics (measured: 3x slower)
-  const uint32_t items = a.plan[0], heavy = a.plan[1];
-  init_dummy(s, mode);
-
-  for (;;) {
-    const uint32_t item = take_ticket(a.plan + kPlanTicketWord, &s_item);
-    if (item >= items) break;
-    const uint32_t tile = a.item_tile[item], seg = a.item_seg[item];
-    const Tiling t = setup_tile(a.grid, tile);
-    const uint32_t tstart = a.ranges[2 * tile], tend = a.ranges[2 * tile + 1];
-    // segment counted from the front (nearest end) of the tile's list
-    const uint32_t front = tend - seg * a.seg_len;
-    const uint32_t back = (item < heavy && front - tstart > a.seg_len) ? front - a.seg_len : tstart;
-    const size_t o = size_t(t.py) * a.grid.width + t.px;
-    DepthLane dl{a.depth_cmp, 0.0f, 0.0f, false};
-    if (DEPTH && t.inside) dl.ref = a.depth[o];
-
-    float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
-    bool done = !t.inside;
-    for (uint32_t bend = front; bend > back;) {
-      const uint32_t cnt = (bend - back) < nt ? (bend - back) : nt;
-      __syncthreads();
-      const bool valid = tid < cnt;
-      // nearest first
-      stage_batch_quads(s, t, a.splats, DEPTH ? a.splat_depth : nullptr, valid,
-                        valid ? a.pair_vals[bend - 1 - tid] : 0u, q8, mode, mode == kModeSplat);
-      if (mode == kModeSplat)
-        blend_quads<kModeSplat, DEPTH, NT>(s, t, w, sd2l, ell2l, T, cr, cg, cb, done, dl);
-      else if (mode == kModeEllipse)
-        blend_quads<kModeEllipse, DEPTH, NT>(s, t, w, sd2l, ell2l, T, cr, cg, cb, done, dl);
-      else
-        blend_quads<kModePoint, DEPTH, NT>(s, t, w, sd2l, ell2l, T, cr, cg, cb, done, dl);
-      bend -= cnt;
-      if (__syncthreads_and(done ? 1 : 0)) break;
-    }
-    if (t.inside) {
-      if (item < heavy) {
-        a.partial[size_t(item) * nt + tid] = make_float4(cr, cg, cb, T);
-      } else {
-        store_pixel(a, t, cr, cg, cb, T);
-        if (DEPTH && a.depth_write && dl.found) a.depth[o] = dl.first;
-      }
-    }
   }
 }
 
@@ -609,87 +329,266 @@ __global__ __launch_bounds__(kCompMaxThreads) void k_merge(CompositeArgs a) {
   store_pixel(a, t, cr, cg, cb, T);
 }
 
-__global__ __launch_bounds__(kCompMaxThreads) void k_composite_exact(CompositeArgs a) {
+// ---- per-pixel lists ------------------------------------------------------------
+//
+// A batch of NT staged splats (one per lane, in list order) is described per
+// tile column and per tile row by a bitset of the slots whose pixel bound (the
+// pixel centres within the splat's half extents) reaches that column / row:
+// NT / 64 ballots per column and row. A pixel's splats are the bits of
+// colm[x] & rowm[y], which its lane walks in slot order, 32-slot word after
+// word (words with no bit for the pixel skipped), so a lane evaluates only the
+// splats whose bound holds its pixel (a discarded fragment inside the bound
+// costs a step; a splat beyond it costs nothing) and a wave steps as often as
+// its busiest lane has splats. The exact blend keeps the destination as an
+// integer 0..255 per channel on an Rgba8Unorm target:
+// q' = round(q (1 - a) + 255 c a), the f32 blend followed by the unorm8
+// rounding of every draw.
+
+constexpr int kPxMaxCols = 32, kPxMaxRows = 16;
+
+template <int NT, bool Z>
+struct PxStage {
+  static constexpr int kWords = NT / 32;
+  float4 geo[NT];
+  float4 col[NT];
+  float z[Z ? NT : 1];
+  uint32_t colm[kPxMaxCols][kWords];
+  uint32_t rowm[kPxMaxRows][kWords];
+};
+
+// Stages splat k (or nothing) into slot threadIdx.x and builds the column and
+// row bitsets of the batch. Contains __syncthreads; every lane must call it.
+template <int NT, bool Z>
+__device__ __forceinline__ void stage_batch_px(PxStage<NT, Z>& s, const Tiling& t, const TileGrid& g,
+                                               const Splat2D* splats, const float* zsrc, bool valid, uint32_t k,
+                                               bool clamp_src, int mode) {
+  const uint32_t tid = threadIdx.x, lane = tid % kWave, sw = tid / kWave;
+  int xa = 1, xb = 0, ya = 1, yb = 0;  // empty
+  if (valid) {
+    const Splat2D r = splats[k];
+    const uint32_t rg = clamp_src ? clamp01_h2(r.rg) : r.rg;
+    const uint32_t ba = clamp_src ? ((clamp01_h2(r.ba) & 0xFFFFu) | (r.ba & 0xFFFF0000u)) : r.ba;
+    const float cx = r.cx - t.sx, cy = r.cy - t.sy;
+    if (mode == kModePoint) {
+      s.geo[tid] = make_float4(cx, cy, r.ux, r.uy);
+      s.col[tid] = make_float4(0.0f, 0.0f, __uint_as_float(rg), __uint_as_float(ba));
+    } else {
+      const float ux = r.ux * kSqrtLog2e, uy = r.uy * kSqrtLog2e;
+      const float vx = -r.rho * uy, vy = r.rho * ux;
+      s.geo[tid] = make_float4(ux, uy, vx, vy);
+      s.col[tid] = make_float4(cx * ux + cy * uy, cx * vx + cy * vy, __uint_as_float(rg), __uint_as_float(ba));
+    }
+    if (Z && zsrc) s.z[tid] = zsrc[k];
+    if (r.hxy != 0u) {
+      // pixel x of the tile holds the centre ox + x + 0.5: inside the bound
+      // [cx - hx, cx + hx] for x in [ceil(cx - hx - ox - 0.5), floor(cx + hx - ox - 0.5)]
+      const float hx = h_lo(r.hxy), hy = h_hi(r.hxy);
+      const float fx = r.cx - t.ox - 0.5f, fy = r.cy - t.oy - 0.5f;
+      xa = max(0, int(ceilf(fx - hx)));
+      xb = min(int(g.tile_w) - 1, int(floorf(fx + hx)));
+      ya = max(0, int(ceilf(fy - hy)));
+      yb = min(int(g.tile_h) - 1, int(floorf(fy + hy)));
+    }
+  }
+  // column c's and row r's ballots: words 2 sw and 2 sw + 1 of their bitsets
+  for (int c = 0; c < int(g.tile_w); ++c) {
+    const uint64_t bal = __ballot(xa <= c && c <= xb);
+    if (lane == 0) {
+      s.colm[c][2 * sw] = uint32_t(bal);
+      s.colm[c][2 * sw + 1] = uint32_t(bal >> 32);
+    }
+  }
+  for (int r = 0; r < int(g.tile_h); ++r) {
+    const uint64_t bal = __ballot(ya <= r && r <= yb);
+    if (lane == 0) {
+      s.rowm[r][2 * sw] = uint32_t(bal);
+      s.rowm[r][2 * sw + 1] = uint32_t(bal >> 32);
+    }
+  }
+  __syncthreads();
+}
+
+// Blend of one batch for this lane's pixel (tile-relative lx, ly), slots in
+// increasing order. EXACT: back to front (q8: the 8-bit per-draw rounding);
+// otherwise front to back with transmittance T, a lane stopping once below
+// the threshold (checked every 16 steps).
+template <int MODE, bool EXACT, bool DEPTH, int NT>
+__device__ __forceinline__ void blend_px(const PxStage<NT, DEPTH>& s, const Tiling& t, uint32_t lx, uint32_t ly,
+                                         bool q8, float sd2l, float ell2l, float& T, float& cr, float& cg, float& cb,
+                                         float& ca, bool& done, DepthLane& dl, uint32_t& frags) {
+#pragma clang fp contract(fast)
+  constexpr int kWords = PxStage<NT, DEPTH>::kWords;
+  const char* geo = reinterpret_cast<const char*>(s.geo);
+  const char* col = reinterpret_cast<const char*>(s.col);
+  // the words (32 slots each) that hold splats for this pixel
+  uint32_t nz = 0;
+  if (t.inside && (EXACT || !done)) {
+#pragma unroll
+    for (int wi = 0; wi < kWords; ++wi) nz |= ((s.colm[lx][wi] & s.rowm[ly][wi]) != 0u ? 1u : 0u) << wi;
+  }
+  // each lane walks its own bits in slot order, word after word: the wave
+  // steps as often as its busiest lane has splats
+  uint32_t cur = 0, base = 0, steps = 0;
+  for (;;) {
+    if (cur == 0u && nz != 0u) {
+      const uint32_t wi = uint32_t(__builtin_ctz(nz));
+      nz &= nz - 1u;
+      cur = s.colm[lx][wi] & s.rowm[ly][wi];
+      base = wi * 32u;
+    }
+    if (!__any(cur != 0u)) break;
+    {
+      const bool act = cur != 0u;
+      const uint32_t b = act ? uint32_t(__builtin_ctz(cur)) : 0u;
+      cur &= cur - 1u;
+      const uint32_t q = (base + b) * 16u;  // byte offset of the slot
+      const float4 g = *reinterpret_cast<const float4*>(geo + q);
+      const float4 c = *reinterpret_cast<const float4*>(col + q);
+      bool kept;
+      float al = fragment_alpha<MODE>(g, c, t, sd2l, ell2l, kept);
+      kept = kept && act;
+      const half2v rg = __builtin_bit_cast(half2v, c.z), bo = __builtin_bit_cast(half2v, c.w);
+      if (MODE == kModeSplat) al = kept ? __builtin_fmaf(al, float(bo.y), 0.0f) : 0.0f;  // times the opacity
+      else al = kept ? al : 0.0f;
+      if (DEPTH) {
+        const float zz = s.z[q >> 4];
+        if (EXACT) {
+          // against the attachment as updated by earlier draws
+          const bool pass = kept && depth_pass(dl.cmp, zz, dl.ref);
+          al = pass ? al : 0.0f;
+          if (pass && dl.found) dl.ref = zz;  // found: depth_write
+          kept = pass;
+        } else {
+          const bool pass = depth_pass(dl.cmp, zz, dl.ref);
+          al = pass ? al : 0.0f;
+          if (kept && pass && !dl.found) {
+            dl.first = zz;
+            dl.found = true;
+          }
+        }
+      }
+      if (EXACT) {
+        frags += kept ? 1u : 0u;
+        const float om = 1.0f - al;
+        if (q8) {
+          // q' = round(q om + 255 c al), q in 0..255
+          const float a255 = 255.0f * al;
+          cr = __builtin_rintf(__builtin_fmaf(float(rg.x), a255, cr * om));
+          cg = __builtin_rintf(__builtin_fmaf(float(rg.y), a255, cg * om));
+          cb = __builtin_rintf(__builtin_fmaf(float(bo.x), a255, cb * om));
+          ca = __builtin_rintf(__builtin_fmaf(ca, om, a255));
+        } else {
+          cr = __builtin_fmaf(float(rg.x), al, cr * om);
+          cg = __builtin_fmaf(float(rg.y), al, cg * om);
+          cb = __builtin_fmaf(float(bo.x), al, cb * om);
+          ca = __builtin_fmaf(ca, om, al);
+        }
+      } else {
+        const float wgt = T * al;
+        cr = __builtin_fmaf(wgt, float(rg.x), cr);
+        cg = __builtin_fmaf(wgt, float(rg.y), cg);
+        cb = __builtin_fmaf(wgt, float(bo.x), cb);
+        T = T - wgt;
+        if ((++steps & 15u) == 0u) {
+          // the wave stops when every lane is done
+          if (__all(done || T < kMinTransmittance)) {
+            done = true;
+            return;
+          }
+        }
+      }
+    }
+  }
+  if (!EXACT) done = done || T < kMinTransmittance;
+}
+
+template <bool EXACT, bool DEPTH, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_composite_px(CompositeArgs a) {
   if (*a.pair_count > a.capacity) return;  // truncated lists: the frame is drawn again
-  __shared__ Stage s;
-  const uint32_t tile = blockIdx.x;
-  const Tiling t = setup_tile(a.grid, tile);
-  const uint32_t start = a.ranges[2 * tile], end = a.ranges[2 * tile + 1];
+  __shared__ PxStage<NT, DEPTH> s;
+  __shared__ uint32_t s_item;
   const FrameConsts& fc = *a.fc;
   const int mode = fc.display_mode;
   const float sd2l = fc.max_std_dev * fc.max_std_dev * kLog2e;
   const float e = fc.max_std_dev - 0.1f;
   const float ell2l = e * e * kLog2e;
   const bool q8 = a.format == kFormatRgba8Unorm;
-  const uint32_t nt = blockDim.x, tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
-  // the lane's 4x4 quadrant list (row-major over the tile's quadrants)
-  const uint32_t ncols = 2u * t.blocks_x, bx = w % t.blocks_x, by = w / t.blocks_x;
-  const uint32_t myq = (2u * by + (lane >> 5)) * ncols + 2u * bx + ((lane >> 2) & 1u);
+  const uint32_t nt = blockDim.x, tid = threadIdx.x;
+  // read once: loads from the ticket's cache line while every workgroup
+  // takes tickets from it wait behind the atomics
+  const uint32_t items = a.plan[0], heavy = a.plan[1];
 
-  uint32_t frags = 0;
-  float dr = 0.0f, dg = 0.0f, db = 0.0f, da = 1.0f;
-  const bool dtest = a.depth != nullptr;
-  float zcur = (dtest && t.inside) ? a.depth[size_t(t.py) * a.grid.width + t.px] : 0.0f;
-  if (a.load && t.inside) {
-    const size_t o = size_t(t.py) * a.grid.width + t.px;
-    if (q8) {
-      const uint32_t d = reinterpret_cast<const uint32_t*>(a.out)[o];
-      dr = float(d & 0xFFu) / 255.0f;
-      dg = float((d >> 8) & 0xFFu) / 255.0f;
-      db = float((d >> 16) & 0xFFu) / 255.0f;
-      da = float(d >> 24) / 255.0f;
-    } else {
-      const float4 d = reinterpret_cast<const float4*>(a.out)[o];
-      dr = d.x;
-      dg = d.y;
-      db = d.z;
-      da = d.w;
+  for (;;) {
+    const uint32_t item = take_ticket(a.plan + kPlanTicketWord, &s_item);
+    if (item >= items) break;
+    const uint32_t tile = EXACT ? item : a.item_tile[item];
+    const uint32_t seg = EXACT ? 0u : a.item_seg[item];
+    const Tiling t = setup_tile(a.grid, tile);
+    const uint32_t lx = t.px - uint32_t(t.ox), ly = t.py - uint32_t(t.oy);
+    const uint32_t tstart = a.ranges[2 * tile], tend = a.ranges[2 * tile + 1];
+    uint32_t front = tend, back = tstart;
+    if (!EXACT) {
+      // segment counted from the front (nearest end) of the tile's list
+      front = tend - seg * a.seg_len;
+      back = (item < heavy && front - tstart > a.seg_len) ? front - a.seg_len : tstart;
     }
-  }
-  for (uint32_t b0 = start; b0 < end; b0 += nt) {
-    const uint32_t cnt = (end - b0) < nt ? (end - b0) : nt;
-    __syncthreads();
-    const bool valid = tid < cnt;
-    // farthest first
-    stage_batch_quads(s, t, a.splats, dtest ? a.splat_depth : nullptr, valid, valid ? a.pair_vals[b0 + tid] : 0u,
-                      q8, mode, false);
-    if (!t.inside) continue;
-    uint32_t len = 0;
-    for (uint32_t p = 0; p < t.waves; ++p) len += uint32_t(__popcll(s.qbal[p][myq]));
-    for (uint32_t j = 0; j < len; ++j) {
-      const uint32_t q = uint32_t(s.list[myq][j]) >> 4;  // lists hold byte offsets
-      float4 col;
-      const float al = fragment(s, q, t, mode, sd2l, ell2l, col);
-      if (al < 0.0f) continue;
-      if (dtest) {
-        // depth test against the attachment as updated by earlier draws
-        const float z = s.z[q];
-        if (!depth_pass(a.depth_cmp, z, zcur)) continue;
-        if (a.depth_write) zcur = z;
-      }
-      ++frags;
-      const float om = 1.0f - al;
-      const float2 gb = s.gb[q];
-      dr = col.w * al + dr * om;
-      dg = gb.x * al + dg * om;
-      db = gb.y * al + db * om;
-      da = al + da * om;
+    const size_t o = size_t(t.py) * a.grid.width + t.px;
+    DepthLane dl{a.depth_cmp, 0.0f, 0.0f, false};
+    if (DEPTH && t.inside) dl.ref = a.depth[o];
+    if (EXACT) dl.found = a.depth_write != 0;  // exact: `found` marks depth writes
+
+    // fast: colour and transmittance; exact: the destination (q8: 0..255)
+    float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f, ca = q8 ? 255.0f : 1.0f;
+    if (EXACT && a.load && t.inside) {
       if (q8) {
-        dr = float(unorm8(dr)) / 255.0f;
-        dg = float(unorm8(dg)) / 255.0f;
-        db = float(unorm8(db)) / 255.0f;
-        da = float(unorm8(da)) / 255.0f;
+        const uint32_t d = reinterpret_cast<const uint32_t*>(a.out)[o];
+        cr = float(d & 0xFFu);
+        cg = float((d >> 8) & 0xFFu);
+        cb = float((d >> 16) & 0xFFu);
+        ca = float(d >> 24);
+      } else {
+        const float4 d = reinterpret_cast<const float4*>(a.out)[o];
+        cr = d.x;
+        cg = d.y;
+        cb = d.z;
+        ca = d.w;
       }
     }
-  }
-  if (t.inside) {
-    const size_t o = size_t(t.py) * a.grid.width + t.px;
-    if (q8)
-      reinterpret_cast<uint32_t*>(a.out)[o] = unorm8(dr) | (unorm8(dg) << 8) | (unorm8(db) << 16) | (unorm8(da) << 24);
-    else
-      reinterpret_cast<float4*>(a.out)[o] = make_float4(dr, dg, db, da);
-    if (dtest && a.depth_write) a.depth[o] = zcur;
-    if (a.frag_count && frags) atomicAdd(a.frag_count, frags);
+    bool done = !t.inside;
+    uint32_t frags = 0;
+    for (uint32_t p0 = 0; p0 < front - back;) {
+      const uint32_t cnt = (front - back - p0) < nt ? (front - back - p0) : nt;
+      __syncthreads();
+      const bool valid = tid < cnt;
+      // exact: farthest first (the list's order); fast: nearest first
+      const uint32_t pidx = EXACT ? back + p0 + tid : front - 1 - p0 - tid;
+      stage_batch_px(s, t, a.grid, a.splats, DEPTH ? a.splat_depth : nullptr, valid, valid ? a.pair_vals[pidx] : 0u,
+                     q8, mode);
+      if (mode == kModeSplat)
+        blend_px<kModeSplat, EXACT, DEPTH, NT>(s, t, lx, ly, q8, sd2l, ell2l, T, cr, cg, cb, ca, done, dl, frags);
+      else if (mode == kModeEllipse)
+        blend_px<kModeEllipse, EXACT, DEPTH, NT>(s, t, lx, ly, q8, sd2l, ell2l, T, cr, cg, cb, ca, done, dl, frags);
+      else
+        blend_px<kModePoint, EXACT, DEPTH, NT>(s, t, lx, ly, q8, sd2l, ell2l, T, cr, cg, cb, ca, done, dl, frags);
+      p0 += cnt;
+      if (!EXACT && __syncthreads_and(done ? 1 : 0)) break;
+    }
+    if (t.inside) {
+      if (EXACT) {
+        if (q8)
+          reinterpret_cast<uint32_t*>(a.out)[o] =
+              uint32_t(cr) | (uint32_t(cg) << 8) | (uint32_t(cb) << 16) | (uint32_t(ca) << 24);
+        else
+          reinterpret_cast<float4*>(a.out)[o] = make_float4(cr, cg, cb, ca);
+        if (DEPTH && a.depth_write) a.depth[o] = dl.ref;
+        if (a.frag_count && frags) atomicAdd(a.frag_count, frags);
+      } else if (item < heavy) {  // a segment of a split tile
+        a.partial[size_t(item) * nt + tid] = make_float4(cr, cg, cb, T);
+      } else {
+        store_pixel(a, t, cr, cg, cb, T);
+        if (DEPTH && a.depth_write && dl.found) a.depth[o] = dl.first;
+      }
+    }
   }
 }
 
@@ -724,39 +623,41 @@ uint32_t composite_max_heavy_items(uint32_t capacity, uint32_t seg_len) {
   return 2u * (capacity / seg_len) + 2u;
 }
 
+template <bool EXACT>
+static void launch_px(hipStream_t st, const CompositeArgs& a, uint32_t grid, uint32_t threads) {
+  if (threads == kWave) {
+    if (a.depth)
+      hipLaunchKernelGGL((k_composite_px<EXACT, true, kWave>), dim3(grid), dim3(threads), 0, st, a);
+    else
+      hipLaunchKernelGGL((k_composite_px<EXACT, false, kWave>), dim3(grid), dim3(threads), 0, st, a);
+  } else {
+    if (a.depth)
+      hipLaunchKernelGGL((k_composite_px<EXACT, true, kCompMaxThreads>), dim3(grid), dim3(threads), 0, st, a);
+    else
+      hipLaunchKernelGGL((k_composite_px<EXACT, false, kCompMaxThreads>), dim3(grid), dim3(threads), 0, st, a);
+  }
+}
+
 void launch_composite(hipStream_t st, const CompositeArgs& a, int num_cus, hipEvent_t main_begin,
                       hipEvent_t main_end) {
   const uint32_t tiles = a.grid.tiles_x * a.grid.tiles_y;
   const uint32_t threads = a.grid.tile_w * a.grid.tile_h;
   if (tiles == 0) return;
-  if (a.blend == kBlendFast) {
-    const uint32_t nb = composite_plan_blocks(tiles);
-    hipLaunchKernelGGL(k_plan_count, dim3(nb), dim3(kPlanTiles), 0, st, a);
-    hipLaunchKernelGGL(k_plan, dim3(nb), dim3(kPlanTiles), 0, st, a);
-    // persistent workgroups take items by ticket; 8 per CU of 256 lanes
-    uint32_t grid = uint32_t(num_cus) * 8u * (kCompMaxThreads / threads);
-    const uint32_t cap = composite_max_items(tiles, a.capacity, a.seg_len);
-    if (grid > cap) grid = cap;
-    if (main_begin) (void)hipEventRecord(main_begin, st);
-    // 8x8 tiles (one wave per workgroup) use a workgroup of 64 lanes
-    if (threads == kWave) {
-      if (a.depth)
-        hipLaunchKernelGGL((k_composite_fast<true, kWave>), dim3(grid), dim3(threads), 0, st, a);
-      else
-        hipLaunchKernelGGL((k_composite_fast<false, kWave>), dim3(grid), dim3(threads), 0, st, a);
-    } else {
-      if (a.depth)
-        hipLaunchKernelGGL((k_composite_fast<true, kCompMaxThreads>), dim3(grid), dim3(threads), 0, st, a);
-      else
-        hipLaunchKernelGGL((k_composite_fast<false, kCompMaxThreads>), dim3(grid), dim3(threads), 0, st, a);
-    }
-    if (main_end) (void)hipEventRecord(main_end, st);
-    hipLaunchKernelGGL(k_merge, dim3(tiles), dim3(threads), 0, st, a);
-  } else {
-    if (main_begin) (void)hipEventRecord(main_begin, st);
-    hipLaunchKernelGGL(k_composite_exact, dim3(tiles), dim3(threads), 0, st, a);
-    if (main_end) (void)hipEventRecord(main_end, st);
-  }
+  // work list (heavy tiles first), then persistent workgroups taking items
+  // by ticket, 8 per CU of 256 lanes
+  const uint32_t nb = composite_plan_blocks(tiles);
+  hipLaunchKernelGGL(k_plan_count, dim3(nb), dim3(kPlanTiles), 0, st, a);
+  hipLaunchKernelGGL(k_plan, dim3(nb), dim3(kPlanTiles), 0, st, a);
+  uint32_t grid = uint32_t(num_cus) * 8u * (kCompMaxThreads / threads);
+  const uint32_t cap = composite_max_items(tiles, a.capacity, a.seg_len);
+  if (grid > cap) grid = cap;
+  if (main_begin) (void)hipEventRecord(main_begin, st);
+  if (a.blend == kBlendFast)
+    launch_px<false>(st, a, grid, threads);
+  else
+    launch_px<true>(st, a, grid, threads);
+  if (main_end) (void)hipEventRecord(main_end, st);
+  if (a.blend == kBlendFast) hipLaunchKernelGGL(k_merge, dim3(tiles), dim3(threads), 0, st, a);
 }
 
 }  // namespace gs3d

@@ -377,12 +377,51 @@ __device__ __forceinline__ void splat_colour(const FrameConsts& fc, const ShWord
   for (int ch = 0; ch < 3; ++ch) rgb[ch] = fmaxf(rgb[ch], 0.0f);
 }
 
-// The rest of the record: centre, axes, bounds and tile rectangle, with the
-// colour from splat_colour.
+// A splat's minor eigenvalue lambda_min = mid - radius (utils.wesl:54-79)
+// cancels when the splat is thin: below kThinEigenRatio * lambda_max its
+// rounding decides the minor axis (and whether the splat is drawn at all), so
+// such splats take their axes from the exact operation order of the oracle
+// and the reference's shader (math.hpp's cov2d_axes on the cov3d rebuilt in
+// that order) instead of the contracted evaluation.
+constexpr float kThinEigenRatio = 1e-3f;
+constexpr float kThinRho = 31.6227766f;  // 1 / sqrt(kThinEigenRatio)
+
+// The record's axes (u, rho) and half extents of a thin splat, in that exact
+// order (render.wesl:75-130 as restated by the oracle: u = 2 (M.x, -M.y) /
+// |M|^2, v = 2 (m.x, -m.y) / |m|^2, of which the record keeps
+// rho = |M| / |m|); ok is false when the splat is not drawn.
 template <int POD>
-__device__ __forceinline__ void splat_geometry(const FrameConsts& fc, const float world[4], const float c[6],
-                                               uint32_t color_bits, const float rgb[3], Splat2D* out,
-                                               float* depth_out, uint2* rect_out, const TileGrid* rect_grid) {
+__device__ __forceinline__ void thin_splat_axes(const FrameConsts& fc, const float pos[3], const CovWords& cw,
+                                             float& ux, float& uy, float& rho, float& hx, float& hy, bool& ok) {
+  float c[6];
+  cov3d_of<POD, false>(cw, c);
+  const float sd = fc.max_std_dev;
+  float ax[4];
+  cov2d_axes(fc, pos, c, sd * fc.gsize, ax);
+  const float mm = ax[0] * ax[0] + ax[1] * ax[1];
+  const float nn = ax[2] * ax[2] + ax[3] * ax[3];
+  ok = (mm > 0.0f) && (nn > 0.0f);
+  ux = uy = rho = hx = hy = 0.0f;
+  if (!ok) return;
+  ux = (2.0f * ax[0]) / mm;
+  uy = -((2.0f * ax[1]) / mm);
+  rho = sqrtf(mm / nn);
+  hx = (0.5f * sd) * sqrtf(ax[0] * ax[0] + ax[2] * ax[2]);
+  hy = (0.5f * sd) * sqrtf(ax[1] * ax[1] + ax[3] * ax[3]);
+}
+
+// The rest of the record: centre, axes, bounds and tile rectangle, with the
+// colour from splat_colour. pos and cw (the model-space position and the
+// covariance words) serve the exact axes of thin splats: with kInlineThin
+// they are computed here; otherwise the record keeps the contracted axes and
+// the return value (true) asks the caller to have thin_splat_record redo
+// them (the projection kernel queues such splats for a kernel of their own,
+// so that the exact evaluation adds nothing to its registers).
+template <int POD, bool kInlineThin>
+__device__ __forceinline__ bool splat_geometry(const FrameConsts& fc, const float world[4], const float c[6],
+                                               const float pos[3], const CovWords& cw, uint32_t color_bits,
+                                               const float rgb[3], Splat2D* out, float* depth_out, uint2* rect_out,
+                                               const TileGrid* rect_grid) {
 #pragma clang fp contract(fast)
   // the view-space centre (rows 0-2) and the clip-space centre
   // (proj * view) * world
@@ -403,6 +442,7 @@ __device__ __forceinline__ void splat_geometry(const FrameConsts& fc, const floa
     *depth_out = c4[2] / c4[3];
   }
   Splat2D s;
+  bool thin = false;
   s.cx = (proj[0] * iw + 1.0f) * 0.5f * fc.size[0];
   s.cy = (1.0f - proj[1] * iw) * 0.5f * fc.size[1];
   s.ux = s.uy = s.rho = 0.0f;
@@ -455,7 +495,12 @@ __device__ __forceinline__ void splat_geometry(const FrameConsts& fc, const floa
     const float radius = __builtin_amdgcn_sqrtf(hd * hd + cb * cb);
     const float lmaj = mid + radius, lmin = mid - radius;
     bool ok = zclip_ok && lmin >= 0.0f;
-    if (ok) {
+    if (kInlineThin) thin = zclip_ok && !(lmin >= kThinEigenRatio * lmaj);
+    if (thin && kInlineThin) {
+      // thin (or degenerate): the exact order decides
+      thin_splat_axes<POD>(fc, pos, cw, s.ux, s.uy, s.rho, hx, hy, ok);
+      thin = false;
+    } else if (ok) {
       const float dx0 = cb, dy0 = lmaj - ca;
       float dx = 0.0f, dy = 1.0f;
       if (!(dx0 == 0.0f && dy0 == 0.0f)) {
@@ -478,11 +523,31 @@ __device__ __forceinline__ void splat_geometry(const FrameConsts& fc, const floa
         hy = 0.5f * sd * __builtin_amdgcn_sqrtf(maj * maj * dy * dy + mnr * mnr * dx * dx);
       }
     }
+    // queued for the exact axes: lambda_min below kThinEigenRatio lambda_max,
+    // i.e. an axis ratio rho = sqrt(lambda_max / lambda_min) above
+    // 1 / sqrt(kThinEigenRatio), or no axes at all
+    if (!kInlineThin) thin = zclip_ok && !(ok && s.rho <= kThinRho);
     drawable = ok;
   }
   s.rg = pack_half2(rgb[0], rgb[1]);
   s.ba = pack_half2(rgb[2], alpha);
   s.hxy = drawable ? pack_half2_up(hx, hy) : 0u;
+  *out = s;
+  if (rect_out) *rect_out = tile_rect(s, fc.max_std_dev * fc.max_std_dev, *rect_grid);
+  return thin;
+}
+
+// Redoes the axes, half extents and tile rectangle of a thin splat's record
+// (splat_geometry returned true for it) in the exact order; the centre and
+// colour stay.
+template <int POD>
+__device__ __forceinline__ void thin_splat_record(const FrameConsts& fc, const float pos[3], const CovWords& cw,
+                                                  Splat2D* out, uint2* rect_out, const TileGrid* rect_grid) {
+  Splat2D s = *out;
+  float hx, hy;
+  bool ok;
+  thin_splat_axes<POD>(fc, pos, cw, s.ux, s.uy, s.rho, hx, hy, ok);
+  s.hxy = ok ? pack_half2_up(hx, hy) : 0u;
   *out = s;
   if (rect_out) *rect_out = tile_rect(s, fc.max_std_dev * fc.max_std_dev, *rect_grid);
 }
@@ -491,13 +556,14 @@ __device__ __forceinline__ void splat_geometry(const FrameConsts& fc, const floa
 template <int POD>
 __device__ __forceinline__ void compute_splat(const FrameConsts& fc, const GaussianStreams& g, uint32_t i,
                                               uint32_t color_bits, const float world[4], const float c[6],
-                                              Splat2D* out, float* depth_out, uint2* rect_out = nullptr,
+                                              const float pos[3], const CovWords& cw, Splat2D* out,
+                                              float* depth_out, uint2* rect_out = nullptr,
                                               const TileGrid* rect_grid = nullptr) {
   ShWords<POD> sw;
   sh_load<POD>(g, i, fc.sh_deg, sw);
   float rgb[3];
   splat_colour<POD>(fc, sw, world, color_bits, rgb);
-  splat_geometry<POD>(fc, world, c, color_bits, rgb, out, depth_out, rect_out, rect_grid);
+  splat_geometry<POD, true>(fc, world, c, pos, cw, color_bits, rgb, out, depth_out, rect_out, rect_grid);
 }
 
 }  // namespace gs3d

@@ -42,7 +42,16 @@ struct PreprocessArgs {
   // sort_max)), zeroed by k_frame_pre (null: the sort counts for itself)
   uint32_t* sort_hist;
   uint32_t sort_max;
+  // thin splats whose axes k_thin_records redoes in the exact order: their
+  // count (word kThinCountWord of the ticket buffer, zeroed by k_frame_pre)
+  // and Gaussian indices (room for every Gaussian)
+  uint32_t* thin_count;
+  uint32_t* thin_idx;
 };
+
+// Word of the projection's ticket buffer (256 bytes) that counts the frame's
+// thin splats: a cache line of its own, away from the ticket.
+constexpr uint32_t kThinCountWord = 32;
 
 uint32_t preprocess_num_tiles(uint32_t n);
 void launch_frame_pre(hipStream_t st, const CameraPod* cam, const ModelTransformPod* mt,

@@ -56,6 +56,7 @@ __global__ __launch_bounds__(256) void k_frame_pre(const CameraPod* __restrict__
     dispatch_args[1] = 1u;
     dispatch_args[2] = 1u;
     *ticket = 0u;
+    ticket[kThinCountWord] = 0u;  // the thin-splat queue of this frame
   }
   for (uint32_t t = threadIdx.x; t < num_tiles; t += blockDim.x) status[t] = 0u;
 }
@@ -233,9 +234,20 @@ __global__ __launch_bounds__(kPreThreads) __attribute__((amdgpu_waves_per_eu(GS3
       cw = load_cov_words<POD>(a.g, i + kPreThreads);
       sh_load<POD>(a.g, i + kPreThreads, fc.sh_deg, sw);
     }
-    if (vis[k])
-      splat_geometry<POD>(fc, world, c, cbits, rgb, a.splats + i, a.splat_depth ? a.splat_depth + i : nullptr,
-                          a.rects ? a.rects + i : nullptr, &a.rect_grid);
+    bool thin = false;
+    if (vis[k]) {
+      // (pos and cw serve only an inline exact evaluation, which this kernel
+      // leaves to k_thin_records: cw may already hold the next item's words)
+      const float pos[3] = {pcs[k].x, pcs[k].y, pcs[k].z};
+      thin = splat_geometry<POD, false>(fc, world, c, pos, cw, cbits, rgb, a.splats + i,
+                                        a.splat_depth ? a.splat_depth + i : nullptr, a.rects ? a.rects + i : nullptr,
+                                        &a.rect_grid);
+    }
+    if (thin) {
+      // queued for k_thin_records (rare: one atomic per thin splat)
+      const uint32_t q = atomicAdd(a.thin_count, 1u);
+      a.thin_idx[q] = i;
+    }
     if (!GS3D_PRE_PIPELINE && k + 1 < kPreItems && vis[k + 1]) {
       cw = load_cov_words<POD>(a.g, i + kPreThreads);
       sh_load<POD>(a.g, i + kPreThreads, fc.sh_deg, sw);
@@ -314,9 +326,29 @@ __global__ __launch_bounds__(256) void k_vertex(const FrameConsts* __restrict__ 
     const float p4[4] = {pc.x, pc.y, pc.z, 1.0f};
     float world[4], c[6];
     m4v(fc.model, p4, world);  // uncontracted: the fragment depth derives from it
-    cov3d_of<POD, true>(load_cov_words<POD>(g, i), c);
-    compute_splat<POD>(fc, g, i, __float_as_uint(pc.w), world, c, splats + i,
+    const CovWords cw = load_cov_words<POD>(g, i);
+    cov3d_of<POD, true>(cw, c);
+    const float pos[3] = {pc.x, pc.y, pc.z};  // thin splats' exact axes are computed inline here
+    compute_splat<POD>(fc, g, i, __float_as_uint(pc.w), world, c, pos, cw, splats + i,
                        splat_depth ? splat_depth + i : nullptr);
+  }
+}
+
+// The exact axes of the thin splats k_preprocess queued (their records were
+// written with the contracted ones): a grid-stride loop over the queue.
+template <int POD>
+__global__ __launch_bounds__(256) void k_thin_records(const FrameConsts* __restrict__ fcp, GaussianStreams g,
+                                                      const uint32_t* __restrict__ thin_count,
+                                                      const uint32_t* __restrict__ thin_idx, Splat2D* splats,
+                                                      uint2* rects, TileGrid rect_grid) {
+  const FrameConsts& fc = *fcp;
+  const uint32_t n = *thin_count;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    const uint32_t i = thin_idx[j];
+    if (i >= g.n) continue;
+    const float4 pc = g.pc[i];
+    const float pos[3] = {pc.x, pc.y, pc.z};
+    thin_splat_record<POD>(fc, pos, load_cov_words<POD>(g, i), splats + i, rects ? rects + i : nullptr, &rect_grid);
   }
 }
 
@@ -341,6 +373,13 @@ void launch_preprocess(hipStream_t st, int pod, const PreprocessArgs& a) {
     hipLaunchKernelGGL(k_preprocess<kPodSingle>, dim3(a.num_tiles), dim3(kPreThreads), 0, st, a);
   else
     hipLaunchKernelGGL(k_preprocess<kPodHalf>, dim3(a.num_tiles), dim3(kPreThreads), 0, st, a);
+  // thin splats are few: 64 workgroups loop over the queue
+  if (pod == kPodSingle)
+    hipLaunchKernelGGL(k_thin_records<kPodSingle>, dim3(64), dim3(256), 0, st, a.fc, a.g, a.thin_count, a.thin_idx,
+                       a.splats, a.rects, a.rect_grid);
+  else
+    hipLaunchKernelGGL(k_thin_records<kPodHalf>, dim3(64), dim3(256), 0, st, a.fc, a.g, a.thin_count, a.thin_idx,
+                       a.splats, a.rects, a.rect_grid);
 }
 
 void launch_vertex(hipStream_t st, int pod, const FrameConsts* fc, const GaussianStreams& g,

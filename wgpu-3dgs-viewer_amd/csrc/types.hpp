@@ -47,8 +47,9 @@ enum CompareFn : int32_t {
 enum BlendMode : int32_t {
   // front-to-back in f32 with early termination, one quantisation at the end
   kBlendFast = 0,
-  // back-to-front, src clamped, destination stored as unorm8 after every
-  // splat: the fixed-function ALPHA_BLENDING into an Rgba8Unorm target
+  // (the default) back-to-front, src clamped, destination stored as unorm8
+  // after every splat: the fixed-function ALPHA_BLENDING into an Rgba8Unorm
+  // target
   kBlendExactUnorm8 = 1,
 };
 

@@ -529,7 +529,8 @@ Preprocessor::Preprocessor(Device& dev, const CameraBuffer& camera, const ModelT
       splats_(splats),
       consts_(dev, sizeof(FrameConsts)),
       status_(dev, size_t(std::max(preprocess_num_tiles(gaussians.len()), 1u)) * 4),
-      ticket_(dev, 64),
+      ticket_(dev, 256),
+      thin_idx_(dev, size_t(std::max(gaussians.len(), 1u)) * 4),
       max_tiles_(preprocess_num_tiles(gaussians.len())) {
   // src/preprocessor.rs:239-246
   if (gaussians.size_bytes() > dev.buffer_limit())
@@ -567,6 +568,8 @@ void Preprocessor::project(uint32_t gaussian_count) {
   a.sort_hist = sort_hist_;
   a.sort_max = sort_max_;
   a.invert_selection = invert_selection_ ? 1u : 0u;
+  a.thin_count = ticket_.as<uint32_t>() + kThinCountWord;
+  a.thin_idx = thin_idx_.as<uint32_t>();
   launch_preprocess(st_, gaussians_.pod(), a);
   hip_check(hipGetLastError(), "launch preprocess");
 }
@@ -691,6 +694,7 @@ bool Renderer::grow_if_overflowed() {
   // the frames have completed: k_ranges left the largest pair total since
   // the last check in mapped memory (and in word 1 of pair_count_)
   const uint32_t total = last_pair_total();
+  checked_pair_total_ = total;
   *static_cast<volatile uint32_t*>(pair_total_host_) = 0u;
   const uint32_t zero = 0u;
   hip_check(hipMemcpy(pair_count_.as<uint32_t>() + 1, &zero, 4, hipMemcpyHostToDevice), "reset pair maximum");

@@ -389,7 +389,7 @@ class Preprocessor {
   IndirectIndicesBuffer& indices_;
   GaussiansDepthBuffer& depth_;
   SplatsBuffer& splats_;
-  DeviceBuffer consts_, status_, ticket_;
+  DeviceBuffer consts_, status_, ticket_, thin_idx_;
   uint32_t max_tiles_;
   const uint32_t* selection_ = nullptr;
   float* splat_depth_ = nullptr;
@@ -485,6 +485,8 @@ class Renderer {
   void settle();
   // (tile, splat) pairs of the last completed frame, and the list capacity
   uint32_t last_pair_total() const { return *static_cast<volatile uint32_t*>(pair_total_host_); }
+  // the largest pair total of the frames drawn before the last poll
+  uint32_t checked_pair_total() const { return checked_pair_total_; }
   uint32_t capacity() const { return capacity_; }
   // Replaces the lists with ones of exactly `pairs` pairs (they still grow
   // when a frame needs more); waits for the device.
@@ -505,7 +507,7 @@ class Renderer {
   SplatsBuffer& splats_;
   const FrameConsts* shared_consts_;
   DeviceBuffer own_consts_;
-  BlendMode blend_ = kBlendFast;
+  BlendMode blend_ = kBlendExactUnorm8;  // the reference's blend
   uint32_t tile_w_ = 16, tile_h_ = 16;
   uint32_t capacity_ = 0;
   DeviceBuffer pk_[2], pv_[2];
@@ -516,6 +518,7 @@ class Renderer {
   uint32_t seg_len_ = GS3D_COMPOSITE_SEG_LEN;
   DeviceBuffer item_tile_, item_seg_, seg_base_, plan_, plan_counts_, partial_;
   uint32_t* pair_total_host_ = nullptr;  // pinned, mapped
+  uint32_t checked_pair_total_ = 0;
   uint32_t* pair_total_dev_ = nullptr;   // its device alias
   DeviceBuffer frag_count_;
   bool counting_ = false;
