@@ -349,6 +349,9 @@ constexpr int kPxMaxCols = 32, kPxMaxRows = 16;
 template <int NT, bool Z>
 struct PxStage {
   static constexpr int kWords = NT / 32;
+  // per slot: geo and col, 16 bytes each, in two arrays (a 16-byte stride
+  // spreads the lanes' reads of random slots over more LDS banks than one
+  // 32-byte record per slot)
   float4 geo[NT];
   float4 col[NT];
   float z[Z ? NT : 1];
@@ -412,10 +415,13 @@ __device__ __forceinline__ void stage_batch_px(PxStage<NT, Z>& s, const Tiling& 
 // increasing order. EXACT: back to front (q8: the 8-bit per-draw rounding);
 // otherwise front to back with transmittance T, a lane stopping once below
 // the threshold (checked every 16 steps).
+// track_a: the destination's alpha may be below 1 (a loaded target); after a
+// clear it stays 1 under the over operator. count: count the kept fragments.
 template <int MODE, bool EXACT, bool DEPTH, int NT>
 __device__ __forceinline__ void blend_px(const PxStage<NT, DEPTH>& s, const Tiling& t, uint32_t lx, uint32_t ly,
-                                         bool q8, float sd2l, float ell2l, float& T, float& cr, float& cg, float& cb,
-                                         float& ca, bool& done, DepthLane& dl, uint32_t& frags) {
+                                         bool q8, bool track_a, bool count, float sd2l, float ell2l, float& T,
+                                         float& cr, float& cg, float& cb, float& ca, bool& done, DepthLane& dl,
+                                         uint32_t& frags) {
 #pragma clang fp contract(fast)
   constexpr int kWords = PxStage<NT, DEPTH>::kWords;
   const char* geo = reinterpret_cast<const char*>(s.geo);
@@ -468,7 +474,7 @@ __device__ __forceinline__ void blend_px(const PxStage<NT, DEPTH>& s, const Tili
         }
       }
       if (EXACT) {
-        frags += kept ? 1u : 0u;
+        if (count) frags += kept ? 1u : 0u;
         const float om = 1.0f - al;
         if (q8) {
           // q' = round(q om + 255 c al), q in 0..255
@@ -476,12 +482,12 @@ __device__ __forceinline__ void blend_px(const PxStage<NT, DEPTH>& s, const Tili
           cr = __builtin_rintf(__builtin_fmaf(float(rg.x), a255, cr * om));
           cg = __builtin_rintf(__builtin_fmaf(float(rg.y), a255, cg * om));
           cb = __builtin_rintf(__builtin_fmaf(float(bo.x), a255, cb * om));
-          ca = __builtin_rintf(__builtin_fmaf(ca, om, a255));
+          if (track_a) ca = __builtin_rintf(__builtin_fmaf(ca, om, a255));
         } else {
           cr = __builtin_fmaf(float(rg.x), al, cr * om);
           cg = __builtin_fmaf(float(rg.y), al, cg * om);
           cb = __builtin_fmaf(float(bo.x), al, cb * om);
-          ca = __builtin_fmaf(ca, om, al);
+          if (track_a) ca = __builtin_fmaf(ca, om, al);
         }
       } else {
         const float wgt = T * al;
@@ -556,6 +562,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
     }
     bool done = !t.inside;
     uint32_t frags = 0;
+    // after a clear the destination's alpha stays 1 (1 a + 1 (1 - a))
+    const bool track_a = a.load != 0;
+    const bool count = a.frag_count != nullptr;
     for (uint32_t p0 = 0; p0 < front - back;) {
       const uint32_t cnt = (front - back - p0) < nt ? (front - back - p0) : nt;
       __syncthreads();
@@ -565,11 +574,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
       stage_batch_px(s, t, a.grid, a.splats, DEPTH ? a.splat_depth : nullptr, valid, valid ? a.pair_vals[pidx] : 0u,
                      q8, mode);
       if (mode == kModeSplat)
-        blend_px<kModeSplat, EXACT, DEPTH, NT>(s, t, lx, ly, q8, sd2l, ell2l, T, cr, cg, cb, ca, done, dl, frags);
+        blend_px<kModeSplat, EXACT, DEPTH, NT>(s, t, lx, ly, q8, track_a, count, sd2l, ell2l, T, cr, cg, cb, ca, done, dl,
+                                                frags);
       else if (mode == kModeEllipse)
-        blend_px<kModeEllipse, EXACT, DEPTH, NT>(s, t, lx, ly, q8, sd2l, ell2l, T, cr, cg, cb, ca, done, dl, frags);
+        blend_px<kModeEllipse, EXACT, DEPTH, NT>(s, t, lx, ly, q8, track_a, count, sd2l, ell2l, T, cr, cg, cb, ca, done, dl,
+                                                frags);
       else
-        blend_px<kModePoint, EXACT, DEPTH, NT>(s, t, lx, ly, q8, sd2l, ell2l, T, cr, cg, cb, ca, done, dl, frags);
+        blend_px<kModePoint, EXACT, DEPTH, NT>(s, t, lx, ly, q8, track_a, count, sd2l, ell2l, T, cr, cg, cb, ca, done, dl,
+                                                frags);
       p0 += cnt;
       if (!EXACT && __syncthreads_and(done ? 1 : 0)) break;
     }
