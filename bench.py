@@ -7,7 +7,8 @@ Sections (all run by default; --sections picks a subset):
            N(-5, 0.7)) stored in the f16 pod (f16 SH / rotation / scale, u8
            opacity), 8 views of 2560x1440 per batch orbiting the scene at
            radius 30, recorded once as a HIP graph over 4 queues and
-           replayed; tile size autotuned per view. One step = one batch.
+           replayed; compositing tile and sort tile autotuned per view.
+           One step = one batch.
   plain    the same model and resolution through one Viewer, one frame per
            call: frames/s, per-frame latency p50/p99, per-stage GPU times,
            and the per-kernel timing the roofline figures come from (always
@@ -15,7 +16,8 @@ Sections (all run by default; --sections picks a subset):
   mix      4 resident models of 0.5M/1M/3M/8M Gaussians (clustered scene,
            f16), 32 views per batch at resolutions drawn from
            {640x360, 1280x720, 1920x1080, 3840x2160}, cameras on spheres of
-           radius 5-30, graph replay, tile size autotuned per view.
+           radius 5-30, graph replay, compositing tile and sort tile
+           autotuned per view.
   serve    8 resident models totalling ~40M Gaussians (sizes log-uniform
            1M-12M), a Poisson stream of 400 requests/s for --serve-seconds,
            each request (model, camera on a smooth trajectory, resolution of
@@ -171,6 +173,7 @@ def run_batched(gs, dev, model, w, h, views, steps, warmup, queues):
     pod_sets = [gs.pods_array([ring[(views * s + j) % 64] for j in range(views)]) for s in range(8)]
     batch.autotune([ring[j] for j in range(views)])
     tiles = [batch.tile_size(j) for j in range(views)]
+    items = [batch.sort_items(j) for j in range(views)]
     # every camera of the timed loop once, twice over: tile lists that a
     # camera outgrows are enlarged by finish() before the timed region
     for _ in range(2):
@@ -186,7 +189,8 @@ def run_batched(gs, dev, model, w, h, views, steps, warmup, queues):
     batch.finish()
     elapsed = time.perf_counter() - t0
     batch.close()
-    return {"fps": views * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps, "tiles": tiles}
+    return {"fps": views * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps, "tiles": tiles,
+            "sort_items": items}
 
 
 # The mixed-model batched workload: 4 resident models (clustered scene, f16),
@@ -225,17 +229,19 @@ def run_mix(gs, dev, steps, warmup, queues, seed=3):
         batch.render(pod_sets[k % 4])
     batch.finish()
     elapsed = time.perf_counter() - t0
-    tiles = {}
+    tiles, items = {}, {}
     for j in range(len(views)):
         key = "%dx%d" % batch.tile_size(j)
         tiles[key] = tiles.get(key, 0) + 1
+        key = str(batch.sort_items(j))
+        items[key] = items.get(key, 0) + 1
     batch.close()
     for m in models:
         m.close()
     pixels = sum(v[1] * v[2] for v in views)
     return {"fps": len(views) * steps / elapsed, "ms_per_batch": 1e3 * elapsed / steps,
             "views": len(views), "megapixels_per_batch": round(pixels / 1e6, 2),
-            "gaussians": sum(sizes), "tile_choices": tiles}
+            "gaussians": sum(sizes), "tile_choices": tiles, "sort_items_choices": items}
 
 
 def serve_requests(gs, rng, sizes, rate, seconds):
@@ -480,7 +486,8 @@ def main():
     }
     if batched is not None:
         out["batched"] = {"fps": round(batched["fps"], 2), "ms_per_batch": round(batched["ms_per_step"], 4),
-                          "views": args.views, "tiles": ["%dx%d" % t for t in batched["tiles"]]}
+                          "views": args.views, "tiles": ["%dx%d" % t for t in batched["tiles"]],
+                          "sort_items": batched["sort_items"]}
     out.update(out_extra)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")

@@ -235,6 +235,10 @@ gs3d_status gs3d_viewer_set_blend_mode(gs3d_viewer* v, gs3d_blend_mode mode);
 gs3d_status gs3d_viewer_set_pair_capacity(gs3d_viewer* v, uint32_t pairs);
 /* Compositing tile size: 8x8, 16x16 (default) or 32x8 pixels. */
 gs3d_status gs3d_viewer_set_tile_size(gs3d_viewer* v, uint32_t tile_w, uint32_t tile_h);
+/* No counterpart in the reference (a tuning choice of this sort's kernels):
+ * keys per thread of the depth and tile sorts' tiles, 8, 12 or 16 (default).
+ * Every choice gives the same sorted order. */
+gs3d_status gs3d_viewer_set_sort_items(gs3d_viewer* v, uint32_t items);
 /* Record HIP events around each stage of every render (off by default). */
 gs3d_status gs3d_viewer_set_profiling(gs3d_viewer* v, int enabled);
 /* Milliseconds of each gs3d_stage in the last profiled frame; waits. */
@@ -274,6 +278,9 @@ void gs3d_radix_sorter_destroy(gs3d_radix_sorter* s);
 gs3d_status gs3d_radix_sorter_write(gs3d_radix_sorter* s, const uint32_t* keys, const uint32_t* vals, uint32_t n);
 /* RadixSorter::sort: stable ascending sort of the n pairs by key. */
 gs3d_status gs3d_radix_sorter_sort(gs3d_radix_sorter* s);
+/* No counterpart in the reference: keys per thread of the sort's tiles, 8, 12
+ * or 16 (default); every choice gives the same order. */
+gs3d_status gs3d_radix_sorter_set_items(gs3d_radix_sorter* s, uint32_t items);
 gs3d_status gs3d_radix_sorter_read(gs3d_radix_sorter* s, uint32_t* keys, uint32_t* vals, uint32_t n);
 
 /* Most pairs gs3d_radix_sort_noalloc sorts in one call. */
@@ -288,7 +295,7 @@ gs3d_status gs3d_radix_sorter_read(gs3d_radix_sorter* s, uint32_t* keys, uint32_
  * errors). The inputs are left unchanged; keys_in may equal keys_out and
  * vals_in vals_out (an in-place sort). It allocates nothing: its ping-pong
  * pairs, histograms and look-back words live in one fixed device array per GPU
- * (37.8 MB, held until the process ends), so n above
+ * (41.9 MB, held until the process ends), so n above
  * GS3D_RADIX_SORT_NOALLOC_MAX_KEYS is GS3D_ERROR_INVALID_ARGUMENT. Every call
  * on one GPU shares that array: calls that overlap on one GPU (from several
  * threads, or on streams the caller orders differently) race. */
@@ -359,7 +366,9 @@ gs3d_status gs3d_batch_new(gs3d_device* dev, gs3d_model* const* models, uint32_t
 void gs3d_batch_destroy(gs3d_batch* b);
 /* One view slot (viewer + RGBA8 target) per view. */
 gs3d_status gs3d_batch_set_layout(gs3d_batch* b, const gs3d_view_spec* views, uint32_t count);
-/* Picks the fastest compositing tile (8x8, 16x16, 32x8) for every view. */
+/* Picks, for every view, the fastest compositing tile (8x8, 16x16, 32x8),
+ * then the fastest sort tile (8, 12 or 16 keys per thread), timing `reps`
+ * frames of each with the given camera pods. */
 gs3d_status gs3d_batch_autotune(gs3d_batch* b, const gs3d_camera_pod* pods, int reps);
 /* Renders every view with its camera pod. use_graph: the first call records
  * the whole batch as one HIP graph, later calls replay it. Enqueue only. */
@@ -368,6 +377,8 @@ gs3d_status gs3d_batch_render(gs3d_batch* b, const gs3d_camera_pod* pods, int us
 gs3d_status gs3d_batch_finish(gs3d_batch* b);
 gs3d_status gs3d_batch_read_target(gs3d_batch* b, uint32_t view, void* dst, size_t bytes);
 gs3d_status gs3d_batch_tile_size(gs3d_batch* b, uint32_t view, uint32_t* tile_w, uint32_t* tile_h);
+/* Keys per thread of the view's sort tiles (what gs3d_batch_autotune picked). */
+gs3d_status gs3d_batch_sort_items(gs3d_batch* b, uint32_t view, uint32_t* items);
 
 /* ---- serving (no counterpart in the reference: the serving form) --------- */
 

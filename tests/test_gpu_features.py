@@ -258,7 +258,29 @@ def test_batch_graph_matches_plain_viewers(gs, device):
     b.autotune(p2, reps=2)
     for i in range(len(layout)):
         assert b.tile_size(i) in ((8, 8), (16, 16), (32, 8))
+        assert b.sort_items(i) in (8, 12, 16)
     b.close()
+
+
+@pytest.mark.parametrize("items", [8, 12])
+def test_sort_items_per_thread_change_nothing_in_a_frame(gs, device, items):
+    """The depth and tile sorts with 2048- or 3072-key tiles give the frame of
+    the default 4096-key tiles: the sorted indices, keys and image."""
+    _case(f"viewer sort items {items}")
+    g = gs.synth_gaussians(2, 300_000, 5)
+    pod = gs.CameraPod.look_at((6.0, 2.0, 14.0), (0.0, 0.0, 0.0), (1280, 720))
+    out = []
+    for it in (16, items):
+        v, img = _render(gs, device, g, pod, 1280, 720, setup=lambda vv: vv.set_sort_items(it),
+                         pod_kind=gs.GaussianPod.HALF)
+        n = int(v.indirect_args()[1])
+        out.append((img, v.indirect_indices(n), v.gaussians_depth(n).view(np.uint32), v.pair_count()))
+        v.close()
+    assert out[0][1].shape[0] > 20_000 and out[0][3] > 20_000
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][2], out[1][2])
+    assert out[0][3] == out[1][3]
 
 
 # ---- serving -----------------------------------------------------------------------
@@ -289,6 +311,19 @@ def test_server_results_same_alone_and_in_the_mix(gs, device):
         alone = srv.render_alone(reqs[i])
         assert np.array_equal(alone, images[i]), f"request {i} differs alone and in the mix"
     srv.close()
+
+
+def test_server_request_beyond_the_budget_is_an_error(gs, device):
+    _case("server: a request that can never fit the HBM budget")
+    model = gs.Model(device, gs.synth_gaussians(2, 10_000, 12))
+    reqs = np.zeros(1, gs.REQUEST_DTYPE)
+    reqs[0]["model"], reqs[0]["width"], reqs[0]["height"], reqs[0]["arrival_s"] = 0, 320, 180, 0.0
+    reqs[0]["pod"] = np.frombuffer(gs.CameraPod.look_at((0, 0, 20), (0, 0, 0), (320, 180)).to_bytes(), np.uint8)
+    srv = gs.Server(device, [model], queues=1, hbm_budget=1 << 20)  # less than one view's buffers
+    with pytest.raises(gs.Gs3dError, match="HBM budget"):
+        srv.run(reqs)
+    srv.close()
+    model.close()
 
 
 # ---- PLY ingestion on the device -------------------------------------------------

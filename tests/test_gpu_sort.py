@@ -1,6 +1,7 @@
 """Standalone RadixSorter (src/radix_sorter.rs:9-96) on the GPU vs a stable
 sort by key (numpy.argsort(kind="stable")), including ragged sizes around the
-4096-key tile, heavy ties and extreme keys."""
+4096-key tile, heavy ties and extreme keys; and the same for the 2048- and
+3072-key tiles (8 and 12 keys per thread) the batch autotuner may pick."""
 from __future__ import annotations
 
 import numpy as np
@@ -9,11 +10,13 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def _check(gs, device, keys, vals=None):
+def _check(gs, device, keys, vals=None, items=None):
     n = keys.shape[0]
     if vals is None:
         vals = np.arange(n, dtype=np.uint32)
     s = gs.RadixSorter(device, max(n, 1))
+    if items is not None:
+        s.set_items(items)
     s.write(keys, vals)
     s.sort()
     k, v = s.read()
@@ -65,4 +68,32 @@ def test_sort_repeated_calls_reuse_state(gs, device):
         k, v = s.read()
         order = np.argsort(keys, kind="stable")
         assert np.array_equal(k, keys[order]) and np.array_equal(v, vals[order])
+    s.close()
+
+
+@pytest.mark.parametrize("items", [8, 12])
+@pytest.mark.parametrize("n", [0, 1, 2047, 2048, 2049, 3071, 3072, 3073, 4097, 1_000_003, 12_000_000])
+def test_sort_items_per_thread_random_sizes(gs, device, items, n):
+    print(f"[case] sort items={items} random n={n}", flush=True)
+    rng = np.random.default_rng(n + items)
+    _check(gs, device, rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32), items=items)
+
+
+@pytest.mark.parametrize("items", [8, 12])
+def test_sort_items_per_thread_ties(gs, device, items):
+    print(f"[case] sort items={items} ties", flush=True)
+    rng = np.random.default_rng(11)
+    n = 300_001
+    keys = rng.choice(np.array([0, 7, 0x3F800000, 0xFFFFFFFF], np.uint32), n)
+    _check(gs, device, keys, rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32), items=items)
+    z = rng.uniform(2.0, 4.0, n).astype(np.float32)
+    keys = (np.float32(1.0) - (np.float32(1.0) - np.float32(0.1) / z)).view(np.uint32)
+    _check(gs, device, keys, items=items)
+
+
+def test_sort_items_per_thread_rejects_other_values(gs, device):
+    s = gs.RadixSorter(device, 16)
+    for bad in (0, 4, 10, 32):
+        with pytest.raises(gs.Gs3dError):
+            s.set_items(bad)
     s.close()

@@ -194,6 +194,7 @@ def _declare(L):
         "gs3d_viewer_draw_standalone": ([vp, vp], i32),
         "gs3d_viewer_set_blend_mode": ([vp, i32], i32),
         "gs3d_viewer_set_tile_size": ([vp, u32, u32], i32),
+        "gs3d_viewer_set_sort_items": ([vp, u32], i32),
         "gs3d_viewer_set_pair_capacity": ([vp, u32], i32),
         "gs3d_viewer_set_profiling": ([vp, i32], i32),
         "gs3d_viewer_stage_times": ([vp, vp], i32),
@@ -215,6 +216,7 @@ def _declare(L):
         "gs3d_radix_sorter_destroy": ([vp], None),
         "gs3d_radix_sorter_write": ([vp, vp, vp, u32], i32),
         "gs3d_radix_sorter_sort": ([vp], i32),
+        "gs3d_radix_sorter_set_items": ([vp, u32], i32),
         "gs3d_radix_sorter_read": ([vp, vp, vp, u32], i32),
         "gs3d_radix_sort_noalloc": ([vp, vp, vp, vp, vp, u32], i32),
         "gs3d_viewer_pair_count": ([vp, C.POINTER(u32)], i32),
@@ -247,6 +249,7 @@ def _declare(L):
         "gs3d_batch_finish": ([vp], i32),
         "gs3d_batch_read_target": ([vp, u32, vp, C.c_size_t], i32),
         "gs3d_batch_tile_size": ([vp, u32, vp, vp], i32),
+        "gs3d_batch_sort_items": ([vp, u32, vp], i32),
         "gs3d_server_new": ([vp, C.POINTER(C.c_void_p), u32, i32, u64, pp], i32),
         "gs3d_server_destroy": ([vp], None),
         "gs3d_server_run": ([vp, vp, u32, C.POINTER(C.c_void_p), vp], i32),
@@ -265,6 +268,7 @@ EXPORTED_SYMBOLS = (
     "gs3d_viewer_update_camera_with_pod", "gs3d_viewer_update_model_transform",
     "gs3d_viewer_update_gaussian_transform", "gs3d_viewer_render", "gs3d_viewer_preprocess", "gs3d_viewer_sort",
     "gs3d_viewer_draw", "gs3d_viewer_draw_standalone", "gs3d_viewer_set_blend_mode", "gs3d_viewer_set_tile_size", "gs3d_viewer_set_pair_capacity",
+    "gs3d_viewer_set_sort_items", "gs3d_radix_sorter_set_items", "gs3d_batch_sort_items",
     "gs3d_viewer_set_profiling", "gs3d_viewer_stage_times", "gs3d_viewer_count_fragments",
     "gs3d_viewer_update_selection", "gs3d_viewer_update_invert_selection",
     "gs3d_viewer_set_depth_stencil", "gs3d_viewer_render_with_depth", "gs3d_texture_write",
@@ -539,6 +543,11 @@ class Viewer:
     def set_tile_size(self, tw: int, th: int) -> None:
         _check(lib().gs3d_viewer_set_tile_size(self._h, int(tw), int(th)))
 
+    def set_sort_items(self, items: int) -> None:
+        """Keys per thread of the depth and tile sorts' tiles: 8, 12 or 16
+        (the default); every choice sorts the same way."""
+        _check(lib().gs3d_viewer_set_sort_items(self._h, int(items)))
+
     def set_pair_capacity(self, pairs: int) -> None:
         """Replaces the (tile, splat) pair lists with ones of `pairs` pairs
         (they grow again when a frame needs more)."""
@@ -636,6 +645,10 @@ class RadixSorter:
 
     def sort(self) -> None:
         _check(lib().gs3d_radix_sorter_sort(self._h))
+
+    def set_items(self, items: int) -> None:
+        """Keys per thread of the sort's tiles: 8, 12 or 16 (the default)."""
+        _check(lib().gs3d_radix_sorter_set_items(self._h, int(items)))
 
     def read(self):
         k = np.zeros(max(self.n, 1), np.uint32)
@@ -806,6 +819,12 @@ class Batch:
         tw, th = C.c_uint32(), C.c_uint32()
         _check(lib().gs3d_batch_tile_size(self._h, int(view), C.byref(tw), C.byref(th)))
         return tw.value, th.value
+
+    def sort_items(self, view: int) -> int:
+        """Keys per thread of the view's sort tiles (the autotuner's pick)."""
+        n = C.c_uint32()
+        _check(lib().gs3d_batch_sort_items(self._h, int(view), C.byref(n)))
+        return n.value
 
     def close(self):
         if self._h:

@@ -239,40 +239,66 @@ void BatchRenderer::finish() {
   if (regrown) drop_graph();  // the lists moved: record again on the next graph render
 }
 
+// Milliseconds of `reps` calls of `work` on stream st, timed with HIP events.
+template <class F>
+static float time_on_stream(hipStream_t st, int reps, F&& work) {
+  hipEvent_t a, b;
+  hip_check(hipEventCreate(&a), "hipEventCreate");
+  hip_check(hipEventCreate(&b), "hipEventCreate");
+  hip_check(hipEventRecord(a, st), "hipEventRecord");
+  for (int r = 0; r < reps; ++r) work();
+  hip_check(hipEventRecord(b, st), "hipEventRecord");
+  hip_check(hipEventSynchronize(b), "hipEventSynchronize");
+  float ms = 0.0f;
+  hip_check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  return ms;
+}
+
 void BatchRenderer::autotune(const CameraPod* pods, int reps) {
   if (reps < 1) reps = 1;
   static const uint32_t sizes[3][2] = {{8, 8}, {16, 16}, {32, 8}};
+  static const uint32_t items[3] = {8, 12, 16};
   drop_graph();
   for (size_t i = 0; i < viewers_.size(); ++i) {
     Viewer& v = *viewers_[i];
+    Texture& t = *targets_[i];
     v.update_camera_with_pod(pods[i]);
     hipStream_t st = dev_.stream(v.queue);
+    // the compositing tile, timed over the draws (binning, tile sort,
+    // compositing) of one projected and sorted frame
     float best = 1e30f;
     int best_k = 1;
     for (int k = 0; k < 3; ++k) {
       v.renderer.set_tile_size(sizes[k][0], sizes[k][1]);
       // warm: sizes the buffers (and the tile lists) for this tile size, so
       // that the timed draws below are complete frames
-      v.render(*targets_[i], false);
+      v.render(t, false);
       dev_.wait();
       v.renderer.settle();
-      hipEvent_t a, b;
-      hip_check(hipEventCreate(&a), "hipEventCreate");
-      hip_check(hipEventCreate(&b), "hipEventCreate");
-      hip_check(hipEventRecord(a, st), "hipEventRecord");
-      for (int r = 0; r < reps; ++r) v.renderer.render(*targets_[i], v.indirect_args_buffer, false);
-      hip_check(hipEventRecord(b, st), "hipEventRecord");
-      hip_check(hipEventSynchronize(b), "hipEventSynchronize");
-      float ms = 0.0f;
-      hip_check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
-      hipEventDestroy(a);
-      hipEventDestroy(b);
+      const float ms = time_on_stream(st, reps, [&] { v.renderer.render(t, v.indirect_args_buffer, false); });
       if (ms < best) {
         best = ms;
         best_k = k;
       }
     }
     v.renderer.set_tile_size(sizes[best_k][0], sizes[best_k][1]);
+    // the sort tile, timed over whole frames (both sorts run in each)
+    best = 1e30f;
+    best_k = 2;
+    for (int k = 0; k < 3; ++k) {
+      v.set_sort_items(items[k]);
+      v.render(t, false);
+      dev_.wait();
+      v.renderer.settle();
+      const float ms = time_on_stream(st, reps, [&] { v.render(t, false); });
+      if (ms < best) {
+        best = ms;
+        best_k = k;
+      }
+    }
+    v.set_sort_items(items[best_k]);
   }
   finish();
 }
@@ -391,6 +417,9 @@ ServeStats Server::run(const std::vector<ServeRequest>& reqs, const std::vector<
       const ServeRequest& r = reqs[pending.front()];
       Slot* s = acquire(r);
       if (!s) {
+        // with nothing in flight no completion will free memory for it
+        if (in_flight.empty())
+          throw Error(GS3D_ERROR_OUT_OF_MEMORY, "a request does not fit the server's HBM budget");
         ++st.deferred_for_memory;
         break;
       }

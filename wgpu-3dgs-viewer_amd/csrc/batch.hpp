@@ -69,8 +69,10 @@ class BatchRenderer {
   ~BatchRenderer();
   // Allocates one view slot (viewer + target) per view. Drops any graph.
   void set_layout(const std::vector<ViewSpec>& views);
-  // Picks the fastest compositing tile (8x8, 16x16, 32x8) for every view by
-  // timing `reps` frames of each with the given cameras.
+  // Picks, for every view, the fastest compositing tile (8x8, 16x16, 32x8)
+  // by timing `reps` draws of each with the given cameras, then the fastest
+  // sort tile (8, 12 or 16 keys per thread, for the depth and tile sorts) by
+  // timing `reps` whole frames of each.
   void autotune(const CameraPod* pods, int reps);
   // Renders the batch. With use_graph, the first call records the whole
   // batch (every view's kernels, spread over the queues) as one HIP graph and

@@ -322,6 +322,13 @@ gs3d_status gs3d_viewer_set_tile_size(gs3d_viewer* v, uint32_t tile_w, uint32_t 
   });
 }
 
+gs3d_status gs3d_viewer_set_sort_items(gs3d_viewer* v, uint32_t items) {
+  return guard([&] {
+    need(v != nullptr, "viewer is null");
+    v->v.set_sort_items(items);
+  });
+}
+
 gs3d_status gs3d_viewer_set_profiling(gs3d_viewer* v, int enabled) {
   return guard([&] {
     need(v != nullptr, "viewer is null");
@@ -431,6 +438,13 @@ gs3d_status gs3d_radix_sorter_sort(gs3d_radix_sorter* s) {
   return guard([&] {
     need(s != nullptr, "sorter is null");
     s->sorter.sort_count(s->count.as<uint32_t>(), 1u);
+  });
+}
+
+gs3d_status gs3d_radix_sorter_set_items(gs3d_radix_sorter* s, uint32_t items) {
+  return guard([&] {
+    need(s != nullptr, "sorter is null");
+    s->sorter.set_items(items);
   });
 }
 
@@ -766,6 +780,14 @@ gs3d_status gs3d_batch_tile_size(gs3d_batch* b, uint32_t view, uint32_t* tile_w,
     need(view < b->b.views(), "no such view");
     *tile_w = b->b.viewer(view).renderer.tile_w();
     *tile_h = b->b.viewer(view).renderer.tile_h();
+  });
+}
+
+gs3d_status gs3d_batch_sort_items(gs3d_batch* b, uint32_t view, uint32_t* items) {
+  return guard([&] {
+    need(b != nullptr && items, "null argument");
+    need(view < b->b.views(), "no such view");
+    *items = b->b.viewer(view).sort_items();
   });
 }
 

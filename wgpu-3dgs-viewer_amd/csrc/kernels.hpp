@@ -80,10 +80,13 @@ struct SortScratch {
   uint32_t* hist;      // [passes][256]
   uint32_t* lookback;  // [passes][max_tiles][256]
   uint32_t* tickets;   // [passes]
-  uint32_t max_tiles;
+  uint32_t max_tiles;  // tiles of the smallest onesweep tile the scratch serves
   uint32_t* err;
+  uint32_t items;  // keys per thread of a onesweep tile: 8, 12 or 16 (0: 16)
 };
 
+// Keys per thread a onesweep tile can have.
+bool sort_items_supported(uint32_t items);
 uint32_t sort_tile_items();
 uint32_t sort_max_tiles(uint32_t count_max);
 size_t sort_lookback_words(uint32_t count_max, int passes);

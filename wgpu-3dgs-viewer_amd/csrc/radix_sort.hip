@@ -7,7 +7,7 @@
 //     keys (per-wave LDS atomics) or from the kernel that wrote the keys;
 //     every pass counts the next pass's digits of the keys it holds;
 //   * each pass is a persistent kernel: a workgroup takes tiles of
-//     256 x GS3D_SORT_ITEMS keys by
+//     256 x ITEMS keys (ITEMS = 16 unless the caller picks 8 or 12) by
 //     ticket, ranks them with wave64 ballot matching (8 ballots per key give
 //     the lanes sharing its digit), publishes per-digit counts, looks back
 //     over its predecessors for the global offset, reorders the tile in LDS
@@ -30,17 +30,13 @@ namespace gs3d {
 
 constexpr int kRsThreads = 256;
 constexpr int kRsWaves = kRsThreads / kWave;
-// keys per thread of a onesweep tile
-#ifndef GS3D_SORT_ITEMS
-#define GS3D_SORT_ITEMS 16
-#endif
-constexpr int kRsItems = GS3D_SORT_ITEMS;
-// onesweep workgroups per CU (the persistent grid): 3 is what is resident
-// at the kernel's ~150 VGPRs (3 waves per SIMD); more only queue behind them
-#ifndef GS3D_SORT_WG_PER_CU
-#define GS3D_SORT_WG_PER_CU 3
-#endif
+// keys per thread of a onesweep tile: 16 by default; 8 and 12 are compiled
+// too (tiles of 2048 / 3072 keys) for callers that time them (the batch
+// autotuner); the look-back scratch is sized for the smallest tile
+constexpr int kRsItems = 16;
+constexpr int kRsMinItems = 8;
 constexpr uint32_t kRsTile = kRsThreads * kRsItems;
+constexpr uint32_t kRsMinTile = kRsThreads * kRsMinItems;
 // predecessors' look-back words requested per step of a digit's look-back
 #ifndef GS3D_SORT_LOOK_WINDOW
 #define GS3D_SORT_LOOK_WINDOW 4
@@ -71,10 +67,11 @@ __device__ __forceinline__ uint32_t block_inclusive_scan_256(uint32_t x, uint32_
   return x + pre;
 }
 
-__global__ __launch_bounds__(256) void k_rs_zero(SortCount cnt, SortScratch sc, int passes, uint32_t* pad_keys,
-                                                 uint32_t pad_begin, uint32_t pad_end, int zero_hist) {
+__global__ __launch_bounds__(256) void k_rs_zero(SortCount cnt, SortScratch sc, int passes, uint32_t tile_keys,
+                                                 uint32_t* pad_keys, uint32_t pad_begin, uint32_t pad_end,
+                                                 int zero_hist) {
   const uint32_t n = sort_n(cnt);
-  const uint32_t tiles = (n + kRsTile - 1) / kRsTile;
+  const uint32_t tiles = (n + tile_keys - 1) / tile_keys;
   const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t stride = gridDim.x * blockDim.x;
   if (zero_hist)
@@ -123,10 +120,11 @@ __global__ __launch_bounds__(256) void k_rs_upsweep(SortCount cnt, const uint32_
 
 // ---- one tile of a pass: load, rank, reorder in LDS ------------------------------
 
-// LDS of a pass over one tile.
+// LDS of a pass over one tile of 256 * ITEMS keys.
+template <int ITEMS>
 struct TileLds {
-  uint32_t keys[kRsTile];
-  uint32_t vals[kRsTile];
+  uint32_t keys[kRsThreads * ITEMS];
+  uint32_t vals[kRsThreads * ITEMS];
   uint32_t wcnt[kRsWaves][256];  // per wave and digit: count, then the wave's offset
   uint32_t dbase[256];           // the digit's first slot in the tile
   uint32_t gbase[256];           // global position of slot 0 of the digit's run
@@ -136,13 +134,14 @@ struct TileLds {
 // Loads tile keys/values [base, base + valid) (key 0xFFFFFFFF past the end:
 // digit 255, ranked after every valid key) in the (wave, item, lane) order
 // the ranking assumes.
+template <int ITEMS>
 __device__ __forceinline__ void load_tile(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                          uint32_t base, uint32_t valid, uint32_t (&key)[kRsItems],
-                                          uint32_t (&val)[kRsItems]) {
+                                          uint32_t base, uint32_t valid, uint32_t (&key)[ITEMS],
+                                          uint32_t (&val)[ITEMS]) {
   const uint32_t lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
 #pragma unroll
-  for (int k = 0; k < kRsItems; ++k) {
-    const uint32_t li = w * (kWave * kRsItems) + uint32_t(k) * kWave + lane;
+  for (int k = 0; k < ITEMS; ++k) {
+    const uint32_t li = w * (kWave * ITEMS) + uint32_t(k) * kWave + lane;
     const bool ok = li < valid;
     key[k] = ok ? kin[base + li] : 0xFFFFFFFFu;
     val[k] = ok ? vin[base + li] : 0u;
@@ -152,11 +151,12 @@ __device__ __forceinline__ void load_tile(const uint32_t* __restrict__ kin, cons
 // Stable rank of every key within its wave by wave64 match-by-ballot (8
 // ballots give the lanes sharing the key's digit); lds.wcnt[w][d] ends as the
 // wave's count of digit d. lds.wcnt must be zero on entry.
-__device__ __forceinline__ void rank_tile(TileLds& lds, const uint32_t (&key)[kRsItems], int shift,
-                                          uint32_t (&lrank)[kRsItems]) {
+template <int ITEMS>
+__device__ __forceinline__ void rank_tile(TileLds<ITEMS>& lds, const uint32_t (&key)[ITEMS], int shift,
+                                          uint32_t (&lrank)[ITEMS]) {
   const uint32_t w = threadIdx.x / kWave;
 #pragma unroll
-  for (int k = 0; k < kRsItems; ++k) {
+  for (int k = 0; k < ITEMS; ++k) {
     const uint32_t d = (key[k] >> shift) & 0xFFu;
     uint64_t m = ~0ull;
 #pragma unroll
@@ -174,7 +174,8 @@ __device__ __forceinline__ void rank_tile(TileLds& lds, const uint32_t (&key)[kR
 
 // Thread d: turns the wave counts of digit d into wave offsets and returns
 // the tile's count of digit d. Call after a barrier that follows rank_tile.
-__device__ __forceinline__ uint32_t digit_wave_offsets(TileLds& lds) {
+template <int ITEMS>
+__device__ __forceinline__ uint32_t digit_wave_offsets(TileLds<ITEMS>& lds) {
   const uint32_t d = threadIdx.x;
   uint32_t tot = 0;
 #pragma unroll
@@ -188,12 +189,13 @@ __device__ __forceinline__ uint32_t digit_wave_offsets(TileLds& lds) {
 
 // Writes every key and value to its slot in the tile sorted by the digit
 // (stable: ranks follow input order). lds.dbase must be set.
-__device__ __forceinline__ void reorder_tile(TileLds& lds, const uint32_t (&key)[kRsItems],
-                                             const uint32_t (&val)[kRsItems], const uint32_t (&lrank)[kRsItems],
+template <int ITEMS>
+__device__ __forceinline__ void reorder_tile(TileLds<ITEMS>& lds, const uint32_t (&key)[ITEMS],
+                                             const uint32_t (&val)[ITEMS], const uint32_t (&lrank)[ITEMS],
                                              int shift) {
   const uint32_t w = threadIdx.x / kWave;
 #pragma unroll
-  for (int k = 0; k < kRsItems; ++k) {
+  for (int k = 0; k < ITEMS; ++k) {
     const uint32_t dk = (key[k] >> shift) & 0xFFu;
     const uint32_t slot = lds.dbase[dk] + lds.wcnt[w][dk] + lrank[k];
     lds.keys[slot] = key[k];
@@ -203,7 +205,8 @@ __device__ __forceinline__ void reorder_tile(TileLds& lds, const uint32_t (&key)
 
 // Writes the tile's valid slots to gbase[digit] + slot: runs of equal digits
 // land contiguously. lds.gbase must be set and visible.
-__device__ __forceinline__ void scatter_tile(const TileLds& lds, uint32_t valid, uint32_t n, int shift,
+template <int ITEMS>
+__device__ __forceinline__ void scatter_tile(const TileLds<ITEMS>& lds, uint32_t valid, uint32_t n, int shift,
                                              uint32_t* __restrict__ kout, uint32_t* __restrict__ vout) {
   for (uint32_t j = threadIdx.x; j < valid; j += kRsThreads) {
     const uint32_t k = lds.keys[j];
@@ -215,19 +218,21 @@ __device__ __forceinline__ void scatter_tile(const TileLds& lds, uint32_t valid,
   }
 }
 
+template <int ITEMS>
 __global__ __launch_bounds__(kRsThreads) void k_rs_onesweep(SortCount cnt, const uint32_t* __restrict__ kin,
                                                            const uint32_t* __restrict__ vin,
                                                            uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                            const uint32_t* __restrict__ hist, uint32_t* lookback,
                                                            uint32_t* ticket, int shift, uint32_t* err,
                                                            uint32_t* hist_next) {
-  __shared__ TileLds lds;
+  constexpr uint32_t kTile = kRsThreads * ITEMS;
+  __shared__ TileLds<ITEMS> lds;
   __shared__ uint32_t s_next[256];  // next pass's digit counts of the tile
   __shared__ uint32_t s_hscan[256];
   __shared__ uint32_t s_tile;
 
   const uint32_t n = sort_n(cnt);
-  const uint32_t num_tiles = (n + kRsTile - 1) / kRsTile;
+  const uint32_t num_tiles = (n + kTile - 1) / kTile;
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid % kWave;
   const uint32_t w = tid / kWave;
@@ -240,21 +245,21 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_onesweep(SortCount cnt, const
   for (;;) {
     const uint32_t t = take_ticket(ticket, &s_tile);
     if (t >= num_tiles) break;
-    const uint32_t base = t * kRsTile;
-    const uint32_t valid = (n - base) < kRsTile ? (n - base) : kRsTile;
+    const uint32_t base = t * kTile;
+    const uint32_t valid = (n - base) < kTile ? (n - base) : kTile;
 
 #pragma unroll
     for (int q = 0; q < kRsWaves; ++q) lds.wcnt[q][tid] = 0u;
     s_next[tid] = 0u;
     __syncthreads();
 
-    uint32_t key[kRsItems], val[kRsItems], lrank[kRsItems];
-    load_tile(kin, vin, base, valid, key, val);
+    uint32_t key[ITEMS], val[ITEMS], lrank[ITEMS];
+    load_tile<ITEMS>(kin, vin, base, valid, key, val);
     if (hist_next) {
       // the next pass's histogram: the same keys, whatever their order
 #pragma unroll
-      for (int k = 0; k < kRsItems; ++k) {
-        const uint32_t li = w * (kWave * kRsItems) + uint32_t(k) * kWave + lane;
+      for (int k = 0; k < ITEMS; ++k) {
+        const uint32_t li = w * (kWave * ITEMS) + uint32_t(k) * kWave + lane;
         if (li < valid) atomicAdd(&s_next[(key[k] >> (shift + 8)) & 0xFFu], 1u);
       }
     }
@@ -264,7 +269,7 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_onesweep(SortCount cnt, const
     // per digit (thread = digit): wave offsets, tile total, publish
     const uint32_t d = tid;
     const uint32_t tot = digit_wave_offsets(lds);
-    const uint32_t pub = tot - ((d == 255u) ? (kRsTile - valid) : 0u);
+    const uint32_t pub = tot - ((d == 255u) ? (kTile - valid) : 0u);
     status_store(lookback + size_t(t) * 256u + d, (t == 0 ? kFlagInclusive : kFlagAggregate) | pub);
     const uint32_t dbase = block_inclusive_scan_256(tot, lds.wsum) - tot;
     lds.dbase[d] = dbase;
@@ -388,7 +393,7 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(SortCount cnt, const 
                                                           uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                           const uint32_t* __restrict__ counts,
                                                           const uint32_t* __restrict__ totals, int shift) {
-  __shared__ TileLds lds;
+  __shared__ TileLds<kRsItems> lds;
   const uint32_t n = sort_n(cnt);
   const uint32_t num_tiles = (n + kRsTile - 1) / kRsTile;
   const uint32_t t = blockIdx.x;
@@ -399,7 +404,7 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(SortCount cnt, const 
 
   // requested first: the loads overlap the scans below
   uint32_t key[kRsItems], val[kRsItems], lrank[kRsItems];
-  load_tile(kin, vin, base, valid, key, val);
+  load_tile<kRsItems>(kin, vin, base, valid, key, val);
   const uint32_t tot_all = totals[tid];
   const uint32_t col = counts[size_t(tid) * num_tiles + t];
 #pragma unroll
@@ -419,22 +424,59 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(SortCount cnt, const 
   scatter_tile(lds, valid, n, shift, kout, vout);
 }
 
+bool sort_items_supported(uint32_t items) { return items == 8 || items == 12 || items == 16; }
+
 uint32_t sort_tile_items() { return kRsTile; }
 
-uint32_t sort_max_tiles(uint32_t count_max) { return (count_max + kRsTile - 1) / kRsTile; }
+uint32_t sort_max_tiles(uint32_t count_max) { return (count_max + kRsMinTile - 1) / kRsMinTile; }
 
 size_t sort_lookback_words(uint32_t count_max, int passes) {
   return size_t(sort_max_tiles(count_max)) * 256u * size_t(passes);
 }
 
 static void launch_zero(hipStream_t st, int num_cus, SortCount cnt, const SortScratch& sc, int passes,
-                        uint32_t* pad_keys, uint32_t pad_begin, uint32_t pad_end, bool zero_hist) {
+                        uint32_t tile_keys, uint32_t* pad_keys, uint32_t pad_begin, uint32_t pad_end,
+                        bool zero_hist) {
   uint32_t blocks = (sc.max_tiles * 256u * uint32_t(passes) + 255u) / 256u;
   const uint32_t cap = uint32_t(num_cus) * 4u;
   if (blocks > cap) blocks = cap;
   if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(k_rs_zero, dim3(blocks), dim3(256), 0, st, cnt, sc, passes, pad_keys, pad_begin, pad_end,
-                     zero_hist ? 1 : 0);
+  hipLaunchKernelGGL(k_rs_zero, dim3(blocks), dim3(256), 0, st, cnt, sc, passes, tile_keys, pad_keys, pad_begin,
+                     pad_end, zero_hist ? 1 : 0);
+}
+
+// Resident onesweep workgroups per CU for ITEMS keys per thread (the
+// persistent grid): what the kernel's registers and LDS allow, as the runtime
+// computes it; more would only queue behind the resident ones.
+template <int ITEMS>
+static uint32_t onesweep_wg_per_cu() {
+  static const uint32_t w = [] {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_rs_onesweep<ITEMS>, kRsThreads, 0) != hipSuccess || b < 1)
+      b = 1;
+    return uint32_t(b);
+  }();
+  return w;
+}
+
+template <int ITEMS>
+static void launch_onesweep_passes(hipStream_t st, int num_cus, SortCount cnt, const uint32_t* ks0,
+                                   const uint32_t* vs0, uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
+                                   int passes, const SortScratch& sc) {
+  const uint32_t tiles = (sc.max_tiles * kRsMinTile + kRsThreads * ITEMS - 1) / (kRsThreads * ITEMS);
+  uint32_t grid = uint32_t(num_cus) * onesweep_wg_per_cu<ITEMS>();
+  if (grid > tiles) grid = tiles;
+  if (grid < 1) grid = 1;
+  for (int p = 0; p < passes; ++p) {
+    const uint32_t* ks = p == 0 ? ks0 : ((p & 1) ? k1 : k0);
+    const uint32_t* vs = p == 0 ? vs0 : ((p & 1) ? v1 : v0);
+    uint32_t* kd = (p & 1) ? k0 : k1;
+    uint32_t* vd = (p & 1) ? v0 : v1;
+    uint32_t* next = (p + 1 < passes) ? sc.hist + (p + 1) * 256 : nullptr;
+    hipLaunchKernelGGL(k_rs_onesweep<ITEMS>, dim3(grid), dim3(kRsThreads), 0, st, cnt, ks, vs, kd, vd,
+                       sc.hist + p * 256, sc.lookback + size_t(p) * sc.max_tiles * 256u, sc.tickets + p, 8 * p,
+                       sc.err, next);
+  }
 }
 
 static int sort_algo() {
@@ -455,7 +497,7 @@ int launch_radix_sort(hipStream_t st, int num_cus, SortCount cnt, uint32_t* k0, 
   if (sort_algo() == 1) {
     // reduce-then-scan: counts [256][tiles] in the look-back words, digit
     // totals in the histogram words; nothing to clear but the pad keys
-    if (pad_keys) launch_zero(st, num_cus, cnt, sc, 0, pad_keys, pad_begin, pad_end, false);
+    if (pad_keys) launch_zero(st, num_cus, cnt, sc, 0, kRsTile, pad_keys, pad_begin, pad_end, false);
     for (int p = 0; p < passes; ++p) {
       const uint32_t* ks = p == 0 ? ks0 : ((p & 1) ? k1 : k0);
       const uint32_t* vs = p == 0 ? vs0 : ((p & 1) ? v1 : v0);
@@ -468,24 +510,20 @@ int launch_radix_sort(hipStream_t st, int num_cus, SortCount cnt, uint32_t* k0, 
     }
     return passes & 1;
   }
-  launch_zero(st, num_cus, cnt, sc, passes, pad_keys, pad_begin, pad_end, !hist_ready);
+  const uint32_t items = sc.items ? sc.items : uint32_t(kRsItems);
+  launch_zero(st, num_cus, cnt, sc, passes, kRsThreads * items, pad_keys, pad_begin, pad_end, !hist_ready);
   if (!hist_ready) {
-    uint32_t up_blocks = (sc.max_tiles * kRsTile + 4095u) / 4096u;
+    uint32_t up_blocks = (sc.max_tiles * kRsMinTile + 4095u) / 4096u;
     if (up_blocks > uint32_t(num_cus) * 4u) up_blocks = uint32_t(num_cus) * 4u;
     if (up_blocks < 1) up_blocks = 1;
     hipLaunchKernelGGL(k_rs_upsweep, dim3(up_blocks), dim3(256), 0, st, cnt, ks0, sc.hist);
   }
-  uint32_t grid = sc.max_tiles;
-  if (grid > uint32_t(num_cus) * GS3D_SORT_WG_PER_CU) grid = uint32_t(num_cus) * GS3D_SORT_WG_PER_CU;
-  for (int p = 0; p < passes; ++p) {
-    const uint32_t* ks = p == 0 ? ks0 : ((p & 1) ? k1 : k0);
-    const uint32_t* vs = p == 0 ? vs0 : ((p & 1) ? v1 : v0);
-    uint32_t* kd = (p & 1) ? k0 : k1;
-    uint32_t* vd = (p & 1) ? v0 : v1;
-    uint32_t* next = (p + 1 < passes) ? sc.hist + (p + 1) * 256 : nullptr;
-    hipLaunchKernelGGL(k_rs_onesweep, dim3(grid), dim3(kRsThreads), 0, st, cnt, ks, vs, kd, vd, sc.hist + p * 256,
-                       sc.lookback + size_t(p) * sc.max_tiles * 256u, sc.tickets + p, 8 * p, sc.err, next);
-  }
+  if (items == 8)
+    launch_onesweep_passes<8>(st, num_cus, cnt, ks0, vs0, k0, v0, k1, v1, passes, sc);
+  else if (items == 12)
+    launch_onesweep_passes<12>(st, num_cus, cnt, ks0, vs0, k0, v0, k1, v1, passes, sc);
+  else
+    launch_onesweep_passes<16>(st, num_cus, cnt, ks0, vs0, k0, v0, k1, v1, passes, sc);
   return passes & 1;
 }
 
@@ -500,7 +538,7 @@ int launch_radix_sort(hipStream_t st, int num_cus, SortCount cnt, uint32_t* k0, 
 // kernel (on the host it is only a stand-in for the symbol): its device address
 // comes from hipGetSymbolAddress, once per device, at device set-up.
 constexpr uint32_t kNoAllocMaxKeys = GS3D_RADIX_SORT_NOALLOC_MAX_KEYS;
-constexpr uint32_t kNoAllocMaxTiles = (kNoAllocMaxKeys + kRsTile - 1) / kRsTile;
+constexpr uint32_t kNoAllocMaxTiles = (kNoAllocMaxKeys + kRsMinTile - 1) / kRsMinTile;
 constexpr int kNoAllocPasses = 4;
 constexpr size_t kNoAllocKeysOff = 0;
 constexpr size_t kNoAllocValsOff = kNoAllocKeysOff + kNoAllocMaxKeys;

@@ -589,8 +589,15 @@ SortScratchBuffers::SortScratchBuffers(Device& dev, uint32_t capacity, int passe
       lookback_(dev, std::max<size_t>(sort_lookback_words(capacity, passes), 1) * 4),
       tickets_(dev, 64) {}
 
+void SortScratchBuffers::set_items(uint32_t items) {
+  if (!sort_items_supported(items))
+    throw Error(GS3D_ERROR_INVALID_ARGUMENT, "sort keys per thread must be 8, 12 or 16");
+  items_ = items;
+}
+
 SortScratch SortScratchBuffers::get() const {
   SortScratch s{};
+  s.items = items_;
   s.hist = hist_.as<uint32_t>();
   s.lookback = lookback_.as<uint32_t>();
   s.tickets = tickets_.as<uint32_t>();
@@ -672,6 +679,13 @@ void Renderer::set_tile_size(uint32_t w, uint32_t h) {
   tile_h_ = h;
 }
 
+void Renderer::set_sort_items(uint32_t items) {
+  if (!sort_items_supported(items))
+    throw Error(GS3D_ERROR_INVALID_ARGUMENT, "sort keys per thread must be 8, 12 or 16");
+  sort_items_ = items;
+  if (scratch_) scratch_->set_items(items);
+}
+
 void Renderer::ensure_capacity(uint32_t pairs) {
   if (pairs <= capacity_) return;
   dev_.wait();
@@ -680,6 +694,7 @@ void Renderer::ensure_capacity(uint32_t pairs) {
     pv_[b] = DeviceBuffer(dev_, size_t(pairs) * 4);
   }
   scratch_ = std::make_unique<SortScratchBuffers>(dev_, pairs, kTileSortPasses);
+  scratch_->set_items(sort_items_);
   capacity_ = pairs;
 }
 
