@@ -359,16 +359,36 @@ struct PxStage {
   uint32_t rowm[kPxMaxRows][kWords];
 };
 
-// Stages splat k (or nothing) into slot threadIdx.x and builds the column and
-// row bitsets of the batch. Contains __syncthreads; every lane must call it.
+// A lane's splat for a batch, requested before the batch before it is
+// blended (the gather's latency then overlaps that blend).
+template <bool Z>
+struct PxFetch {
+  Splat2D r;
+  float z;
+  bool valid;
+};
+
+// k is a valid splat index even for a lane without a splat in the batch (the
+// load is then unused): an unconditional load leaves the compiler nothing to
+// merge, so it does not wait for the load until the staging uses it.
+template <bool Z>
+__device__ __forceinline__ void fetch_px(PxFetch<Z>& f, const Splat2D* splats, const float* zsrc, bool valid,
+                                         uint32_t k) {
+  f.valid = valid;
+  f.r = splats[k];
+  if (Z && zsrc) f.z = zsrc[k];
+}
+
+// Stages the fetched splat (or nothing) into slot threadIdx.x and builds the
+// column and row bitsets of the batch. Contains __syncthreads; every lane must
+// call it.
 template <int NT, bool Z>
 __device__ __forceinline__ void stage_batch_px(PxStage<NT, Z>& s, const Tiling& t, const TileGrid& g,
-                                               const Splat2D* splats, const float* zsrc, bool valid, uint32_t k,
-                                               bool clamp_src, int mode) {
+                                               const PxFetch<Z>& f, bool has_z, bool clamp_src, int mode) {
   const uint32_t tid = threadIdx.x, lane = tid % kWave, sw = tid / kWave;
   int xa = 1, xb = 0, ya = 1, yb = 0;  // empty
-  if (valid) {
-    const Splat2D r = splats[k];
+  if (f.valid) {
+    const Splat2D& r = f.r;
     const uint32_t rg = clamp_src ? clamp01_h2(r.rg) : r.rg;
     const uint32_t ba = clamp_src ? ((clamp01_h2(r.ba) & 0xFFFFu) | (r.ba & 0xFFFF0000u)) : r.ba;
     const float cx = r.cx - t.sx, cy = r.cy - t.sy;
@@ -381,7 +401,7 @@ __device__ __forceinline__ void stage_batch_px(PxStage<NT, Z>& s, const Tiling& 
       s.geo[tid] = make_float4(ux, uy, vx, vy);
       s.col[tid] = make_float4(cx * ux + cy * uy, cx * vx + cy * vy, __uint_as_float(rg), __uint_as_float(ba));
     }
-    if (Z && zsrc) s.z[tid] = zsrc[k];
+    if (Z && has_z) s.z[tid] = f.z;
     if (r.hxy != 0u) {
       // pixel x of the tile holds the centre ox + x + 0.5: inside the bound
       // [cx - hx, cx + hx] for x in [ceil(cx - hx - ox - 0.5), floor(cx + hx - ox - 0.5)]
@@ -393,20 +413,29 @@ __device__ __forceinline__ void stage_batch_px(PxStage<NT, Z>& s, const Tiling& 
       yb = min(int(g.tile_h) - 1, int(floorf(fy + hy)));
     }
   }
-  // column c's and row r's ballots: words 2 sw and 2 sw + 1 of their bitsets
+  // column c's and row r's ballots: words 2 sw and 2 sw + 1 of their
+  // bitsets. Column c's pair is gathered into lane c, row r's into lane
+  // 32 + r (at most 32 columns and 16 rows), and those lanes store them
+  // together after the loops.
+  uint32_t w0 = 0u, w1 = 0u;
   for (int c = 0; c < int(g.tile_w); ++c) {
     const uint64_t bal = __ballot(xa <= c && c <= xb);
-    if (lane == 0) {
-      s.colm[c][2 * sw] = uint32_t(bal);
-      s.colm[c][2 * sw + 1] = uint32_t(bal >> 32);
-    }
+    const bool mine = lane == uint32_t(c);
+    w0 = mine ? uint32_t(bal) : w0;
+    w1 = mine ? uint32_t(bal >> 32) : w1;
   }
   for (int r = 0; r < int(g.tile_h); ++r) {
     const uint64_t bal = __ballot(ya <= r && r <= yb);
-    if (lane == 0) {
-      s.rowm[r][2 * sw] = uint32_t(bal);
-      s.rowm[r][2 * sw + 1] = uint32_t(bal >> 32);
-    }
+    const bool mine = lane == uint32_t(32 + r);
+    w0 = mine ? uint32_t(bal) : w0;
+    w1 = mine ? uint32_t(bal >> 32) : w1;
+  }
+  if (lane < g.tile_w) {
+    s.colm[lane][2 * sw] = w0;
+    s.colm[lane][2 * sw + 1] = w1;
+  } else if (lane >= 32u && lane < 32u + g.tile_h) {
+    s.rowm[lane - 32u][2 * sw] = w0;
+    s.rowm[lane - 32u][2 * sw + 1] = w1;
   }
   __syncthreads();
 }
@@ -417,13 +446,19 @@ __device__ __forceinline__ void stage_batch_px(PxStage<NT, Z>& s, const Tiling& 
 // the threshold (checked every 16 steps).
 // track_a: the destination's alpha may be below 1 (a loaded target); after a
 // clear it stays 1 under the over operator. count: count the kept fragments.
-template <int MODE, bool EXACT, bool DEPTH, int NT>
+// PLAIN: q8 is true and track_a and count are false (the common draw: an
+// Rgba8Unorm target after a clear), fixed at compile time so that the
+// per-splat step carries no test of them.
+template <int MODE, bool EXACT, bool DEPTH, int NT, bool PLAIN>
 __device__ __forceinline__ void blend_px(const PxStage<NT, DEPTH>& s, const Tiling& t, uint32_t lx, uint32_t ly,
-                                         bool q8, bool track_a, bool count, float sd2l, float ell2l, float& T,
-                                         float& cr, float& cg, float& cb, float& ca, bool& done, DepthLane& dl,
-                                         uint32_t& frags) {
+                                         bool q8_in, bool track_a_in, bool count_in, float sd2l, float ell2l,
+                                         float& T, float& cr, float& cg, float& cb, float& ca, bool& done,
+                                         DepthLane& dl, uint32_t& frags) {
 #pragma clang fp contract(fast)
   constexpr int kWords = PxStage<NT, DEPTH>::kWords;
+  const bool q8 = PLAIN || q8_in;
+  const bool track_a = !PLAIN && track_a_in;
+  const bool count = !PLAIN && count_in;
   const char* geo = reinterpret_cast<const char*>(s.geo);
   const char* col = reinterpret_cast<const char*>(s.col);
   // the words (32 slots each) that hold splats for this pixel
@@ -508,8 +543,11 @@ __device__ __forceinline__ void blend_px(const PxStage<NT, DEPTH>& s, const Tili
   if (!EXACT) done = done || T < kMinTransmittance;
 }
 
-template <bool EXACT, bool DEPTH, int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_composite_px(CompositeArgs a) {
+// 8 waves per SIMD; 7 for depth-tested draws (their depth state and prefetched
+// fragment depths need the registers)
+template <bool EXACT, bool DEPTH, int NT, bool PLAIN>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(DEPTH ? 7 : 8))) void k_composite_px(
+    CompositeArgs a) {
   if (*a.pair_count > a.capacity) return;  // truncated lists: the frame is drawn again
   __shared__ PxStage<NT, DEPTH> s;
   __shared__ uint32_t s_item;
@@ -518,7 +556,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
   const float sd2l = fc.max_std_dev * fc.max_std_dev * kLog2e;
   const float e = fc.max_std_dev - 0.1f;
   const float ell2l = e * e * kLog2e;
-  const bool q8 = a.format == kFormatRgba8Unorm;
+  const bool q8 = PLAIN || a.format == kFormatRgba8Unorm;
   const uint32_t nt = blockDim.x, tid = threadIdx.x;
   // read once: loads from the ticket's cache line while every workgroup
   // takes tickets from it wait behind the atomics
@@ -545,7 +583,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
 
     // fast: colour and transmittance; exact: the destination (q8: 0..255)
     float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f, ca = q8 ? 255.0f : 1.0f;
-    if (EXACT && a.load && t.inside) {
+    if (EXACT && !PLAIN && a.load && t.inside) {
       if (q8) {
         const uint32_t d = reinterpret_cast<const uint32_t*>(a.out)[o];
         cr = float(d & 0xFFu);
@@ -563,25 +601,41 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
     bool done = !t.inside;
     uint32_t frags = 0;
     // after a clear the destination's alpha stays 1 (1 a + 1 (1 - a))
-    const bool track_a = a.load != 0;
-    const bool count = a.frag_count != nullptr;
-    for (uint32_t p0 = 0; p0 < front - back;) {
-      const uint32_t cnt = (front - back - p0) < nt ? (front - back - p0) : nt;
+    const bool track_a = !PLAIN && a.load != 0;
+    const bool count = !PLAIN && a.frag_count != nullptr;
+    // batches of nt pairs; exact: farthest first (the list's order), fast:
+    // nearest first. Lane tid stages pair `slot(p0)` of the batch at p0; its
+    // splat is requested one batch ahead and its pair value two batches ahead.
+    const uint32_t len = front - back;
+    const float* zsrc = DEPTH ? a.splat_depth : nullptr;
+    auto slot = [&](uint32_t p0) { return EXACT ? back + p0 + tid : front - 1u - p0 - tid; };
+    auto in_batch = [&](uint32_t p0) { return p0 < len && tid < len - p0; };
+    // (a lane past the end of the list reads pair `back`, so that every load
+    // is in bounds and unconditional)
+    PxFetch<DEPTH> f;
+    {
+      const bool v0 = in_batch(0u);
+      fetch_px(f, a.splats, zsrc, v0, a.pair_vals[v0 ? slot(0u) : back]);
+    }
+    bool v_next = in_batch(nt);
+    uint32_t k_next = a.pair_vals[v_next ? slot(nt) : back];
+    for (uint32_t p0 = 0; p0 < len;) {
+      const uint32_t cnt = (len - p0) < nt ? (len - p0) : nt;
       __syncthreads();
-      const bool valid = tid < cnt;
-      // exact: farthest first (the list's order); fast: nearest first
-      const uint32_t pidx = EXACT ? back + p0 + tid : front - 1 - p0 - tid;
-      stage_batch_px(s, t, a.grid, a.splats, DEPTH ? a.splat_depth : nullptr, valid, valid ? a.pair_vals[pidx] : 0u,
-                     q8, mode);
+      stage_batch_px(s, t, a.grid, f, zsrc != nullptr, q8, mode);
+      // the next batch's splat, the one after's pair value
+      fetch_px(f, a.splats, zsrc, v_next, k_next);
+      v_next = in_batch(p0 + 2u * nt);
+      k_next = a.pair_vals[v_next ? slot(p0 + 2u * nt) : back];
       if (mode == kModeSplat)
-        blend_px<kModeSplat, EXACT, DEPTH, NT>(s, t, lx, ly, q8, track_a, count, sd2l, ell2l, T, cr, cg, cb, ca, done, dl,
-                                                frags);
+        blend_px<kModeSplat, EXACT, DEPTH, NT, PLAIN>(s, t, lx, ly, q8, track_a, count, sd2l, ell2l, T, cr, cg, cb, ca,
+                                                       done, dl, frags);
       else if (mode == kModeEllipse)
-        blend_px<kModeEllipse, EXACT, DEPTH, NT>(s, t, lx, ly, q8, track_a, count, sd2l, ell2l, T, cr, cg, cb, ca, done, dl,
-                                                frags);
+        blend_px<kModeEllipse, EXACT, DEPTH, NT, PLAIN>(s, t, lx, ly, q8, track_a, count, sd2l, ell2l, T, cr, cg, cb,
+                                                         ca, done, dl, frags);
       else
-        blend_px<kModePoint, EXACT, DEPTH, NT>(s, t, lx, ly, q8, track_a, count, sd2l, ell2l, T, cr, cg, cb, ca, done, dl,
-                                                frags);
+        blend_px<kModePoint, EXACT, DEPTH, NT, PLAIN>(s, t, lx, ly, q8, track_a, count, sd2l, ell2l, T, cr, cg, cb, ca,
+                                                       done, dl, frags);
       p0 += cnt;
       if (!EXACT && __syncthreads_and(done ? 1 : 0)) break;
     }
@@ -637,16 +691,23 @@ uint32_t composite_max_heavy_items(uint32_t capacity, uint32_t seg_len) {
 
 template <bool EXACT>
 static void launch_px(hipStream_t st, const CompositeArgs& a, uint32_t grid, uint32_t threads) {
+  // the exact blend of an Rgba8Unorm target after a clear, not counting
+  // fragments, has its own instantiation (no per-splat test of those flags)
+  const bool plain = EXACT && !a.depth && a.format == kFormatRgba8Unorm && !a.load && a.frag_count == nullptr;
   if (threads == kWave) {
     if (a.depth)
-      hipLaunchKernelGGL((k_composite_px<EXACT, true, kWave>), dim3(grid), dim3(threads), 0, st, a);
+      hipLaunchKernelGGL((k_composite_px<EXACT, true, kWave, false>), dim3(grid), dim3(threads), 0, st, a);
+    else if (plain)
+      hipLaunchKernelGGL((k_composite_px<EXACT, false, kWave, EXACT>), dim3(grid), dim3(threads), 0, st, a);
     else
-      hipLaunchKernelGGL((k_composite_px<EXACT, false, kWave>), dim3(grid), dim3(threads), 0, st, a);
+      hipLaunchKernelGGL((k_composite_px<EXACT, false, kWave, false>), dim3(grid), dim3(threads), 0, st, a);
   } else {
     if (a.depth)
-      hipLaunchKernelGGL((k_composite_px<EXACT, true, kCompMaxThreads>), dim3(grid), dim3(threads), 0, st, a);
+      hipLaunchKernelGGL((k_composite_px<EXACT, true, kCompMaxThreads, false>), dim3(grid), dim3(threads), 0, st, a);
+    else if (plain)
+      hipLaunchKernelGGL((k_composite_px<EXACT, false, kCompMaxThreads, EXACT>), dim3(grid), dim3(threads), 0, st, a);
     else
-      hipLaunchKernelGGL((k_composite_px<EXACT, false, kCompMaxThreads>), dim3(grid), dim3(threads), 0, st, a);
+      hipLaunchKernelGGL((k_composite_px<EXACT, false, kCompMaxThreads, false>), dim3(grid), dim3(threads), 0, st, a);
   }
 }
 
