@@ -19,7 +19,9 @@ Sections (all run by default; --sections picks a subset):
            radius 5-30, graph replay, compositing tile and sort tile
            autotuned per view.
   serve    8 resident models totalling ~40M Gaussians (sizes log-uniform
-           1M-12M), a Poisson stream of 400 requests/s for --serve-seconds,
+           1M-12M), a Poisson stream of 400 requests/s for --serve-seconds
+           (then 1200 requests/s for --serve-burst-seconds, with several
+           requests in flight at once),
            each request (model, camera on a smooth trajectory, resolution of
            the mix set); continuous batching with HBM-budgeted admission:
            p50/p99 latency (completion - arrival) and frames/s.
@@ -277,7 +279,7 @@ def serve_model_sizes(rng):
     return [int(x) for x in np.round(raw * (40e6 / raw.sum()))]
 
 
-def run_serve(gs, dev, seconds, rate, queues, seed=4):
+def run_serve(gs, dev, seconds, rate, queues, burst_rate=0.0, burst_seconds=0.0, seed=4):
     rng = np.random.default_rng(seed)
     sizes = serve_model_sizes(rng)
     models = []
@@ -294,6 +296,13 @@ def run_serve(gs, dev, seconds, rate, queues, seed=4):
         warm[i]["pod"] = np.frombuffer(orbit_pod(gs, w, h, 0.3 * i).to_bytes(), np.uint8)
     server.run(warm)
     stats, _ = server.run(req)
+    if burst_rate > 0.0 and burst_seconds > 0.0:
+        # a denser stream on the same resident models: requests overlap on
+        # the device (several in flight), which the 400/s stream rarely needs
+        burst = serve_requests(gs, np.random.default_rng(seed + 1), sizes, burst_rate, burst_seconds)
+        bstats, _ = server.run(burst)
+        bstats["rate_req_per_s"] = burst_rate
+        stats["burst"] = bstats
     server.close()
     for m in models:
         m.close()
@@ -351,6 +360,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=400.0)
     ap.add_argument("--serve-queues", type=int, default=8, help="device queues of the serving loop")
+    ap.add_argument("--serve-burst-rate", type=float, default=1200.0,
+                    help="requests/s of a second, denser stream after the first (0: none)")
+    ap.add_argument("--serve-burst-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -391,8 +403,13 @@ def main():
         out_extra["mix"] = run_mix(gs, dev, max(args.steps // 4, 4), 2, args.queues)
         log(f"mix: {out_extra['mix']['fps']:.1f} frames/s")
     if "serve" in sections:
-        out_extra["serve"] = run_serve(gs, dev, args.serve_seconds, args.serve_rate, args.serve_queues)
+        out_extra["serve"] = run_serve(gs, dev, args.serve_seconds, args.serve_rate, args.serve_queues,
+                                       args.serve_burst_rate, args.serve_burst_seconds)
         log(f"serve: p50 {out_extra['serve']['p50_ms']:.2f} ms p99 {out_extra['serve']['p99_ms']:.2f} ms")
+        if "burst" in out_extra["serve"]:
+            b = out_extra["serve"]["burst"]
+            log(f"serve burst {b['rate_req_per_s']:.0f}/s: p50 {b['p50_ms']:.2f} ms p99 {b['p99_ms']:.2f} ms, "
+                f"{b['mean_in_flight']:.2f} in flight")
 
     # rooflines from the plain section's HIP events (same model, resolution
     # and cameras as the batched section)
@@ -401,7 +418,9 @@ def main():
     comp_ms = float(stage_ms[5])
     comp_flops = plain["fragments_mean"] * FLOPS_PER_FRAGMENT[args.blend]
     comp_tflops = comp_flops / (comp_ms * 1e-3) / 1e12
-    comp_kernel = "k_composite_px<%s, false, 256>" % ("true" if args.blend == "exact" else "false")
+    # the exact blend of a cleared Rgba8Unorm target has its own instantiation
+    comp_kernel = ("k_composite_px<true, false, 256, true>" if args.blend == "exact"
+                   else "k_composite_px<false, false, 256, false>")
     comp_traffic, traffic_src = profile_traffic(comp_kernel)
     composite = {"bound": "valu", "achieved": round(comp_tflops, 3), "peak": VALU_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                  "frac": round(comp_tflops / VALU_FP32_PEAK_TFLOPS, 4), "traffic": comp_traffic,

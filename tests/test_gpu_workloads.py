@@ -10,8 +10,9 @@ and at least 45 dB. The reference analogue is MultiModelViewer::render
 (src/multi_model.rs:476-530: per model preprocess and sort, then the draws).
 
 Serving (bench.py's serve section): 8 resident models totalling 40M
-Gaussians, a Poisson stream of 400 requests/s; one captured request per model
-is the same alone and in the mix, and matches the oracle as above.
+Gaussians, a Poisson stream of 400 requests/s and a denser one of 1500/s
+(several requests on the device at once); one captured request per model is
+the same alone and in the mix, and matches the oracle as above.
 """
 from __future__ import annotations
 
@@ -134,6 +135,24 @@ def test_served_frames_same_alone_and_in_the_mix(serve):
         _case(f"served request {i}: model {int(req[i]['model'])} {int(req[i]['width'])}x{int(req[i]['height'])}")
         alone = srv.render_alone(req[i])
         assert np.array_equal(alone, images[i]), f"request {i} differs alone and in the mix"
+
+
+def test_dense_stream_overlaps_and_is_the_same_alone(gs, serve):
+    """1500 requests/s: by Little's law several requests are on the device at
+    once (continuous batching, not a queue); each captured request renders the
+    same alone as in that mix."""
+    sizes, srv, req, capture, stats, images = serve
+    rng = np.random.default_rng(9)
+    dense = bench.serve_requests(gs, rng, sizes, 1500.0, 1.0)
+    cap = [int(np.nonzero(dense["model"] == m)[0][0]) for m in range(len(sizes))]
+    _case(f"dense stream: {len(dense)} requests")
+    dstats, dimages = srv.run(dense, capture=cap)
+    print(f"dense serve: {dstats}", flush=True)
+    assert dstats["completed"] == dstats["requests"] == len(dense)
+    assert dstats["mean_in_flight"] > 1.0
+    for i in cap:
+        alone = srv.render_alone(dense[i])
+        assert np.array_equal(alone, dimages[i]), f"dense request {i} differs alone and in the mix"
 
 
 def test_served_frames_match_oracle(gs, orc, serve):
