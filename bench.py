@@ -25,6 +25,11 @@ Sections (all run by default; --sections picks a subset):
            each request (model, camera on a smooth trajectory, resolution of
            the mix set); continuous batching with HBM-budgeted admission:
            p50/p99 latency (completion - arrival) and frames/s.
+  sortscale (not run by default) the standalone RadixSorter on 5.2M, 20M
+           and 80M random (key, value) pairs with 16, 12 and 8 keys per
+           thread: per-sort wall time and algorithmic GB/s (16 B per pair
+           per pass); the per-pass kernel times come from a kernel trace of
+           this section.
 
 Every frame runs the whole path: preprocess (cull + projection + depth keys)
 -> depth radix sort -> tile binning -> tile sort -> compositing. Gaussians are
@@ -312,6 +317,35 @@ def run_serve(gs, dev, seconds, rate, queues, burst_rate=0.0, burst_seconds=0.0,
     return stats
 
 
+SORTSCALE_SIZES = (5_200_000, 20_000_000, 80_000_000)
+
+
+def run_sortscale(gs, dev, reps=3, seed=0):
+    """The standalone sort at several sizes and tile shapes: best of `reps`
+    sorts of the same random pairs (uploaded again before each)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for n in SORTSCALE_SIZES:
+        keys = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+        vals = np.arange(n, dtype=np.uint32)
+        for items in (16, 12, 8):
+            s = gs.RadixSorter(dev, n)
+            s.set_items(items)
+            best = float("inf")
+            for _ in range(reps):
+                s.write(keys, vals)
+                dev.poll()
+                t0 = time.perf_counter()
+                s.sort()
+                dev.poll()
+                best = min(best, time.perf_counter() - t0)
+            s.close()
+            out.append({"pairs": n, "keys_per_thread": items, "ms": round(best * 1e3, 4),
+                        "gbs_per_pass": round(16 * n / (best / 4) / 1e9, 1)})
+            log(f"sortscale: {n} pairs, {items} keys per thread: {best * 1e3:.3f} ms")
+    return out
+
+
 def cpu_baseline(gs, g, w, h, threads, budget_s):
     """The oracle (a CPU restatement of the reference's path, not the
     reference, which cannot be built here) on frames of the benchmark's own
@@ -373,7 +407,7 @@ def main():
         sys.exit(2)
     sections = [s for s in args.sections.split(",") if s]
     for s in sections:
-        if s not in ("batched", "plain", "mix", "serve"):
+        if s not in ("batched", "plain", "mix", "serve", "sortscale"):
             ap.error(f"unknown section {s}")
 
     import __graft_entry__
@@ -402,6 +436,8 @@ def main():
     if "mix" in sections:
         out_extra["mix"] = run_mix(gs, dev, max(args.steps // 4, 4), 2, args.queues)
         log(f"mix: {out_extra['mix']['fps']:.1f} frames/s")
+    if "sortscale" in sections:
+        out_extra["sortscale"] = run_sortscale(gs, dev)
     if "serve" in sections:
         out_extra["serve"] = run_serve(gs, dev, args.serve_seconds, args.serve_rate, args.serve_queues,
                                        args.serve_burst_rate, args.serve_burst_seconds)

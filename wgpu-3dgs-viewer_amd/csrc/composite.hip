@@ -610,15 +610,18 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(DEPTH ? 7 : 
     const float* zsrc = DEPTH ? a.splat_depth : nullptr;
     auto slot = [&](uint32_t p0) { return EXACT ? back + p0 + tid : front - 1u - p0 - tid; };
     auto in_batch = [&](uint32_t p0) { return p0 < len && tid < len - p0; };
-    // (a lane past the end of the list reads pair `back`, so that every load
-    // is in bounds and unconditional)
+    // (a lane past the end of a non-empty list reads pair `back`, written by
+    // this frame's binning, so that every load is in bounds and unconditional;
+    // an empty list loads nothing)
     PxFetch<DEPTH> f;
-    {
+    bool v_next = false;
+    uint32_t k_next = 0u;
+    if (len > 0u) {
       const bool v0 = in_batch(0u);
       fetch_px(f, a.splats, zsrc, v0, a.pair_vals[v0 ? slot(0u) : back]);
+      v_next = in_batch(nt);
+      k_next = a.pair_vals[v_next ? slot(nt) : back];
     }
-    bool v_next = in_batch(nt);
-    uint32_t k_next = a.pair_vals[v_next ? slot(nt) : back];
     for (uint32_t p0 = 0; p0 < len;) {
       const uint32_t cnt = (len - p0) < nt ? (len - p0) : nt;
       __syncthreads();
