@@ -158,13 +158,17 @@ __device__ __forceinline__ void rank_tile(TileLds<ITEMS>& lds, const uint32_t (&
 #pragma unroll
   for (int k = 0; k < ITEMS; ++k) {
     const uint32_t d = (key[k] >> shift) & 0xFFu;
-    uint64_t m = ~0ull;
+    // lanes whose digit differs in some bit: the OR over the digit's bits of
+    // (the lanes with the bit set) XOR (all lanes if this lane's bit is set)
+    uint32_t ml = 0u, mh = 0u;
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t bb = __ballot(bit);
-      m &= bit ? bb : ~bb;
+      const int32_t sb = int32_t(d << (31 - b)) >> 31;  // 0 or -1
+      const uint64_t bb = __builtin_amdgcn_ballot_w64(sb != 0);
+      ml |= uint32_t(bb) ^ uint32_t(sb);
+      mh |= uint32_t(bb >> 32) ^ uint32_t(sb);
     }
+    const uint64_t m = (uint64_t(~mh) << 32) | ~ml;  // the lanes sharing this lane's digit
     const uint32_t r = mask_rank(m);
     const uint32_t prev = lds.wcnt[w][d];
     if (r == 0u) lds.wcnt[w][d] = prev + uint32_t(__popcll(m));
