@@ -125,8 +125,7 @@ template <int ITEMS>
 struct TileLds {
   uint32_t keys[kRsThreads * ITEMS];
   uint32_t vals[kRsThreads * ITEMS];
-  uint32_t wcnt[kRsWaves][256];  // per wave and digit: count, then the wave's offset
-  uint32_t dbase[256];           // the digit's first slot in the tile
+  uint32_t wcnt[kRsWaves][256];  // per wave and digit: count, then the wave's first slot
   uint32_t gbase[256];           // global position of slot 0 of the digit's run
   uint32_t wsum[kRsWaves];
 };
@@ -191,8 +190,17 @@ __device__ __forceinline__ uint32_t digit_wave_offsets(TileLds<ITEMS>& lds) {
   return tot;
 }
 
+// Thread d: adds the digit's first slot in the tile to its wave offsets, so
+// that wcnt[w][d] is wave w's first slot for digit d.
+template <int ITEMS>
+__device__ __forceinline__ void set_wave_slots(TileLds<ITEMS>& lds, uint32_t dbase) {
+#pragma unroll
+  for (int q = 0; q < kRsWaves; ++q) lds.wcnt[q][threadIdx.x] += dbase;
+}
+
 // Writes every key and value to its slot in the tile sorted by the digit
-// (stable: ranks follow input order). lds.dbase must be set.
+// (stable: ranks follow input order). lds.wcnt[w][d] must hold wave w's
+// first slot for digit d (set_wave_slots).
 template <int ITEMS>
 __device__ __forceinline__ void reorder_tile(TileLds<ITEMS>& lds, const uint32_t (&key)[ITEMS],
                                              const uint32_t (&val)[ITEMS], const uint32_t (&lrank)[ITEMS],
@@ -201,7 +209,7 @@ __device__ __forceinline__ void reorder_tile(TileLds<ITEMS>& lds, const uint32_t
 #pragma unroll
   for (int k = 0; k < ITEMS; ++k) {
     const uint32_t dk = (key[k] >> shift) & 0xFFu;
-    const uint32_t slot = lds.dbase[dk] + lds.wcnt[w][dk] + lrank[k];
+    const uint32_t slot = lds.wcnt[w][dk] + lrank[k];
     lds.keys[slot] = key[k];
     lds.vals[slot] = val[k];
   }
@@ -276,7 +284,7 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_onesweep(SortCount cnt, const
     const uint32_t pub = tot - ((d == 255u) ? (kTile - valid) : 0u);
     status_store(lookback + size_t(t) * 256u + d, (t == 0 ? kFlagInclusive : kFlagAggregate) | pub);
     const uint32_t dbase = block_inclusive_scan_256(tot, lds.wsum) - tot;
-    lds.dbase[d] = dbase;
+    set_wave_slots(lds, dbase);
     __syncthreads();
 
     reorder_tile(lds, key, val, lrank, shift);
@@ -420,7 +428,7 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(SortCount cnt, const 
   __syncthreads();
   const uint32_t tot = digit_wave_offsets(lds);
   const uint32_t dbase = block_inclusive_scan_256(tot, lds.wsum) - tot;
-  lds.dbase[tid] = dbase;
+  set_wave_slots(lds, dbase);
   lds.gbase[tid] = gb - dbase;
   __syncthreads();
   reorder_tile(lds, key, val, lrank, shift);
@@ -468,7 +476,12 @@ static void launch_onesweep_passes(hipStream_t st, int num_cus, SortCount cnt, c
                                    const uint32_t* vs0, uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
                                    int passes, const SortScratch& sc) {
   const uint32_t tiles = (sc.max_tiles * kRsMinTile + kRsThreads * ITEMS - 1) / (kRsThreads * ITEMS);
-  uint32_t grid = uint32_t(num_cus) * onesweep_wg_per_cu<ITEMS>();
+  // 4096-key tiles: 3 workgroups per CU, although 4 fit (126 VGPRs, 40 KB of
+  // LDS): the fourth is 5% slower on sorts of ~5M keys (faster only from
+  // ~20M keys on)
+  uint32_t wpc = onesweep_wg_per_cu<ITEMS>();
+  if (ITEMS == 16 && wpc > 3u) wpc = 3u;
+  uint32_t grid = uint32_t(num_cus) * wpc;
   if (grid > tiles) grid = tiles;
   if (grid < 1) grid = 1;
   for (int p = 0; p < passes; ++p) {
