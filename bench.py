@@ -26,8 +26,8 @@ Sections (all run by default; --sections picks a subset):
            the mix set); continuous batching with HBM-budgeted admission:
            p50/p99 latency (completion - arrival) and frames/s.
   sortscale (not run by default) the standalone RadixSorter on 5.2M, 20M
-           and 80M random (key, value) pairs with 16, 12 and 8 keys per
-           thread: per-sort wall time and algorithmic GB/s (16 B per pair
+           and 80M random (key, value) pairs with 24, 20, 16, 12 and 8 keys
+           per thread: per-sort wall time and algorithmic GB/s (16 B per pair
            per pass); the per-pass kernel times come from a kernel trace of
            this section.
 
@@ -318,6 +318,7 @@ def run_serve(gs, dev, seconds, rate, queues, burst_rate=0.0, burst_seconds=0.0,
 
 
 SORTSCALE_SIZES = (5_200_000, 20_000_000, 80_000_000)
+SORTSCALE_ITEMS = (24, 20, 16, 12, 8)
 
 
 def run_sortscale(gs, dev, reps=3, seed=0):
@@ -328,7 +329,7 @@ def run_sortscale(gs, dev, reps=3, seed=0):
     for n in SORTSCALE_SIZES:
         keys = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
         vals = np.arange(n, dtype=np.uint32)
-        for items in (16, 12, 8):
+        for items in SORTSCALE_ITEMS:
             s = gs.RadixSorter(dev, n)
             s.set_items(items)
             best = float("inf")

@@ -236,7 +236,8 @@ gs3d_status gs3d_viewer_set_pair_capacity(gs3d_viewer* v, uint32_t pairs);
 /* Compositing tile size: 8x8, 16x16 (default) or 32x8 pixels. */
 gs3d_status gs3d_viewer_set_tile_size(gs3d_viewer* v, uint32_t tile_w, uint32_t tile_h);
 /* No counterpart in the reference (a tuning choice of this sort's kernels):
- * keys per thread of the depth and tile sorts' tiles, 8, 12 or 16 (default).
+ * keys per thread of the depth and tile sorts' tiles, 8, 12, 16, 20 (default)
+ * or 24.
  * Every choice gives the same sorted order. */
 gs3d_status gs3d_viewer_set_sort_items(gs3d_viewer* v, uint32_t items);
 /* Record HIP events around each stage of every render (off by default). */
@@ -278,8 +279,8 @@ void gs3d_radix_sorter_destroy(gs3d_radix_sorter* s);
 gs3d_status gs3d_radix_sorter_write(gs3d_radix_sorter* s, const uint32_t* keys, const uint32_t* vals, uint32_t n);
 /* RadixSorter::sort: stable ascending sort of the n pairs by key. */
 gs3d_status gs3d_radix_sorter_sort(gs3d_radix_sorter* s);
-/* No counterpart in the reference: keys per thread of the sort's tiles, 8, 12
- * or 16 (default); every choice gives the same order. */
+/* No counterpart in the reference: keys per thread of the sort's tiles, 8, 12,
+ * 16, 20 (default) or 24; every choice gives the same order. */
 gs3d_status gs3d_radix_sorter_set_items(gs3d_radix_sorter* s, uint32_t items);
 gs3d_status gs3d_radix_sorter_read(gs3d_radix_sorter* s, uint32_t* keys, uint32_t* vals, uint32_t n);
 
@@ -367,7 +368,7 @@ void gs3d_batch_destroy(gs3d_batch* b);
 /* One view slot (viewer + RGBA8 target) per view. */
 gs3d_status gs3d_batch_set_layout(gs3d_batch* b, const gs3d_view_spec* views, uint32_t count);
 /* Picks, for every view, the fastest compositing tile (8x8, 16x16, 32x8),
- * then the fastest sort tile (8, 12 or 16 keys per thread), timing `reps`
+ * then the fastest sort tile (8, 12, 16, 20 or 24 keys per thread), timing `reps`
  * frames of each with the given camera pods. */
 gs3d_status gs3d_batch_autotune(gs3d_batch* b, const gs3d_camera_pod* pods, int reps);
 /* Renders every view with its camera pod. use_graph: the first call records

@@ -258,19 +258,19 @@ def test_batch_graph_matches_plain_viewers(gs, device):
     b.autotune(p2, reps=2)
     for i in range(len(layout)):
         assert b.tile_size(i) in ((8, 8), (16, 16), (32, 8))
-        assert b.sort_items(i) in (8, 12, 16)
+        assert b.sort_items(i) in (8, 12, 16, 20, 24)
     b.close()
 
 
-@pytest.mark.parametrize("items", [8, 12])
+@pytest.mark.parametrize("items", [8, 12, 16, 24])
 def test_sort_items_per_thread_change_nothing_in_a_frame(gs, device, items):
-    """The depth and tile sorts with 2048- or 3072-key tiles give the frame of
-    the default 4096-key tiles: the sorted indices, keys and image."""
+    """The depth and tile sorts with 2048- to 6144-key tiles give the frame of
+    the default 5120-key tiles: the sorted indices, keys and image."""
     _case(f"viewer sort items {items}")
     g = gs.synth_gaussians(2, 300_000, 5)
     pod = gs.CameraPod.look_at((6.0, 2.0, 14.0), (0.0, 0.0, 0.0), (1280, 720))
     out = []
-    for it in (16, items):
+    for it in (20, items):
         v, img = _render(gs, device, g, pod, 1280, 720, setup=lambda vv: vv.set_sort_items(it),
                          pod_kind=gs.GaussianPod.HALF)
         n = int(v.indirect_args()[1])

@@ -1,7 +1,8 @@
 """Standalone RadixSorter (src/radix_sorter.rs:9-96) on the GPU vs a stable
 sort by key (numpy.argsort(kind="stable")), including ragged sizes around the
-4096-key tile, heavy ties and extreme keys; and the same for the 2048- and
-3072-key tiles (8 and 12 keys per thread) the batch autotuner may pick."""
+default 5120-key tile, heavy ties and extreme keys; and the same for the
+2048-, 3072-, 4096- and 6144-key tiles (8, 12, 16 and 24 keys per thread) the
+batch autotuner may pick."""
 from __future__ import annotations
 
 import numpy as np
@@ -26,7 +27,8 @@ def _check(gs, device, keys, vals=None, items=None):
     s.close()
 
 
-@pytest.mark.parametrize("n", [0, 1, 2, 255, 4095, 4096, 4097, 65537, 1_000_003, 8_000_000, 12_000_000])
+@pytest.mark.parametrize("n", [0, 1, 2, 255, 4095, 4096, 4097, 5119, 5120, 5121, 65537, 1_000_003, 8_000_000,
+                               12_000_000])
 def test_sort_random_sizes(gs, device, n):
     print(f"[case] sort random n={n}", flush=True)
     rng = np.random.default_rng(n)
@@ -71,15 +73,16 @@ def test_sort_repeated_calls_reuse_state(gs, device):
     s.close()
 
 
-@pytest.mark.parametrize("items", [8, 12])
-@pytest.mark.parametrize("n", [0, 1, 2047, 2048, 2049, 3071, 3072, 3073, 4097, 1_000_003, 12_000_000])
+@pytest.mark.parametrize("items", [8, 12, 16, 24])
+@pytest.mark.parametrize("n", [0, 1, 2047, 2048, 2049, 3071, 3072, 3073, 4097, 6143, 6144, 6145, 1_000_003,
+                               12_000_000])
 def test_sort_items_per_thread_random_sizes(gs, device, items, n):
     print(f"[case] sort items={items} random n={n}", flush=True)
     rng = np.random.default_rng(n + items)
     _check(gs, device, rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32), items=items)
 
 
-@pytest.mark.parametrize("items", [8, 12])
+@pytest.mark.parametrize("items", [8, 12, 16, 24])
 def test_sort_items_per_thread_ties(gs, device, items):
     print(f"[case] sort items={items} ties", flush=True)
     rng = np.random.default_rng(11)
@@ -93,7 +96,7 @@ def test_sort_items_per_thread_ties(gs, device, items):
 
 def test_sort_items_per_thread_rejects_other_values(gs, device):
     s = gs.RadixSorter(device, 16)
-    for bad in (0, 4, 10, 32):
+    for bad in (0, 4, 10, 28, 32):
         with pytest.raises(gs.Gs3dError):
             s.set_items(bad)
     s.close()

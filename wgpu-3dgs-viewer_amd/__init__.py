@@ -544,8 +544,8 @@ class Viewer:
         _check(lib().gs3d_viewer_set_tile_size(self._h, int(tw), int(th)))
 
     def set_sort_items(self, items: int) -> None:
-        """Keys per thread of the depth and tile sorts' tiles: 8, 12 or 16
-        (the default); every choice sorts the same way."""
+        """Keys per thread of the depth and tile sorts' tiles: 8, 12, 16, 20
+        (the default) or 24; every choice sorts the same way."""
         _check(lib().gs3d_viewer_set_sort_items(self._h, int(items)))
 
     def set_pair_capacity(self, pairs: int) -> None:
@@ -647,7 +647,7 @@ class RadixSorter:
         _check(lib().gs3d_radix_sorter_sort(self._h))
 
     def set_items(self, items: int) -> None:
-        """Keys per thread of the sort's tiles: 8, 12 or 16 (the default)."""
+        """Keys per thread of the sort's tiles: 8, 12, 16, 20 (the default) or 24."""
         _check(lib().gs3d_radix_sorter_set_items(self._h, int(items)))
 
     def read(self):

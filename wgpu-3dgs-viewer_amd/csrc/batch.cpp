@@ -259,7 +259,7 @@ static float time_on_stream(hipStream_t st, int reps, F&& work) {
 void BatchRenderer::autotune(const CameraPod* pods, int reps) {
   if (reps < 1) reps = 1;
   static const uint32_t sizes[3][2] = {{8, 8}, {16, 16}, {32, 8}};
-  static const uint32_t items[3] = {8, 12, 16};
+  static const uint32_t items[5] = {8, 12, 16, 20, 24};
   drop_graph();
   for (size_t i = 0; i < viewers_.size(); ++i) {
     Viewer& v = *viewers_[i];
@@ -286,8 +286,8 @@ void BatchRenderer::autotune(const CameraPod* pods, int reps) {
     v.renderer.set_tile_size(sizes[best_k][0], sizes[best_k][1]);
     // the sort tile, timed over whole frames (both sorts run in each)
     best = 1e30f;
-    best_k = 2;
-    for (int k = 0; k < 3; ++k) {
+    best_k = 3;
+    for (int k = 0; k < 5; ++k) {
       v.set_sort_items(items[k]);
       v.render(t, false);
       dev_.wait();

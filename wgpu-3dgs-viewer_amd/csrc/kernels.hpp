@@ -82,7 +82,7 @@ struct SortScratch {
   uint32_t* tickets;   // [passes]
   uint32_t max_tiles;  // tiles of the smallest onesweep tile the scratch serves
   uint32_t* err;
-  uint32_t items;  // keys per thread of a onesweep tile: 8, 12 or 16 (0: 16)
+  uint32_t items;  // keys per thread of a onesweep tile: 8, 12, 16, 20 or 24 (0: 20)
 };
 
 // Keys per thread a onesweep tile can have.

@@ -30,10 +30,10 @@ namespace gs3d {
 
 constexpr int kRsThreads = 256;
 constexpr int kRsWaves = kRsThreads / kWave;
-// keys per thread of a onesweep tile: 16 by default; 8 and 12 are compiled
-// too (tiles of 2048 / 3072 keys) for callers that time them (the batch
+// keys per thread of a onesweep tile: 20 by default (5120-key tiles); 8, 12,
+// 16 and 24 are compiled too for callers that time them (the batch
 // autotuner); the look-back scratch is sized for the smallest tile
-constexpr int kRsItems = 16;
+constexpr int kRsItems = 20;
 constexpr int kRsMinItems = 8;
 constexpr uint32_t kRsTile = kRsThreads * kRsItems;
 constexpr uint32_t kRsMinTile = kRsThreads * kRsMinItems;
@@ -436,7 +436,9 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(SortCount cnt, const 
   scatter_tile(lds, valid, n, shift, kout, vout);
 }
 
-bool sort_items_supported(uint32_t items) { return items == 8 || items == 12 || items == 16; }
+bool sort_items_supported(uint32_t items) {
+  return items == 8 || items == 12 || items == 16 || items == 20 || items == 24;
+}
 
 uint32_t sort_tile_items() { return kRsTile; }
 
@@ -539,8 +541,12 @@ int launch_radix_sort(hipStream_t st, int num_cus, SortCount cnt, uint32_t* k0, 
     launch_onesweep_passes<8>(st, num_cus, cnt, ks0, vs0, k0, v0, k1, v1, passes, sc);
   else if (items == 12)
     launch_onesweep_passes<12>(st, num_cus, cnt, ks0, vs0, k0, v0, k1, v1, passes, sc);
-  else
+  else if (items == 16)
     launch_onesweep_passes<16>(st, num_cus, cnt, ks0, vs0, k0, v0, k1, v1, passes, sc);
+  else if (items == 24)
+    launch_onesweep_passes<24>(st, num_cus, cnt, ks0, vs0, k0, v0, k1, v1, passes, sc);
+  else
+    launch_onesweep_passes<20>(st, num_cus, cnt, ks0, vs0, k0, v0, k1, v1, passes, sc);
   return passes & 1;
 }
 

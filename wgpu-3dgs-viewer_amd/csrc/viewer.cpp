@@ -591,7 +591,7 @@ SortScratchBuffers::SortScratchBuffers(Device& dev, uint32_t capacity, int passe
 
 void SortScratchBuffers::set_items(uint32_t items) {
   if (!sort_items_supported(items))
-    throw Error(GS3D_ERROR_INVALID_ARGUMENT, "sort keys per thread must be 8, 12 or 16");
+    throw Error(GS3D_ERROR_INVALID_ARGUMENT, "sort keys per thread must be 8, 12, 16, 20 or 24");
   items_ = items;
 }
 
@@ -681,7 +681,7 @@ void Renderer::set_tile_size(uint32_t w, uint32_t h) {
 
 void Renderer::set_sort_items(uint32_t items) {
   if (!sort_items_supported(items))
-    throw Error(GS3D_ERROR_INVALID_ARGUMENT, "sort keys per thread must be 8, 12 or 16");
+    throw Error(GS3D_ERROR_INVALID_ARGUMENT, "sort keys per thread must be 8, 12, 16, 20 or 24");
   sort_items_ = items;
   if (scratch_) scratch_->set_items(items);
 }

@@ -406,7 +406,7 @@ class SortScratchBuffers {
   SortScratch get() const;
   uint32_t capacity() const { return capacity_; }
   // Keys per thread of the onesweep tiles the sorts with this scratch use:
-  // 8, 12 or 16 (the default); every choice gives the same order.
+  // 8, 12, 16, 20 (the default) or 24; every choice gives the same order.
   void set_items(uint32_t items);
   uint32_t items() const { return items_; }
   // Replaces the lists with ones of exactly `pairs` pairs (they still grow
@@ -417,7 +417,7 @@ class SortScratchBuffers {
   Device& dev_;
   uint32_t capacity_;
   int passes_;
-  uint32_t items_ = 16;
+  uint32_t items_ = 20;
   DeviceBuffer hist_, lookback_, tickets_;
 };
 
@@ -433,7 +433,7 @@ class RadixSorter {
   void sort_count(const uint32_t* count_dev, uint32_t count_mul, bool hist_ready = false);
   uint32_t* hist() const { return scratch_.get().hist; }
   uint32_t len() const { return indices_.len(); }
-  // Keys per thread of the sort's onesweep tiles (8, 12 or 16).
+  // Keys per thread of the sort's onesweep tiles (8, 12, 16, 20 or 24).
   void set_items(uint32_t items) { scratch_.set_items(items); }
   uint32_t items() const { return scratch_.items(); }
 
@@ -479,7 +479,7 @@ class Renderer {
   void set_blend(BlendMode b) { blend_ = b; }
   BlendMode blend() const { return blend_; }
   void set_tile_size(uint32_t w, uint32_t h);
-  // Keys per thread of the tile sort's onesweep tiles (8, 12 or 16).
+  // Keys per thread of the tile sort's onesweep tiles (8, 12, 16, 20 or 24).
   void set_sort_items(uint32_t items);
   uint32_t sort_items() const { return sort_items_; }
   StageTimer& timer() { return timer_; }
@@ -520,7 +520,7 @@ class Renderer {
   DeviceBuffer own_consts_;
   BlendMode blend_ = kBlendExactUnorm8;  // the reference's blend
   uint32_t tile_w_ = 16, tile_h_ = 16;
-  uint32_t sort_items_ = 16;
+  uint32_t sort_items_ = 20;
   uint32_t capacity_ = 0;
   DeviceBuffer pk_[2], pv_[2];
   std::unique_ptr<SortScratchBuffers> scratch_;
@@ -583,8 +583,8 @@ class Viewer {
   // 151-172). The bitset starts all zero and invert true: nothing is hidden.
   void update_selection(const uint32_t* words, uint32_t count);
   void update_invert_selection(bool invert) { preprocessor.set_invert_selection(invert); }
-  // Keys per thread of the depth and tile sorts' onesweep tiles: 8, 12 or 16
-  // (the default). A tuning choice only: every choice sorts the same way.
+  // Keys per thread of the depth and tile sorts' onesweep tiles: 8, 12, 16,
+  // 20 (the default) or 24. A tuning choice only: every choice sorts the same way.
   void set_sort_items(uint32_t items) {
     radix_sorter.set_items(items);
     renderer.set_sort_items(items);
