@@ -51,7 +51,7 @@ def _compile(src: str, build_dir: str = BUILD, defines=()) -> str:
 
 def build(verbose: bool = False, out: str = LIB, defines=()) -> str:
     """Compile and link; `defines` (NAME=VALUE) set tuning macros such as
-    GS3D_COMPOSITE_UNROLL, `out` another library path (load it with
+    GS3D_BIN_ITEMS, `out` another library path (load it with
     GS3D_LIBRARY)."""
     build_dir = BUILD if out == LIB else out + ".build"
     os.makedirs(build_dir, exist_ok=True)
