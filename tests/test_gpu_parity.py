@@ -128,6 +128,33 @@ def test_standalone_renderer_matches_fused(gs, orc, device, scene0):
     assert np.array_equal(a.read(), b.read())
 
 
+def test_large_target_three_tile_sort_passes(gs, orc, device):
+    """A 4096x4096 target: 8x8 tiles give a 512x512 grid, whose 18-bit tile
+    keys take three tile-sort passes (16x16 tiles: 65,536 tiles, two passes).
+    Both images equal each other and match the oracle."""
+    config, n, seed, _, _ = CONFIG0
+    w = h = 4096
+    _case(f"config0 at {w}x{h}, 8x8 and 16x16 tiles")
+    g, pod, frame = _frame(gs, orc, config, n, seed, w, h)
+    viewer = gs.Viewer(device, gs.TextureFormat.RGBA8_UNORM, g)
+    viewer.update_camera_with_pod(pod)
+    imgs = []
+    for tile in ((8, 8), (16, 16)):
+        viewer.set_tile_size(*tile)
+        t = gs.Texture(device, w, h, gs.TextureFormat.RGBA8_UNORM)
+        viewer.render(t)
+        device.poll()
+        imgs.append(t.read())
+        t.close()
+    viewer.close()
+    assert np.array_equal(imgs[0], imgs[1])
+    pos, color, sh, cov = oracle_inputs(orc, g)
+    _, _, oimg = orc.full_frame(frame, pos, color, sh, cov, w, h, orc.TARGET_RGBA8_STEP, threads=16)
+    d, p = max_abs_diff(imgs[0], oimg), psnr(imgs[0], oimg)
+    print(f"{w}x{h}: max diff {d}/255, PSNR {p:.2f} dB", flush=True)
+    assert d <= 2 and p >= 45.0
+
+
 @pytest.mark.parametrize("tile", [(8, 8), (32, 8)])
 def test_tile_sizes_identical(gs, device, scene0, tile):
     _case(f"tile size {tile}")
