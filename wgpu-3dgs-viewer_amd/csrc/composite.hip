@@ -359,6 +359,12 @@ struct PxStage {
   uint32_t rowm[kPxMaxRows][kWords];
 };
 
+// v_writelane_b32: lane `lane` (wave-uniform) of `old` becomes the
+// wave-uniform `val`. Declared through the LLVM intrinsic's name, for which
+// this compiler has no builtin; the compiler schedules it like any other
+// instruction (no inline assembly).
+extern "C" __device__ int gs3d_writelane(int val, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
+
 // A lane's splat for a batch, requested before the batch before it is
 // blended (the gather's latency then overlaps that blend).
 template <bool Z>
@@ -414,28 +420,30 @@ __device__ __forceinline__ void stage_batch_px(PxStage<NT, Z>& s, const Tiling& 
     }
   }
   // column c's and row r's ballots: words 2 sw and 2 sw + 1 of their
-  // bitsets. Column c's pair is gathered into lane c, row r's into lane
+  // bitsets. Column c's pair is written into lane c, row r's into lane
   // 32 + r (at most 32 columns and 16 rows), and those lanes store them
-  // together after the loops.
-  uint32_t w0 = 0u, w1 = 0u;
+  // together after the loops. A range [a, b] is tested as c - a <= b - a in
+  // unsigned arithmetic (one compare; an empty range starts at 2^30).
+  const bool xe = xa <= xb, ye = ya <= yb;
+  const uint32_t x0 = xe ? uint32_t(xa) : 0x40000000u, xs = xe ? uint32_t(xb - xa) : 0u;
+  const uint32_t y0 = ye ? uint32_t(ya) : 0x40000000u, ys = ye ? uint32_t(yb - ya) : 0u;
+  int w0 = 0, w1 = 0;
   for (int c = 0; c < int(g.tile_w); ++c) {
-    const uint64_t bal = __ballot(xa <= c && c <= xb);
-    const bool mine = lane == uint32_t(c);
-    w0 = mine ? uint32_t(bal) : w0;
-    w1 = mine ? uint32_t(bal >> 32) : w1;
+    const uint64_t bal = __builtin_amdgcn_ballot_w64(uint32_t(c) - x0 <= xs);
+    w0 = gs3d_writelane(int(uint32_t(bal)), c, w0);
+    w1 = gs3d_writelane(int(uint32_t(bal >> 32)), c, w1);
   }
   for (int r = 0; r < int(g.tile_h); ++r) {
-    const uint64_t bal = __ballot(ya <= r && r <= yb);
-    const bool mine = lane == uint32_t(32 + r);
-    w0 = mine ? uint32_t(bal) : w0;
-    w1 = mine ? uint32_t(bal >> 32) : w1;
+    const uint64_t bal = __builtin_amdgcn_ballot_w64(uint32_t(r) - y0 <= ys);
+    w0 = gs3d_writelane(int(uint32_t(bal)), 32 + r, w0);
+    w1 = gs3d_writelane(int(uint32_t(bal >> 32)), 32 + r, w1);
   }
   if (lane < g.tile_w) {
-    s.colm[lane][2 * sw] = w0;
-    s.colm[lane][2 * sw + 1] = w1;
+    s.colm[lane][2 * sw] = uint32_t(w0);
+    s.colm[lane][2 * sw + 1] = uint32_t(w1);
   } else if (lane >= 32u && lane < 32u + g.tile_h) {
-    s.rowm[lane - 32u][2 * sw] = w0;
-    s.rowm[lane - 32u][2 * sw + 1] = w1;
+    s.rowm[lane - 32u][2 * sw] = uint32_t(w0);
+    s.rowm[lane - 32u][2 * sw + 1] = uint32_t(w1);
   }
   __syncthreads();
 }
