@@ -488,6 +488,11 @@ __device__ __forceinline__ void blend_px(const PxStage<NT, DEPTH>& s, const Tili
     if (!__any(cur != 0u)) break;
     {
       const bool act = cur != 0u;
+      // a lane with no bit left reads slot `base` (slot 0 of a word with a
+      // splat of this batch, or of the batch): slots fill from 0, so it was
+      // staged and its values are finite, and 0 * finite leaves the colour
+      // unchanged. A slot past the batch's end holds stale data (an Inf or
+      // NaN there would reach the colour through the zero-alpha blend).
       const uint32_t b = act ? uint32_t(__builtin_ctz(cur)) : 0u;
       cur &= cur - 1u;
       const uint32_t q = (base + b) * 16u;  // byte offset of the slot
