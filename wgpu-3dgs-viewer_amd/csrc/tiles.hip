@@ -40,25 +40,18 @@ __global__ __launch_bounds__(256) void k_bin_reset(uint32_t* ranges, uint32_t ra
   }
 }
 
-__device__ __forceinline__ uint32_t scan256_inclusive(uint32_t x, uint32_t* s_wsum) {
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
   const uint32_t lane = threadIdx.x % kWave;
-  const uint32_t w = threadIdx.x / kWave;
 #pragma unroll
   for (int off = 1; off < kWave; off <<= 1) {
     const uint32_t y = __shfl_up(x, off, kWave);
     if (lane >= uint32_t(off)) x += y;
   }
-  if (lane == kWave - 1) s_wsum[w] = x;
-  __syncthreads();
-  uint32_t pre = 0;
-  for (uint32_t q = 0; q < w; ++q) pre += s_wsum[q];
-  __syncthreads();
-  return x + pre;
+  return x;
 }
 
 __global__ __launch_bounds__(kBinThreads) void k_bin(BinArgs a) {
-  __shared__ uint32_t s_wsum[kBinThreads / kWave];
-  __shared__ uint32_t s_ktot[kBinItems];
+  __shared__ uint32_t s_wtot[kBinItems][kBinThreads / kWave];  // each wave's count per item
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_base;
   __shared__ uint32_t s_hist[256];  // digit 0 of this block's stored tile ids
@@ -73,33 +66,50 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(BinArgs a) {
 
   uint32_t cnt[kBinItems], excl[kBinItems], gidx[kBinItems];
   uint2 rr[kBinItems];  // packed rectangles (tile_rect)
+  // every item's index, then every item's rectangle: unconditional loads
+  // (an item past the end reads instance v - 1 and is dropped below), so the
+  // wave waits once for each set instead of twice per item
+  const uint32_t last = v - 1u;  // v > 0: this tile holds an instance
 #pragma unroll
   for (int k = 0; k < kBinItems; ++k) {
     const uint32_t s = base + uint32_t(k) * kBinThreads + tid;
-    cnt[k] = 0u;
-    gidx[k] = 0u;
-    rr[k] = make_uint2(0xFFFFFFFFu, 0u);
-    if (s < v) {
-      const uint32_t i = a.indices[s];
-      gidx[k] = i;
-      if (a.rects) {
-        rr[k] = a.rects[i];  // 8 bytes per instance written by the vertex work
-      } else {
-        const float sd = a.fc->max_std_dev;
-        rr[k] = tile_rect(a.splats[i], sd * sd, a.grid);
-      }
-      cnt[k] = rect_tiles(unpack_rect(rr[k]));
-    }
+    gidx[k] = a.indices[s < v ? s : last];
   }
-  // exclusive scan in (item, thread) order == sorted instance order
+  if (a.rects) {
+#pragma unroll
+    for (int k = 0; k < kBinItems; ++k) rr[k] = a.rects[gidx[k]];  // 8 bytes per instance written by the vertex work
+  } else {
+    const float sd = a.fc->max_std_dev;
+#pragma unroll
+    for (int k = 0; k < kBinItems; ++k) rr[k] = tile_rect(a.splats[gidx[k]], sd * sd, a.grid);
+  }
+#pragma unroll
+  for (int k = 0; k < kBinItems; ++k) {
+    const uint32_t s = base + uint32_t(k) * kBinThreads + tid;
+    cnt[k] = s < v ? rect_tiles(unpack_rect(rr[k])) : 0u;
+  }
+  // exclusive scan in (item, thread) order == sorted instance order: a wave
+  // scan per item, then one table of the waves' totals (one barrier)
+  const uint32_t w = tid / kWave;
+#pragma unroll
+  for (int k = 0; k < kBinItems; ++k) {
+    const uint32_t inc = wave_inclusive_scan(cnt[k]);
+    excl[k] = inc - cnt[k];
+    if (tid % kWave == kWave - 1) s_wtot[k][w] = inc;
+  }
+  __syncthreads();
   uint32_t run = 0;
 #pragma unroll
   for (int k = 0; k < kBinItems; ++k) {
-    const uint32_t inc = scan256_inclusive(cnt[k], s_wsum);
-    excl[k] = run + inc - cnt[k];
-    if (tid == kBinThreads - 1) s_ktot[k] = inc;
-    __syncthreads();
-    run += s_ktot[k];
+    uint32_t before = 0, all = 0;  // waves before this one, all waves
+#pragma unroll
+    for (uint32_t q = 0; q < kBinThreads / kWave; ++q) {
+      const uint32_t c = s_wtot[k][q];
+      before += q < w ? c : 0u;
+      all += c;
+    }
+    excl[k] += run + before;
+    run += all;
   }
   if (tid < kWave) {
     // the first wave looks back (run is the same in every thread)
