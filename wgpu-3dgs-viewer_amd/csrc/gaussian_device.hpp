@@ -250,14 +250,15 @@ __device__ __forceinline__ uint32_t rect_tiles(const TileRect& t) {
 
 // First R rows of the column-major 4x4 matrix m times v, contracted to fused
 // multiply-adds (for the splat record, which is not compared bit for bit).
-template <int R>
-__device__ __forceinline__ void mv4_fast(const float* m, const float v[4], float* out) {
+template <int R, class T>
+__device__ __forceinline__ void mv4_fast(const T* m, const float v[4], float* out) {
 #pragma clang fp contract(fast)
 #pragma unroll
   for (int r = 0; r < R; ++r) out[r] = m[0 * 4 + r] * v[0] + m[1 * 4 + r] * v[1] + m[2 * 4 + r] * v[2] + m[3 * 4 + r] * v[3];
 }
 
-__device__ __forceinline__ void mv3_fast(const float* m, const float v[3], float out[3]) {
+template <class T>
+__device__ __forceinline__ void mv3_fast(const T* m, const float v[3], float out[3]) {
 #pragma clang fp contract(fast)
 #pragma unroll
   for (int r = 0; r < 3; ++r) out[r] = m[0 * 3 + r] * v[0] + m[1 * 3 + r] * v[1] + m[2 * 3 + r] * v[2];
@@ -360,8 +361,8 @@ __device__ __forceinline__ void sh_fold(const ShWords<POD>& w, float x, float y,
 
 // color(): render.wesl:58-73 — the view direction in model space and the SH
 // colour, clamped at 0 (`sw` holds the Gaussian's loaded SH words).
-template <int POD>
-__device__ __forceinline__ void splat_colour(const FrameConsts& fc, const ShWords<POD>& sw, const float world[4],
+template <int POD, class F>
+__device__ __forceinline__ void splat_colour(const F& fc, const ShWords<POD>& sw, const float world[4],
                                              uint32_t color_bits, float rgb[3]) {
 #pragma clang fp contract(fast)
   const float wvd[3] = {fc.campos[0] - world[0], fc.campos[1] - world[1], fc.campos[2] - world[2]};
@@ -390,8 +391,8 @@ constexpr float kThinRho = 31.6227766f;  // 1 / sqrt(kThinEigenRatio)
 // order (render.wesl:75-130 as restated by the oracle: u = 2 (M.x, -M.y) /
 // |M|^2, v = 2 (m.x, -m.y) / |m|^2, of which the record keeps
 // rho = |M| / |m|); ok is false when the splat is not drawn.
-template <int POD>
-__device__ __forceinline__ void thin_splat_axes(const FrameConsts& fc, const float pos[3], const CovWords& cw,
+template <int POD, class F>
+__device__ __forceinline__ void thin_splat_axes(const F& fc, const float pos[3], const CovWords& cw,
                                              float& ux, float& uy, float& rho, float& hx, float& hy, bool& ok) {
   float c[6];
   cov3d_of<POD, false>(cw, c);
@@ -417,8 +418,8 @@ __device__ __forceinline__ void thin_splat_axes(const FrameConsts& fc, const flo
 // the return value (true) asks the caller to have thin_splat_record redo
 // them (the projection kernel queues such splats for a kernel of their own,
 // so that the exact evaluation adds nothing to its registers).
-template <int POD, bool kInlineThin>
-__device__ __forceinline__ bool splat_geometry(const FrameConsts& fc, const float world[4], const float c[6],
+template <int POD, bool kInlineThin, class F>
+__device__ __forceinline__ bool splat_geometry(const F& fc, const float world[4], const float c[6],
                                                const float pos[3], const CovWords& cw, uint32_t color_bits,
                                                const float rgb[3], Splat2D* out, float* depth_out, uint2* rect_out,
                                                const TileGrid* rect_grid) {
@@ -540,8 +541,8 @@ __device__ __forceinline__ bool splat_geometry(const FrameConsts& fc, const floa
 // Redoes the axes, half extents and tile rectangle of a thin splat's record
 // (splat_geometry returned true for it) in the exact order; the centre and
 // colour stay.
-template <int POD>
-__device__ __forceinline__ void thin_splat_record(const FrameConsts& fc, const float pos[3], const CovWords& cw,
+template <int POD, class F>
+__device__ __forceinline__ void thin_splat_record(const F& fc, const float pos[3], const CovWords& cw,
                                                   Splat2D* out, uint2* rect_out, const TileGrid* rect_grid) {
   Splat2D s = *out;
   float hx, hy;
@@ -553,8 +554,8 @@ __device__ __forceinline__ void thin_splat_record(const FrameConsts& fc, const f
 }
 
 // render.wesl:75-130 for one instance: the SH words are loaded here.
-template <int POD>
-__device__ __forceinline__ void compute_splat(const FrameConsts& fc, const GaussianStreams& g, uint32_t i,
+template <int POD, class F>
+__device__ __forceinline__ void compute_splat(const F& fc, const GaussianStreams& g, uint32_t i,
                                               uint32_t color_bits, const float world[4], const float c[6],
                                               const float pos[3], const CovWords& cw, Splat2D* out,
                                               float* depth_out, uint2* rect_out = nullptr,

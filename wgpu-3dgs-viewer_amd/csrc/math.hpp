@@ -24,7 +24,9 @@
 
 namespace gs3d {
 
-GS3D_HD void m4v(const float* m, const float v[4], float out[4]) {
+// (T: float in any address space)
+template <class T>
+GS3D_HD void m4v(const T* m, const float v[4], float out[4]) {
 #pragma unroll
   for (int r = 0; r < 4; ++r)
     out[r] = ((m[0 * 4 + r] * v[0] + m[1 * 4 + r] * v[1]) + m[2 * 4 + r] * v[2]) + m[3 * 4 + r] * v[3];
@@ -38,7 +40,8 @@ GS3D_HD void m4m4(const float* a, const float* b, float* out) {
           a[3 * 4 + r] * b[c * 4 + 3];
 }
 
-GS3D_HD void m3m3(const float* a, const float* b, float* out) {
+template <class A, class B>
+GS3D_HD void m3m3(const A* a, const B* b, float* out) {
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -137,7 +140,9 @@ GS3D_HD bool cull(float x, float y, float z) {
 }
 
 // utils.wesl:25-48: (cov.xx, cov.xy, cov.yy) in pixels^2.
-GS3D_HD void cov2d(const FrameConsts& fc, const float pos[3], const float c[6], float out[3]) {
+// (F: FrameConsts in any address space)
+template <class F>
+GS3D_HD void cov2d(const F& fc, const float pos[3], const float c[6], float out[3]) {
   const float vrk[9] = {c[0], c[1], c[2], c[1], c[3], c[4], c[2], c[4], c[5]};
   const float p4[4] = {pos[0], pos[1], pos[2], 1.0f};
   float t[4];
@@ -162,7 +167,8 @@ GS3D_HD void cov2d(const FrameConsts& fc, const float pos[3], const float c[6], 
 }
 
 // utils.wesl:54-79: (major.xy, minor.xy); all zero when lambda_min < 0.
-GS3D_HD void cov2d_axes(const FrameConsts& fc, const float pos[3], const float c[6], float sd, float out[4]) {
+template <class F>
+GS3D_HD void cov2d_axes(const F& fc, const float pos[3], const float c[6], float sd, float out[4]) {
   float cv[3];
   cov2d(fc, pos, c, cv);
   const float mid = 0.5f * (cv[0] + cv[2]);

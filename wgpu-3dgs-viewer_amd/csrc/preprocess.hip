@@ -75,7 +75,8 @@ __global__ __launch_bounds__(64) void k_frame_consts(const CameraPod* __restrict
 
 // world = model * (pos, 1) and the centre's NDC ((proj * view) * world /
 // w, preprocess.wesl:80-82) in the bit-exact order of math.hpp.
-__device__ __forceinline__ void project_centre(const FrameConsts& fc, float4 pc, float world[4], float& nx,
+template <class F>
+__device__ __forceinline__ void project_centre(const F& fc, float4 pc, float world[4], float& nx,
                                                float& ny, float& nz) {
   const float p4[4] = {pc.x, pc.y, pc.z, 1.0f};
   float clip[4];
@@ -88,7 +89,8 @@ __device__ __forceinline__ void project_centre(const FrameConsts& fc, float4 pc,
 
 // preprocess.wesl:87-98: a culled centre is re-tested after moving it
 // towards the screen by the projected major axis.
-__device__ __forceinline__ bool bound_visible(const FrameConsts& fc, const float pos[3], const float c[6], float nx,
+template <class F>
+__device__ __forceinline__ bool bound_visible(const F& fc, const float pos[3], const float c[6], float nx,
                                               float ny, float nz) {
   const float sd = fc.max_std_dev;
   float ax[4];
@@ -126,7 +128,7 @@ __global__ __launch_bounds__(kPreThreads) __attribute__((amdgpu_waves_per_eu(GS3
   if (a.sort_hist) s_hist[threadIdx.x] = 0u;
   if (threadIdx.x == 0) s_qlen = 0u;
 
-  const FrameConsts& fc = *a.fc;
+  const ConstFrame& fc = const_frame(a.fc);
   const uint32_t tile = take_ticket(a.ticket, &s_tile);
   const uint32_t tid = threadIdx.x;
   const uint32_t w = tid / kWave;
@@ -317,7 +319,7 @@ __global__ __launch_bounds__(256) void k_vertex(const FrameConsts* __restrict__ 
                                                 const uint32_t* __restrict__ indirect_args,
                                                 const uint32_t* __restrict__ indices, Splat2D* splats,
                                                 float* splat_depth) {
-  const FrameConsts& fc = *fcp;
+  const ConstFrame& fc = const_frame(fcp);
   const uint32_t v = indirect_args[1];
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < v; s += gridDim.x * blockDim.x) {
     const uint32_t i = indices[s];
@@ -341,7 +343,7 @@ __global__ __launch_bounds__(256) void k_thin_records(const FrameConsts* __restr
                                                       const uint32_t* __restrict__ thin_count,
                                                       const uint32_t* __restrict__ thin_idx, Splat2D* splats,
                                                       uint2* rects, TileGrid rect_grid) {
-  const FrameConsts& fc = *fcp;
+  const ConstFrame& fc = const_frame(fcp);
   const uint32_t n = *thin_count;
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
     const uint32_t i = thin_idx[j];

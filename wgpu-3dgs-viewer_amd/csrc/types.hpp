@@ -107,6 +107,18 @@ struct FrameConsts {
   int32_t no_sh0;
 };
 
+#if defined(__HIPCC__)
+// The frame constants as kernels read them: in the constant address space.
+// An earlier launch writes them and this one only reads them, so the
+// compiler may use scalar loads (their own counter, no vector registers)
+// anywhere in the kernel, also after its stores and atomics; read through a
+// generic pointer, they would be vector loads that count on the same in-order
+// counter as the Gaussian loads requested ahead, so every wait for a matrix
+// would also wait for those.
+using ConstFrame = __attribute__((address_space(4))) const FrameConsts;
+__host__ __device__ inline const ConstFrame& const_frame(const FrameConsts* p) { return *(ConstFrame*)p; }
+#endif
+
 // Per visible Gaussian, the 2D splat the compositor draws (the pixel-space
 // restatement of what vert_main hands the rasterizer, render.wesl:75-130).
 // Indexed by Gaussian index; 32 bytes (one aligned 32-byte HBM sector).
