@@ -578,7 +578,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(DEPTH ? 7 : 
   for (;;) {
     const uint32_t item = take_ticket(a.plan + kPlanTicketWord, &s_item);
     if (item >= items) break;
-    const uint32_t tile = EXACT ? item : a.item_tile[item];
+    // in the work list's order, heavy tiles first: taken in raster order, the
+    // exact blend's densest tiles (one workgroup each) start late and run on
+    // alone after the rest
+    const uint32_t tile = a.item_tile[item];
     const uint32_t seg = EXACT ? 0u : a.item_seg[item];
     const Tiling t = setup_tile(a.grid, tile);
     const uint32_t lx = t.px - uint32_t(t.ox), ly = t.py - uint32_t(t.oy);
