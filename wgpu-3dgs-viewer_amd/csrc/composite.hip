@@ -352,9 +352,13 @@ struct PxStage {
   // per slot: geo and col, 16 bytes each, in two arrays (a 16-byte stride
   // spreads the lanes' reads of random slots over more LDS banks than one
   // 32-byte record per slot)
-  float4 geo[NT];
-  float4 col[NT];
-  float z[Z ? NT : 1];
+  // Slot j of the batch is at index j + 1; index 0 holds a zero record
+  // (alpha 0 in every display mode), written once per workgroup. (col
+  // first: its 16 bytes are read as two 8-byte halves, which cannot carry the
+  // other array's offset; the 16-byte geo read can.)
+  float4 col[NT + 1];
+  float4 geo[NT + 1];
+  float z[Z ? NT + 1 : 1];
   uint32_t colm[kPxMaxCols][kWords];
   uint32_t rowm[kPxMaxRows][kWords];
 };
@@ -364,6 +368,16 @@ struct PxStage {
 // this compiler has no builtin; the compiler schedules it like any other
 // instruction (no inline assembly).
 extern "C" __device__ int gs3d_writelane(int val, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
+
+// Index of the lowest set bit of x, all ones for x = 0: v_ffbl_b32 alone (the
+// compiler's count of trailing zeros is undefined at 0, and its defined form
+// adds a select after the shift that follows it here). An ALU instruction; no
+// memory access.
+__device__ __forceinline__ uint32_t lowest_bit_or_ones(uint32_t x) {
+  uint32_t b;
+  asm("v_ffbl_b32 %0, %1" : "=v"(b) : "v"(x));
+  return b;
+}
 
 // A lane's splat for a batch, requested before the batch before it is
 // blended (the gather's latency then overlaps that blend).
@@ -399,15 +413,15 @@ __device__ __forceinline__ void stage_batch_px(PxStage<NT, Z>& s, const Tiling& 
     const uint32_t ba = clamp_src ? ((clamp01_h2(r.ba) & 0xFFFFu) | (r.ba & 0xFFFF0000u)) : r.ba;
     const float cx = r.cx - t.sx, cy = r.cy - t.sy;
     if (mode == kModePoint) {
-      s.geo[tid] = make_float4(cx, cy, r.ux, r.uy);
-      s.col[tid] = make_float4(0.0f, 0.0f, __uint_as_float(rg), __uint_as_float(ba));
+      s.geo[tid + 1] = make_float4(cx, cy, r.ux, r.uy);
+      s.col[tid + 1] = make_float4(0.0f, 0.0f, __uint_as_float(rg), __uint_as_float(ba));
     } else {
       const float ux = r.ux * kSqrtLog2e, uy = r.uy * kSqrtLog2e;
       const float vx = -r.rho * uy, vy = r.rho * ux;
-      s.geo[tid] = make_float4(ux, uy, vx, vy);
-      s.col[tid] = make_float4(cx * ux + cy * uy, cx * vx + cy * vy, __uint_as_float(rg), __uint_as_float(ba));
+      s.geo[tid + 1] = make_float4(ux, uy, vx, vy);
+      s.col[tid + 1] = make_float4(cx * ux + cy * uy, cx * vx + cy * vy, __uint_as_float(rg), __uint_as_float(ba));
     }
-    if (Z && has_z) s.z[tid] = f.z;
+    if (Z && has_z) s.z[tid + 1] = f.z;
     if (r.hxy != 0u) {
       // pixel x of the tile holds the centre ox + x + 0.5: inside the bound
       // [cx - hx, cx + hx] for x in [ceil(cx - hx - ox - 0.5), floor(cx + hx - ox - 0.5)]
@@ -467,8 +481,6 @@ __device__ __forceinline__ void blend_px(const PxStage<NT, DEPTH>& s, const Tili
   const bool q8 = PLAIN || q8_in;
   const bool track_a = !PLAIN && track_a_in;
   const bool count = !PLAIN && count_in;
-  const char* geo = reinterpret_cast<const char*>(s.geo);
-  const char* col = reinterpret_cast<const char*>(s.col);
   // the words (32 slots each) that hold splats for this pixel
   uint32_t nz = 0;
   if (t.inside && (EXACT || !done)) {
@@ -477,27 +489,30 @@ __device__ __forceinline__ void blend_px(const PxStage<NT, DEPTH>& s, const Tili
   }
   // each lane walks its own bits in slot order, word after word: the wave
   // steps as often as its busiest lane has splats
-  uint32_t cur = 0, base = 0, steps = 0;
+  // base: byte offset of index (first slot of the lane's word) + 1
+  uint32_t cur = 0, base = 16u, steps = 0;
   for (;;) {
     if (cur == 0u && nz != 0u) {
       const uint32_t wi = uint32_t(__builtin_ctz(nz));
       nz &= nz - 1u;
       cur = s.colm[lx][wi] & s.rowm[ly][wi];
-      base = wi * 32u;
+      base = (wi * 32u + 1u) * 16u;
     }
     if (!__any(cur != 0u)) break;
     {
       const bool act = cur != 0u;
-      // a lane with no bit left reads slot `base` (slot 0 of a word with a
-      // splat of this batch, or of the batch): slots fill from 0, so it was
-      // staged and its values are finite, and 0 * finite leaves the colour
-      // unchanged. A slot past the batch's end holds stale data (an Inf or
-      // NaN there would reach the colour through the zero-alpha blend).
-      const uint32_t b = act ? uint32_t(__builtin_ctz(cur)) : 0u;
+      // the lowest set bit, or -1 for a lane with no bit left.
+      // Such a lane reads index 32 wi, the slot just below its last word's
+      // first (staged: slots fill from 0, so its values are finite and
+      // 0 * finite leaves the colour unchanged), or the zero record; its
+      // alpha is masked by act. A slot past the batch's end holds stale data
+      // (an Inf or NaN there would reach the colour through the zero-alpha
+      // blend), and no lane reads one.
+      const uint32_t b = lowest_bit_or_ones(cur);
       cur &= cur - 1u;
-      const uint32_t q = (base + b) * 16u;  // byte offset of the slot
-      const float4 g = *reinterpret_cast<const float4*>(geo + q);
-      const float4 c = *reinterpret_cast<const float4*>(col + q);
+      const uint32_t q = (b << 4) + base;  // byte offset of the slot's index
+      const float4 g = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s.geo) + q);
+      const float4 c = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s.col) + q);
       bool kept;
       float al = fragment_alpha<MODE>(g, c, t, sd2l, ell2l, kept);
       kept = kept && act;
@@ -564,6 +579,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(DEPTH ? 7 : 
   if (*a.pair_count > a.capacity) return;  // truncated lists: the frame is drawn again
   __shared__ PxStage<NT, DEPTH> s;
   __shared__ uint32_t s_item;
+  if (threadIdx.x == 0) {  // the zero record (the first ticket's barrier publishes it)
+    s.col[0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    s.geo[0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (DEPTH) s.z[0] = 0.0f;
+  }
   const FrameConsts& fc = *a.fc;
   const int mode = fc.display_mode;
   const float sd2l = fc.max_std_dev * fc.max_std_dev * kLog2e;
