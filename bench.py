@@ -7,7 +7,7 @@ Sections (all run by default; --sections picks a subset):
            N(-5, 0.7)) stored in the f16 pod (f16 SH / rotation / scale, u8
            opacity), 8 views of 2560x1440 per batch orbiting the scene at
            radius 30, recorded once as a HIP graph over 4 queues and
-           replayed; compositing tile and sort tile autotuned per view.
+           replayed; compositing tile autotuned per view, sort tile per batch.
            One step = one batch.
   plain    the same model and resolution through one Viewer, one frame per
            call: frames/s, per-frame latency p50/p99, per-stage GPU times,

@@ -368,8 +368,9 @@ void gs3d_batch_destroy(gs3d_batch* b);
 /* One view slot (viewer + RGBA8 target) per view. */
 gs3d_status gs3d_batch_set_layout(gs3d_batch* b, const gs3d_view_spec* views, uint32_t count);
 /* Picks, for every view, the fastest compositing tile (8x8, 16x16, 32x8),
- * then the fastest sort tile (8, 12, 16, 20 or 24 keys per thread), timing `reps`
- * frames of each with the given camera pods. */
+ * timing `reps` draws of each with the given camera pods, then the fastest
+ * sort tile (8, 12, 16, 20 or 24 keys per thread, the same for every view),
+ * timing `reps` whole batches of each (the views run concurrently). */
 gs3d_status gs3d_batch_autotune(gs3d_batch* b, const gs3d_camera_pod* pods, int reps);
 /* Renders every view with its camera pod. use_graph: the first call records
  * the whole batch as one HIP graph, later calls replay it. Enqueue only. */

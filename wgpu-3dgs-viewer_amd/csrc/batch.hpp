@@ -71,8 +71,8 @@ class BatchRenderer {
   void set_layout(const std::vector<ViewSpec>& views);
   // Picks, for every view, the fastest compositing tile (8x8, 16x16, 32x8)
   // by timing `reps` draws of each with the given cameras, then the fastest
-  // sort tile (8, 12, 16, 20 or 24 keys per thread, for the depth and tile sorts) by
-  // timing `reps` whole frames of each.
+  // sort tile (8, 12, 16, 20 or 24 keys per thread, for the depth and tile
+  // sorts, the same for every view) by timing `reps` whole batches of each.
   void autotune(const CameraPod* pods, int reps);
   // Renders the batch. With use_graph, the first call records the whole
   // batch (every view's kernels, spread over the queues) as one HIP graph and

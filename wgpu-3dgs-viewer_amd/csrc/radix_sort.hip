@@ -138,12 +138,23 @@ __device__ __forceinline__ void load_tile(const uint32_t* __restrict__ kin, cons
                                           uint32_t base, uint32_t valid, uint32_t (&key)[ITEMS],
                                           uint32_t (&val)[ITEMS]) {
   const uint32_t lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  const uint32_t first = w * (kWave * ITEMS) + lane;  // item 0's slot
+  const uint32_t* kp = kin + base + first;
+  const uint32_t* vp = vin + base + first;
+  if (valid >= uint32_t(kRsThreads * ITEMS)) {
+    // a full tile (every one but the last): no per-item test, one address
 #pragma unroll
-  for (int k = 0; k < ITEMS; ++k) {
-    const uint32_t li = w * (kWave * ITEMS) + uint32_t(k) * kWave + lane;
-    const bool ok = li < valid;
-    key[k] = ok ? kin[base + li] : 0xFFFFFFFFu;
-    val[k] = ok ? vin[base + li] : 0u;
+    for (int k = 0; k < ITEMS; ++k) {
+      key[k] = kp[k * kWave];
+      val[k] = vp[k * kWave];
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      const bool ok = first + uint32_t(k) * kWave < valid;
+      key[k] = ok ? kp[k * kWave] : 0xFFFFFFFFu;
+      val[k] = ok ? vp[k * kWave] : 0u;
+    }
   }
 }
 
