@@ -606,6 +606,19 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(DEPTH ? 7 : 
     const Tiling t = setup_tile(a.grid, tile);
     const uint32_t lx = t.px - uint32_t(t.ox), ly = t.py - uint32_t(t.oy);
     const uint32_t tstart = a.ranges[2 * tile], tend = a.ranges[2 * tile + 1];
+    if (EXACT) {
+      // the longest lists set the kernel's length (one workgroup each): their
+      // waves issue first on a SIMD they share
+      const uint32_t tl = __builtin_amdgcn_readfirstlane(tend - tstart);
+      if (tl > 8192u)
+        __builtin_amdgcn_s_setprio(3);
+      else if (tl > 4096u)
+        __builtin_amdgcn_s_setprio(2);
+      else if (tl > 2048u)
+        __builtin_amdgcn_s_setprio(1);
+      else
+        __builtin_amdgcn_s_setprio(0);
+    }
     uint32_t front = tend, back = tstart;
     if (!EXACT) {
       // segment counted from the front (nearest end) of the tile's list
