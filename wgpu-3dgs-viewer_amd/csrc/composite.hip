@@ -390,13 +390,18 @@ struct PxFetch {
 
 // k is a valid splat index even for a lane without a splat in the batch (the
 // load is then unused): an unconditional load leaves the compiler nothing to
-// merge, so it does not wait for the load until the staging uses it.
+// merge, so it does not wait for the load until the staging uses it. A pair
+// value at or above num (a sort that gave up waiting leaves such entries) is
+// reported and its splat neither read nor drawn.
 template <bool Z>
 __device__ __forceinline__ void fetch_px(PxFetch<Z>& f, const Splat2D* splats, const float* zsrc, bool valid,
-                                         uint32_t k) {
-  f.valid = valid;
-  f.r = splats[k];
-  if (Z && zsrc) f.z = zsrc[k];
+                                         uint32_t k, uint32_t num, uint32_t* err) {
+  const bool ok = k < num;
+  if (valid && !ok) atomicOr(err, kErrSplatIndex);
+  f.valid = valid && ok;
+  const uint32_t kk = ok ? k : 0u;
+  f.r = splats[kk];
+  if (Z && zsrc) f.z = zsrc[kk];
 }
 
 // Stages the fetched splat (or nothing) into slot threadIdx.x and builds the
@@ -667,7 +672,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(DEPTH ? 7 : 
     uint32_t k_next = 0u;
     if (len > 0u) {
       const bool v0 = in_batch(0u);
-      fetch_px(f, a.splats, zsrc, v0, a.pair_vals[v0 ? slot(0u) : back]);
+      fetch_px(f, a.splats, zsrc, v0, a.pair_vals[v0 ? slot(0u) : back], a.num_gaussians, a.err);
       v_next = in_batch(nt);
       k_next = a.pair_vals[v_next ? slot(nt) : back];
     }
@@ -676,7 +681,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(DEPTH ? 7 : 
       __syncthreads();
       stage_batch_px(s, t, a.grid, f, zsrc != nullptr, q8, mode);
       // the next batch's splat, the one after's pair value
-      fetch_px(f, a.splats, zsrc, v_next, k_next);
+      fetch_px(f, a.splats, zsrc, v_next, k_next, a.num_gaussians, a.err);
       v_next = in_batch(p0 + 2u * nt);
       k_next = a.pair_vals[v_next ? slot(p0 + 2u * nt) : back];
       if (mode == kModeSplat)

@@ -138,6 +138,7 @@ struct BinArgs {
   // the tile sort's first-pass histogram (digit 0 of the stored pairs' tile
   // ids), zeroed by k_bin_reset (null: the sort counts for itself)
   uint32_t* sort_hist;
+  uint32_t num_gaussians;  // entries of splats and rects: indices at or above it are reported, not read
 };
 
 struct CompositeArgs {
@@ -145,6 +146,8 @@ struct CompositeArgs {
   const uint32_t* pair_vals;
   const uint32_t* ranges;  // [tiles][2]
   const Splat2D* splats;
+  uint32_t num_gaussians;  // entries of splats and splat_depth: pair values at or above it are reported, not read
+  uint32_t* err;
   TileGrid grid;
   int format;
   int blend;

@@ -75,18 +75,31 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(BinArgs a) {
     const uint32_t s = base + uint32_t(k) * kBinThreads + tid;
     gidx[k] = a.indices[s < v ? s : last];
   }
+  // an index outside the model (a sort that gave up waiting leaves such
+  // entries) is reported, and neither read through nor listed
+  uint32_t bad = 0;
   if (a.rects) {
 #pragma unroll
-    for (int k = 0; k < kBinItems; ++k) rr[k] = a.rects[gidx[k]];  // 8 bytes per instance written by the vertex work
+    for (int k = 0; k < kBinItems; ++k) {  // 8 bytes per instance written by the vertex work
+      const bool b = gidx[k] >= a.num_gaussians;
+      bad |= uint32_t(b) << k;
+      rr[k] = a.rects[b ? 0u : gidx[k]];
+    }
   } else {
     const float sd = a.fc->max_std_dev;
 #pragma unroll
-    for (int k = 0; k < kBinItems; ++k) rr[k] = tile_rect(a.splats[gidx[k]], sd * sd, a.grid);
+    for (int k = 0; k < kBinItems; ++k) {
+      const bool b = gidx[k] >= a.num_gaussians;
+      bad |= uint32_t(b) << k;
+      rr[k] = tile_rect(a.splats[b ? 0u : gidx[k]], sd * sd, a.grid);
+    }
   }
 #pragma unroll
   for (int k = 0; k < kBinItems; ++k) {
     const uint32_t s = base + uint32_t(k) * kBinThreads + tid;
-    cnt[k] = s < v ? rect_tiles(unpack_rect(rr[k])) : 0u;
+    const bool listed = s < v && ((bad >> k) & 1u) == 0u;
+    if (s < v && !listed) atomicOr(a.err, kErrSplatIndex);
+    cnt[k] = listed ? rect_tiles(unpack_rect(rr[k])) : 0u;
   }
   // exclusive scan in (item, thread) order == sorted instance order: a wave
   // scan per item, then one table of the waves' totals (one barrier)

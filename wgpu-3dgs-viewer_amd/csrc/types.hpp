@@ -151,6 +151,7 @@ constexpr uint32_t kValueMask = (1u << 30) - 1u;
 constexpr uint32_t kErrLookbackSpin = 1u;
 constexpr uint32_t kErrTileOverflow = 2u;
 constexpr uint32_t kErrTileKey = 4u;  // a tile pair with a tile id outside the grid
+constexpr uint32_t kErrSplatIndex = 8u;  // a sorted list held a Gaussian index outside the model
 
 // Screen tiles of a render target.
 struct TileGrid {

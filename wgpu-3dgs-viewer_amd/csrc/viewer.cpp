@@ -83,6 +83,7 @@ void Device::forget_texture(const Texture* t) {
 void throw_kernel_errors(uint32_t e) {
   if (e & kErrLookbackSpin) throw Error(GS3D_ERROR_KERNEL, "a kernel exceeded its look-back wait limit");
   if (e & kErrTileKey) throw Error(GS3D_ERROR_KERNEL, "a tile pair held a tile id outside the grid");
+  if (e & kErrSplatIndex) throw Error(GS3D_ERROR_KERNEL, "a sorted list held a Gaussian index outside the model");
 }
 
 void Device::poll() {
@@ -92,7 +93,7 @@ void Device::poll() {
     // the overflow bit is not needed here: every renderer keeps its own
     // largest pair total since it was last asked
     const uint32_t e = take_errors();
-    if (e & (kErrLookbackSpin | kErrTileKey)) {
+    if (e & (kErrLookbackSpin | kErrTileKey | kErrSplatIndex)) {
       units_.clear();
       throw_kernel_errors(e);
     }
@@ -836,6 +837,7 @@ void Renderer::render(Texture& target, const IndirectArgsBuffer& indirect_args, 
   b.indirect_args = indirect_args.dev();
   b.indices = indices_.dev();
   b.splats = splats_.dev();
+  b.num_gaussians = splats_.len();
   b.fc = fc;
   // rectangles written with this frame's splats for this grid (not for a
   // standalone draw, which recomputes the splats)
@@ -883,6 +885,8 @@ void Renderer::render(Texture& target, const IndirectArgsBuffer& indirect_args, 
   ca.pair_vals = pv_[which].as<uint32_t>();
   ca.ranges = ranges_.as<uint32_t>();
   ca.splats = splats_.dev();
+  ca.num_gaussians = splats_.len();
+  ca.err = dev_.err();
   ca.grid = g;
   ca.format = format_;
   // The fast blend tests every fragment against the attachment as it was

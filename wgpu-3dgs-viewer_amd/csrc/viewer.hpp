@@ -280,6 +280,7 @@ class SplatsBuffer {
  public:
   SplatsBuffer(Device& dev, uint32_t n);
   Splat2D* dev() const { return buf_.as<Splat2D>(); }
+  uint32_t len() const { return n_; }  // Gaussians (records, depths, rectangles)
   // Per-Gaussian fragment depth, kept only for depth-tested draws (null
   // until enable_depth).
   float* depth() const { return depth_.as<float>(); }
