@@ -52,6 +52,7 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
 
 __global__ __launch_bounds__(kBinThreads) void k_bin(BinArgs a) {
   __shared__ uint32_t s_wtot[kBinItems][kBinThreads / kWave];  // each wave's count per item
+  __shared__ uint32_t s_total;
   __shared__ uint32_t s_tile;
   __shared__ uint32_t s_base;
   __shared__ uint32_t s_hist[256];  // digit 0 of this block's stored tile ids
@@ -102,7 +103,9 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(BinArgs a) {
     cnt[k] = listed ? rect_tiles(unpack_rect(rr[k])) : 0u;
   }
   // exclusive scan in (item, thread) order == sorted instance order: a wave
-  // scan per item, then one table of the waves' totals (one barrier)
+  // scan per item, then a table of the waves' totals that one wave turns into
+  // offsets (two barriers; each thread then reads its waves' 8 offsets
+  // instead of summing the whole table, which held 32 more registers)
   const uint32_t w = tid / kWave;
 #pragma unroll
   for (int k = 0; k < kBinItems; ++k) {
@@ -111,19 +114,20 @@ __global__ __launch_bounds__(kBinThreads) void k_bin(BinArgs a) {
     if (tid % kWave == kWave - 1) s_wtot[k][w] = inc;
   }
   __syncthreads();
-  uint32_t run = 0;
-#pragma unroll
-  for (int k = 0; k < kBinItems; ++k) {
-    uint32_t before = 0, all = 0;  // waves before this one, all waves
-#pragma unroll
-    for (uint32_t q = 0; q < kBinThreads / kWave; ++q) {
-      const uint32_t c = s_wtot[k][q];
-      before += q < w ? c : 0u;
-      all += c;
-    }
-    excl[k] += run + before;
-    run += all;
+  // the first wave turns the table into offsets: (item, wave) order is the
+  // instances' order
+  constexpr uint32_t kCells = kBinItems * (kBinThreads / kWave);
+  static_assert(kCells <= kWave, "one lane per table cell");
+  if (tid < kWave) {
+    const uint32_t c = tid < kCells ? (&s_wtot[0][0])[tid] : 0u;
+    const uint32_t inc = wave_inclusive_scan(c);
+    if (tid < kCells) (&s_wtot[0][0])[tid] = inc - c;
+    if (tid == kCells - 1) s_total = inc;
   }
+  __syncthreads();
+  const uint32_t run = s_total;
+#pragma unroll
+  for (int k = 0; k < kBinItems; ++k) excl[k] += s_wtot[k][w];
   if (tid < kWave) {
     // the first wave looks back (run is the same in every thread)
     uint32_t e = 0;
