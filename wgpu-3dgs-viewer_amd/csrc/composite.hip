@@ -493,85 +493,109 @@ __device__ __forceinline__ void blend_px(const PxStage<NT, DEPTH>& s, const Tili
     for (int wi = 0; wi < kWords; ++wi) nz |= ((s.colm[lx][wi] & s.rowm[ly][wi]) != 0u ? 1u : 0u) << wi;
   }
   // each lane walks its own bits in slot order, word after word: the wave
-  // steps as often as its busiest lane has splats
+  // steps as often as its busiest lane has splats. The next step's slot is
+  // read from LDS before the current one is blended, so that the read's
+  // latency overlaps the blend (unrolled by two: no copies between them).
   // base: byte offset of index (first slot of the lane's word) + 1
   uint32_t cur = 0, base = 16u, steps = 0;
-  for (;;) {
+  // the lane's next slot: the byte offset of its index, and whether the lane
+  // has one. The lowest set bit is -1 for a lane with no bit left: such a
+  // lane reads index 32 wi, the slot just below its last word's first
+  // (staged: slots fill from 0, so its values are finite and 0 * finite
+  // leaves the colour unchanged), or the zero record; its alpha is masked by
+  // act. A slot past the batch's end holds stale data (an Inf or NaN there
+  // would reach the colour through the zero-alpha blend), and no lane reads
+  // one.
+  auto advance = [&](uint32_t& q, bool& act) {
     if (cur == 0u && nz != 0u) {
       const uint32_t wi = uint32_t(__builtin_ctz(nz));
       nz &= nz - 1u;
       cur = s.colm[lx][wi] & s.rowm[ly][wi];
       base = (wi * 32u + 1u) * 16u;
     }
-    if (!__any(cur != 0u)) break;
-    {
-      const bool act = cur != 0u;
-      // the lowest set bit, or -1 for a lane with no bit left.
-      // Such a lane reads index 32 wi, the slot just below its last word's
-      // first (staged: slots fill from 0, so its values are finite and
-      // 0 * finite leaves the colour unchanged), or the zero record; its
-      // alpha is masked by act. A slot past the batch's end holds stale data
-      // (an Inf or NaN there would reach the colour through the zero-alpha
-      // blend), and no lane reads one.
-      const uint32_t b = lowest_bit_or_ones(cur);
-      cur &= cur - 1u;
-      const uint32_t q = (b << 4) + base;  // byte offset of the slot's index
-      const float4 g = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s.geo) + q);
-      const float4 c = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s.col) + q);
-      bool kept;
-      float al = fragment_alpha<MODE>(g, c, t, sd2l, ell2l, kept);
-      kept = kept && act;
-      const half2v rg = __builtin_bit_cast(half2v, c.z), bo = __builtin_bit_cast(half2v, c.w);
-      if (MODE == kModeSplat) al = kept ? __builtin_fmaf(al, float(bo.y), 0.0f) : 0.0f;  // times the opacity
-      else al = kept ? al : 0.0f;
-      if (DEPTH) {
-        const float zz = s.z[q >> 4];
-        if (EXACT) {
-          // against the attachment as updated by earlier draws
-          const bool pass = kept && depth_pass(dl.cmp, zz, dl.ref);
-          al = pass ? al : 0.0f;
-          if (pass && dl.found) dl.ref = zz;  // found: depth_write
-          kept = pass;
-        } else {
-          const bool pass = depth_pass(dl.cmp, zz, dl.ref);
-          al = pass ? al : 0.0f;
-          if (kept && pass && !dl.found) {
-            dl.first = zz;
-            dl.found = true;
-          }
-        }
-      }
+    act = cur != 0u;
+    const uint32_t b = lowest_bit_or_ones(cur);
+    cur &= cur - 1u;
+    q = (b << 4) + base;
+  };
+  auto geo_at = [&](uint32_t q) { return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s.geo) + q); };
+  auto col_at = [&](uint32_t q) { return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s.col) + q); };
+  // one step: the splat in (g, c, zz) for this lane's pixel; true when the
+  // wave stops (fast blend: every lane done)
+  auto step = [&](const float4& g, const float4& c, bool act, float zz) -> bool {
+    bool kept;
+    float al = fragment_alpha<MODE>(g, c, t, sd2l, ell2l, kept);
+    kept = kept && act;
+    const half2v rg = __builtin_bit_cast(half2v, c.z), bo = __builtin_bit_cast(half2v, c.w);
+    if (MODE == kModeSplat) al = kept ? __builtin_fmaf(al, float(bo.y), 0.0f) : 0.0f;  // times the opacity
+    else al = kept ? al : 0.0f;
+    if (DEPTH) {
       if (EXACT) {
-        if (count) frags += kept ? 1u : 0u;
-        const float om = 1.0f - al;
-        if (q8) {
-          // q' = round(q om + 255 c al), q in 0..255
-          const float a255 = 255.0f * al;
-          cr = __builtin_rintf(__builtin_fmaf(float(rg.x), a255, cr * om));
-          cg = __builtin_rintf(__builtin_fmaf(float(rg.y), a255, cg * om));
-          cb = __builtin_rintf(__builtin_fmaf(float(bo.x), a255, cb * om));
-          if (track_a) ca = __builtin_rintf(__builtin_fmaf(ca, om, a255));
-        } else {
-          cr = __builtin_fmaf(float(rg.x), al, cr * om);
-          cg = __builtin_fmaf(float(rg.y), al, cg * om);
-          cb = __builtin_fmaf(float(bo.x), al, cb * om);
-          if (track_a) ca = __builtin_fmaf(ca, om, al);
-        }
+        // against the attachment as updated by earlier draws
+        const bool pass = kept && depth_pass(dl.cmp, zz, dl.ref);
+        al = pass ? al : 0.0f;
+        if (pass && dl.found) dl.ref = zz;  // found: depth_write
+        kept = pass;
       } else {
-        const float wgt = T * al;
-        cr = __builtin_fmaf(wgt, float(rg.x), cr);
-        cg = __builtin_fmaf(wgt, float(rg.y), cg);
-        cb = __builtin_fmaf(wgt, float(bo.x), cb);
-        T = T - wgt;
-        if ((++steps & 15u) == 0u) {
-          // the wave stops when every lane is done
-          if (__all(done || T < kMinTransmittance)) {
-            done = true;
-            return;
-          }
+        const bool pass = depth_pass(dl.cmp, zz, dl.ref);
+        al = pass ? al : 0.0f;
+        if (kept && pass && !dl.found) {
+          dl.first = zz;
+          dl.found = true;
         }
       }
     }
+    if (EXACT) {
+      if (count) frags += kept ? 1u : 0u;
+      const float om = 1.0f - al;
+      if (q8) {
+        // q' = round(q om + 255 c al), q in 0..255
+        const float a255 = 255.0f * al;
+        cr = __builtin_rintf(__builtin_fmaf(float(rg.x), a255, cr * om));
+        cg = __builtin_rintf(__builtin_fmaf(float(rg.y), a255, cg * om));
+        cb = __builtin_rintf(__builtin_fmaf(float(bo.x), a255, cb * om));
+        if (track_a) ca = __builtin_rintf(__builtin_fmaf(ca, om, a255));
+      } else {
+        cr = __builtin_fmaf(float(rg.x), al, cr * om);
+        cg = __builtin_fmaf(float(rg.y), al, cg * om);
+        cb = __builtin_fmaf(float(bo.x), al, cb * om);
+        if (track_a) ca = __builtin_fmaf(ca, om, al);
+      }
+    } else {
+      const float wgt = T * al;
+      cr = __builtin_fmaf(wgt, float(rg.x), cr);
+      cg = __builtin_fmaf(wgt, float(rg.y), cg);
+      cb = __builtin_fmaf(wgt, float(bo.x), cb);
+      T = T - wgt;
+      if ((++steps & 15u) == 0u) {
+        // the wave stops when every lane is done
+        if (__all(done || T < kMinTransmittance)) {
+          done = true;
+          return true;
+        }
+      }
+    }
+    return false;
+  };
+  // two slots in flight (a, b): each half of the loop reads one slot while
+  // it blends the other
+  uint32_t qa, qb;
+  bool acta, actb;
+  advance(qa, acta);
+  float4 ga = geo_at(qa), ca_ = col_at(qa);
+  float zza = DEPTH ? s.z[qa >> 4] : 0.0f;
+  for (;;) {
+    if (!__any(acta)) break;
+    advance(qb, actb);
+    const float4 gb = geo_at(qb), cb_ = col_at(qb);
+    const float zzb = DEPTH ? s.z[qb >> 4] : 0.0f;
+    if (step(ga, ca_, acta, zza)) return;
+    if (!__any(actb)) break;
+    advance(qa, acta);
+    ga = geo_at(qa);
+    ca_ = col_at(qa);
+    zza = DEPTH ? s.z[qa >> 4] : 0.0f;
+    if (step(gb, cb_, actb, zzb)) return;
   }
   if (!EXACT) done = done || T < kMinTransmittance;
 }
