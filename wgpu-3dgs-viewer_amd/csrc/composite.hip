@@ -363,11 +363,44 @@ struct PxStage {
   uint32_t rowm[kPxMaxRows][kWords];
 };
 
+// Transpose of a 32 x 32 bit matrix held one row per lane, in each half of
+// the wave: afterwards lane j of a half holds bit j of that half's 32 input
+// words (bit i from lane i). Five butterfly steps: at step k, the lanes i and
+// i ^ k swap the k-bit blocks that sit off the diagonal of their 2k x 2k
+// block. Every lane must be active.
 // v_writelane_b32: lane `lane` (wave-uniform) of `old` becomes the
 // wave-uniform `val`. Declared through the LLVM intrinsic's name, for which
 // this compiler has no builtin; the compiler schedules it like any other
 // instruction (no inline assembly).
 extern "C" __device__ int gs3d_writelane(int val, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
+
+template <int ST>
+__device__ __forceinline__ uint32_t transpose_step(uint32_t x, uint32_t lane) {
+  constexpr uint32_t k = 16u >> ST;
+  constexpr uint32_t m = ST == 0 ? 0x0000FFFFu : ST == 1 ? 0x00FF00FFu : ST == 2 ? 0x0F0F0F0Fu : ST == 3 ? 0x33333333u
+                                                                                                   : 0x55555555u;
+  // the word of lane i ^ k (ds_swizzle, bit mode: and 0x1f, xor k)
+  const uint32_t p = uint32_t(__builtin_amdgcn_ds_swizzle(int(x), int(0x1Fu | (k << 10))));
+  // all ones in the upper lane of the pair (bit k of the lane set)
+  const uint32_t up = uint32_t(__builtin_amdgcn_sbfe(int(lane), 4 - ST, 1));
+  // the partner's off-diagonal blocks, moved over this lane's: upper lanes
+  // take p >> k into their low blocks, lower lanes p << k into their high ones
+  const uint32_t sh = ((p >> k) & up) | ((p << k) & ~up);
+  const uint32_t keep = m ^ up;  // lower lanes keep the low blocks, upper the high ones
+  return (x & keep) | (sh & ~keep);
+}
+
+__device__ __forceinline__ uint32_t transpose_half_waves(uint32_t x) {
+  uint32_t lane = threadIdx.x % kWave;
+  // (the lane id made opaque here: the steps' per-lane masks are then
+  // recomputed at each use instead of held in registers across the kernel)
+  asm volatile("" : "+v"(lane));
+  x = transpose_step<0>(x, lane);
+  x = transpose_step<1>(x, lane);
+  x = transpose_step<2>(x, lane);
+  x = transpose_step<3>(x, lane);
+  return transpose_step<4>(x, lane);
+}
 
 // Index of the lowest set bit of x, all ones for x = 0: v_ffbl_b32 alone (the
 // compiler's count of trailing zeros is undefined at 0, and its defined form
@@ -438,31 +471,46 @@ __device__ __forceinline__ void stage_batch_px(PxStage<NT, Z>& s, const Tiling& 
       yb = min(int(g.tile_h) - 1, int(floorf(fy + hy)));
     }
   }
-  // column c's and row r's ballots: words 2 sw and 2 sw + 1 of their
-  // bitsets. Column c's pair is written into lane c, row r's into lane
-  // 32 + r (at most 32 columns and 16 rows), and those lanes store them
-  // together after the loops. A range [a, b] is tested as c - a <= b - a in
-  // unsigned arithmetic (one compare; an empty range starts at 2^30).
-  const bool xe = xa <= xb, ye = ya <= yb;
-  const uint32_t x0 = xe ? uint32_t(xa) : 0x40000000u, xs = xe ? uint32_t(xb - xa) : 0u;
-  const uint32_t y0 = ye ? uint32_t(ya) : 0x40000000u, ys = ye ? uint32_t(yb - ya) : 0u;
-  int w0 = 0, w1 = 0;
-  for (int c = 0; c < int(g.tile_w); ++c) {
-    const uint64_t bal = __builtin_amdgcn_ballot_w64(uint32_t(c) - x0 <= xs);
-    w0 = gs3d_writelane(int(uint32_t(bal)), c, w0);
-    w1 = gs3d_writelane(int(uint32_t(bal >> 32)), c, w1);
-  }
-  for (int r = 0; r < int(g.tile_h); ++r) {
-    const uint64_t bal = __builtin_amdgcn_ballot_w64(uint32_t(r) - y0 <= ys);
-    w0 = gs3d_writelane(int(uint32_t(bal)), 32 + r, w0);
-    w1 = gs3d_writelane(int(uint32_t(bal >> 32)), 32 + r, w1);
-  }
-  if (lane < g.tile_w) {
-    s.colm[lane][2 * sw] = uint32_t(w0);
-    s.colm[lane][2 * sw + 1] = uint32_t(w1);
-  } else if (lane >= 32u && lane < 32u + g.tile_h) {
-    s.rowm[lane - 32u][2 * sw] = uint32_t(w0);
-    s.rowm[lane - 32u][2 * sw + 1] = uint32_t(w1);
+  // this lane's column and row masks (bit c: its pixel bound reaches column
+  // c). The bitsets are these masks transposed (bit l of column c's word 2 sw
+  // + h is bit c of lane 32 h + l's mask): a butterfly transpose within each
+  // half-wave leaves lane 32 h + j holding column (or row) j's word 2 sw + h.
+  const bool xe = xa <= xb, ye = ya <= yb;  // then 0 <= a <= b < 32
+  const uint32_t cm = xe ? (2u << uint32_t(xb)) - (1u << uint32_t(xa)) : 0u;
+  const uint32_t rm = ye ? (2u << uint32_t(yb)) - (1u << uint32_t(ya)) : 0u;
+  const uint32_t j = lane & 31u, h = lane >> 5;
+  if (g.tile_w <= 16u) {
+    // columns in bits 0-15, rows in bits 16-31 of one word
+    const uint32_t wd = transpose_half_waves(cm | (rm << 16));
+    if (j < 16u) {
+      if (j < g.tile_w) s.colm[j][2 * sw + h] = wd;
+    } else if (j - 16u < g.tile_h) {
+      s.rowm[j - 16u][2 * sw + h] = wd;
+    }
+  } else {
+    // wider tiles: one ballot per column and row, gathered into lanes with
+    // v_writelane (column c into lane c, row r into lane 32 + r: at most 32
+    // columns and 8 rows here) and stored together
+    int w0 = 0, w1 = 0;
+    for (int c = 0; c < int(g.tile_w); ++c) {
+      const uint64_t bal = __builtin_amdgcn_ballot_w64(((cm >> c) & 1u) != 0u);
+      w0 = gs3d_writelane(int(uint32_t(bal)), c, w0);
+      w1 = gs3d_writelane(int(uint32_t(bal >> 32)), c, w1);
+    }
+    int v0 = 0, v1 = 0;
+    for (int r = 0; r < int(g.tile_h); ++r) {
+      const uint64_t bal = __builtin_amdgcn_ballot_w64(((rm >> r) & 1u) != 0u);
+      v0 = gs3d_writelane(int(uint32_t(bal)), r, v0);
+      v1 = gs3d_writelane(int(uint32_t(bal >> 32)), r, v1);
+    }
+    if (lane < g.tile_w) {
+      s.colm[lane][2 * sw] = uint32_t(w0);
+      s.colm[lane][2 * sw + 1] = uint32_t(w1);
+    }
+    if (lane < g.tile_h) {
+      s.rowm[lane][2 * sw] = uint32_t(v0);
+      s.rowm[lane][2 * sw + 1] = uint32_t(v1);
+    }
   }
   __syncthreads();
 }
