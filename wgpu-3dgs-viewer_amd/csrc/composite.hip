@@ -358,9 +358,11 @@ struct PxStage {
   // other array's offset; the 16-byte geo read can.)
   float4 col[NT + 1];
   float4 geo[NT + 1];
-  float z[Z ? NT + 1 : 1];
+  // (the bitsets before z: their rows start 16-byte aligned and are read as
+  // 16-byte words)
   uint32_t colm[kPxMaxCols][kWords];
   uint32_t rowm[kPxMaxRows][kWords];
+  float z[Z ? NT + 1 : 1];
 };
 
 // Transpose of a 32 x 32 bit matrix held one row per lane, in each half of
@@ -538,7 +540,17 @@ __device__ __forceinline__ void blend_px(const PxStage<NT, DEPTH>& s, const Tili
   uint32_t nz = 0;
   if (t.inside && (EXACT || !done)) {
 #pragma unroll
-    for (int wi = 0; wi < kWords; ++wi) nz |= ((s.colm[lx][wi] & s.rowm[ly][wi]) != 0u ? 1u : 0u) << wi;
+    for (int wi = 0; wi < kWords; wi += (kWords % 4 == 0 ? 4 : 1)) {
+      if constexpr (kWords % 4 == 0) {
+        // four words per 16-byte read of each row
+        const uint4 c = *reinterpret_cast<const uint4*>(&s.colm[lx][wi]);
+        const uint4 r = *reinterpret_cast<const uint4*>(&s.rowm[ly][wi]);
+        nz |= (min(c.x & r.x, 1u) | (min(c.y & r.y, 1u) << 1) | (min(c.z & r.z, 1u) << 2) | (min(c.w & r.w, 1u) << 3))
+              << wi;
+      } else {
+        nz |= min(s.colm[lx][wi] & s.rowm[ly][wi], 1u) << wi;
+      }
+    }
   }
   // each lane walks its own bits in slot order, word after word: the wave
   // steps as often as its busiest lane has splats. The next step's slot is
