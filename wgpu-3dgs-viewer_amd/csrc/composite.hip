@@ -107,11 +107,11 @@ __device__ __forceinline__ bool depth_pass(int cmp, float z, float d) {
   }
 }
 
-// Fragment alpha for the fast path: 0 for a discarded fragment (blending
-// alpha 0 leaves colour and transmittance unchanged), so the loop has no
-// branches. `kept` tells whether the fragment survives the discard test. In
-// splat mode the opacity is left out (the caller folds it into the
-// transmittance factor, one mixed-precision FMA).
+// Fragment alpha: 0 for a discarded fragment (blending alpha 0 leaves colour
+// and transmittance unchanged), so the loop has no branches. `kept` tells
+// whether the fragment survives the discard test. In splat mode the opacity
+// is left out and so is the zero for a discarded fragment: the caller applies
+// both (its select on `kept` already covers the discard).
 template <int MODE>
 __device__ __forceinline__ float fragment_alpha(float4 g, float4 c, const Tiling& t, float sd2l, float ell2l,
                                                 bool& kept) {
@@ -134,7 +134,7 @@ __device__ __forceinline__ float fragment_alpha(float4 g, float4 c, const Tiling
     al = __builtin_amdgcn_exp2f(-r2);
   }
   kept = r2 <= sd2l;
-  return kept ? al : 0.0f;
+  return MODE == kModeSplat ? al : (kept ? al : 0.0f);
 }
 
 // Depth state of one lane of a depth-tested draw: the attachment's value
