@@ -279,25 +279,33 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_onesweep(SortCount cnt, const
     uint32_t key[ITEMS], val[ITEMS], lrank[ITEMS];
     load_tile<ITEMS>(kin, vin, base, valid, key, val);
     if (hist_next) {
-      // the next pass's histogram: the same keys, whatever their order. Keys
-      // whose next digit is the wave's first key's are counted in a register
-      // and added once per thread: a concentrated digit (the top byte of
-      // depth keys takes a few values) would otherwise serialize the wave's
-      // LDS atomics on one address.
-      const uint32_t dom = __builtin_amdgcn_readfirstlane((key[0] >> (shift + 8)) & 0xFFu);
-      uint32_t own = 0u;
+      // the next pass's histogram: the same keys, whatever their order.
+      if (shift + 8 == 24) {
+        // The top byte (sign and high exponent bits of float keys such as
+        // depths) takes a few values, on which the wave's LDS atomics would
+        // serialize: keys whose top byte is the wave's first key's are
+        // counted in a register and added once per thread.
+        const uint32_t dom = __builtin_amdgcn_readfirstlane(key[0] >> 24);
+        uint32_t own = 0u;
 #pragma unroll
-      for (int k = 0; k < ITEMS; ++k) {
-        const uint32_t li = w * (kWave * ITEMS) + uint32_t(k) * kWave + lane;
-        const uint32_t nd = (key[k] >> (shift + 8)) & 0xFFu;
-        if (li < valid) {
-          if (nd == dom)
-            ++own;
-          else
-            atomicAdd(&s_next[nd], 1u);
+        for (int k = 0; k < ITEMS; ++k) {
+          const uint32_t li = w * (kWave * ITEMS) + uint32_t(k) * kWave + lane;
+          const uint32_t nd = key[k] >> 24;
+          if (li < valid) {
+            if (nd == dom)
+              ++own;
+            else
+              atomicAdd(&s_next[nd], 1u);
+          }
+        }
+        if (own) atomicAdd(&s_next[dom], own);
+      } else {
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+          const uint32_t li = w * (kWave * ITEMS) + uint32_t(k) * kWave + lane;
+          if (li < valid) atomicAdd(&s_next[(key[k] >> (shift + 8)) & 0xFFu], 1u);
         }
       }
-      if (own) atomicAdd(&s_next[dom], own);
     }
     rank_tile(lds, key, shift, lrank);
     __syncthreads();
